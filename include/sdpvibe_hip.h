@@ -1,0 +1,166 @@
+/*
+ * sdpvibe_hip.h -- C ABI of the MI355X (gfx950) multiprecision linear-algebra core.
+ *
+ * Number format (shared by every entry point):
+ *   - a multiword integer of L limbs is stored in limb-plane layout: limb k of entry e lives at
+ *     X[k * plane + e] (int32, little-endian limbs, two's complement in the top limb);
+ *   - exponents are int64; an MPF value is m * 2^e, zero has m = 0 and e = SDPV_EZ.
+ *
+ * Conventions:
+ *   - every pointer is a device pointer unless stated otherwise; the caller owns every buffer;
+ *   - `stream` is a hipStream_t; every kernel is enqueued on it and no entry point synchronises
+ *     with the host, so calls can be recorded into a hipGraph;
+ *   - return value: SDPV_OK, or a negative SDPV_E* code (bad argument, unsupported size, launch
+ *     failure). Numeric overflow of a rounding step is reported through a caller-supplied int32
+ *     device flag instead (sticky: kernels only ever set it to 1).
+ *
+ * Each declaration names the reference interface it replaces (paths relative to the
+ * AntoineTilloy/SDPviBe sources).
+ */
+#ifndef SDPVIBE_HIP_H
+#define SDPVIBE_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SDPV_OK 0
+#define SDPV_EARG (-1)      /* bad argument (null pointer, negative size, bad mode) */
+#define SDPV_ESIZE (-2)     /* size outside what the kernels support (e.g. L too large) */
+#define SDPV_ELAUNCH (-3)   /* kernel launch failed */
+
+#define SDPV_EZ (-(1LL << 40)) /* exponent of zero (sdpvibe/mpf.py:21) */
+
+/* Library version; bumped whenever an entry point changes. */
+int sdpv_version(void);
+
+/* ---------------------------------------------------------------- limb bookkeeping */
+
+/* out = round_half_up(X / 2^s) per entry as Lout limbs (s <= 0: exact left shift).
+ * X: (Lin, B) planes, s: (B,), out: (Lout, B). flag: per_entry ? (B,) int32 of 0/1
+ * : one int32 set to 1 when any entry overflows Lout limbs.
+ * Replaces limbops.shift_round / _shift_round_kernel (sdpvibe/limbops.py:173-185, :32-62). */
+int sdpv_shift_round(const int32_t* X, int64_t Lin, int64_t B, const int64_t* s, int32_t* out,
+                     int64_t Lout, int32_t* flag, int32_t per_entry, void* stream);
+
+/* out = A + sgn * Bm (mod 2^(32 L)), sgn = +1 or -1. (L, B) planes.
+ * Replaces limbops.add_same / _add_kernel (sdpvibe/limbops.py:188-193, :66-81). */
+int sdpv_add_same(const int32_t* A, const int32_t* Bm, int32_t* out, int64_t L, int64_t B,
+                  int32_t sgn, void* stream);
+
+/* out[e] = bit length of v (of ~v when v < 0), v = X[:, e].
+ * Replaces limbops.bitlen_words / _bitlen_kernel (sdpvibe/limbops.py:213-218, :197-210). */
+int sdpv_bitlen(const int32_t* X, int64_t L, int64_t B, int64_t* out, void* stream);
+
+/* Batched aligned add: entry (b, r, c) of A and Bm (each (L, nb, rows, cols) planes) with
+ * column exponents EA[b*cols + c], EB[...]: ex = max(ea, eb) + 1, both operands rounded
+ * (half up) to 2^ex independently, then out = a' + sgn * b' (wraps); EO[b*cols + c] = ex.
+ * Replaces batched.badd(..., 'align') / _badd_align_kernel (sdpvibe/batched.py:56-92). */
+int sdpv_badd_align(const int32_t* A, const int32_t* Bm, const int64_t* EA, const int64_t* EB,
+                    int32_t* out, int64_t* EO, int64_t L, int64_t nb, int64_t rows, int64_t cols,
+                    int32_t sgn, void* stream);
+
+/* ---------------------------------------------------------------- per-entry floats (MPF) */
+
+/* top[e] = e + bitlen(m) (bitlen of ~m for m < 0), or SDPV_EZ when e == SDPV_EZ.
+ * Replaces mpf._top (sdpvibe/mpf.py:348-351). */
+int sdpv_mpf_top(const int32_t* m, const int64_t* e, int64_t L, int64_t n, int64_t* top,
+                 void* stream);
+
+/* Exact product, magnitude rounded half up to NB = 32L-2 bits; B may be one broadcast entry.
+ * Replaces mpf.mul / mpf_mul kernel (sdpvibe/mpf.py:316-323, :58-89). */
+int sdpv_mpf_mul(const int32_t* A, const int64_t* eA, const int32_t* Bm, const int64_t* eB,
+                 int32_t* O, int64_t* eO, int64_t L, int64_t n, int32_t bscalar, void* stream);
+
+/* O = A + sgn * Bm, normalised. Replaces mpf.add / mpf_add (sdpvibe/mpf.py:326-331, :92-124). */
+int sdpv_mpf_add(const int32_t* A, const int64_t* eA, const int32_t* Bm, const int64_t* eB,
+                 int32_t* O, int64_t* eO, int64_t L, int64_t n, int32_t sgn, void* stream);
+
+/* Normalise LI-limb two's-complement integers X (LI, n) with exponents eI to L limbs.
+ * Replaces mpf.normalize / mpf_norm (sdpvibe/mpf.py:307-313, :127-140). */
+int sdpv_mpf_norm(const int32_t* X, const int64_t* eI, int64_t LI, int32_t* O, int64_t* eO,
+                  int64_t L, int64_t n, void* stream);
+
+/* ---------------------------------------------------------------- exact RNS GEMM */
+
+/* Operand geometry shared by the split entry points: entry (b, r, k) of a batch of nb matrices
+ * with `rows` rows lives at b*ldb + r*ldr + k*ldk inside every limb plane (plane stride
+ * `plane`) and inside the exponent array e. */
+
+/* Row alignment exponents: Erow[b*rows + r] = max_k top(b, r, k) - (32 L - 2), or 0 when the
+ * row is all zero, over k in [0, K). Replaces the exponent half of mpf._align_rows
+ * (sdpvibe/mpf.py:390-398). */
+int sdpv_row_exp(const int32_t* X, const int64_t* e, int64_t L, int64_t plane, int64_t nb,
+                 int64_t rows, int64_t K, int64_t ldb, int64_t ldr, int64_t ldk, int64_t* Erow,
+                 void* stream);
+
+/* Residue split. For every entry (b, r, k0 + k), k < kc: the value is first shift-rounded
+ * (as sdpv_shift_round) by s = Erow[b*rows + r] - e[entry] (s = 2^30 when e == SDPV_EZ;
+ * s = 0 for every entry when e == NULL) from L to Lout limbs, then reduced modulo the primes
+ * g0 .. g0+G-1 to the centred residue r = 128*hi + lo (lo in [-64, 63]).
+ * Rh, Rl: (G, nb*rows, kp) int8; columns kc <= k < kp hold residue 0.
+ * Tables (built by the caller for Lout): Thi/Tlo (npad, NBp) int8 with NBp = 32*ceil(4 Lout/32),
+ * C1, C2, mods (n,) int32, minv (n,) double = 1/m.
+ * flag: optional int32 set to 1 when the shifted value overflows Lout limbs.
+ * Replaces RNS.residues / _residue_kernel (sdpvibe/rns.py:196-214, :67-103) fused with the
+ * shift_round of mpf._align_rows / GramAcc.add (sdpvibe/mpf.py:390-398, :574-576). */
+int sdpv_rns_split(const int32_t* X, const int64_t* e, const int64_t* Erow, int64_t L,
+                   int64_t Lout, int64_t plane, int64_t nb, int64_t rows, int64_t ldb,
+                   int64_t ldr, int64_t ldk, int64_t k0, int64_t kc, int64_t kp,
+                   const int8_t* Thi, const int8_t* Tlo, int64_t NBp, const int32_t* C1,
+                   const int32_t* C2, const int32_t* mods, const double* minv, int64_t g0,
+                   int64_t G, int8_t* Rh, int8_t* Rl, int32_t* flag, void* stream);
+
+#define SDPV_GEMM_ACCUMULATE 1 /* acc = (acc + product) mod m instead of acc = product mod m */
+#define SDPV_GEMM_SYRK_LOWER 2 /* B is A (M == N); only tiles on or below the diagonal */
+
+/* Modular products of residues, per prime: for gi < G, b < nb, with a = 128*Ah + Al and
+ * b = 128*Bh + Bl the centred residues,
+ *   acc[(g0+gi)*ld_g + (b*M + i)*N + j] = (sum_k a[gi, b*M+i, k] * b[gi, b*N+j, k]) mod m_(g0+gi)
+ * stored as int16 in [0, m).
+ * Ah/Al: (G, nb*M, kp), Bh/Bl: (G, nb*N, kp) int8 with kp a multiple of 64, kp <= 65536.
+ * kcoef: (n, 3) int32 = (16256, 128, -127) mod m: ra*rb = 16256*HH + 128*SS - 127*LL.
+ * Replaces RNS.accumulate / _gemm_kernel (sdpvibe/rns.py:217-224, :107-137),
+ * _bgemm_kernel (sdpvibe/batched.py:97-127) and _syrk_lower_kernel (sdpvibe/batched.py:131-162). */
+int sdpv_rns_gemm(const int8_t* Ah, const int8_t* Al, const int8_t* Bh, const int8_t* Bl,
+                  int64_t G, int64_t nb, int64_t M, int64_t N, int64_t kp, int16_t* acc,
+                  int64_t ld_g, const int32_t* mods, const double* minv, const int32_t* kcoef,
+                  int64_t g0, int32_t mode, void* stream);
+
+#define SDPV_CRT_WORDS 0     /* W (E/4, B) int32 two's complement + bitlen (B,) */
+#define SDPV_CRT_NORMALIZE 1 /* MPF output: L limbs with NB bits, exponent ein + shift */
+#define SDPV_CRT_ROUND 2     /* fixed-exponent output: round_half_up(value / 2^s) to L limbs */
+
+/* CRT reconstruction of the exact integers held as residues acc[g*ld_g + i*N + j]
+ * (g < n, 0 <= value residue < m_g) for the block of entries (i, j0 + jj), i < rows, jj < w;
+ * entry index q = i*w + jj. Tables: mods, minv (1/m double), inv ((M/m)^-1 mod m), Ct (E, npad)
+ * int8 base-256 signed digits of M/m, Md (E,) digits of M.
+ * Ct is (Epad, npad) and Md (Epad,), Epad = 32*ceil(E/32), zero-padded.
+ * mode SDPV_CRT_WORDS:     W[t*Bq + q] (t < E/4), bitlen[q]              (Bq = rows * w)
+ * mode SDPV_CRT_NORMALIZE: outm[k*plane + o], oute[o], o = i*ldo + jo + jj, with
+ *                          ein = erow[i] + ecol[c], c = (i / Mb)*ldo + jo + jj (NULL arrays = 0)
+ * mode SDPV_CRT_ROUND:     outm[k*plane + o] = round_half_up(value / 2^scol[c]); overflow -> flag
+ * Replaces RNS.crt_block (sdpvibe/rns.py:227-245) = crt_pre + _crt_dot_kernel + crt_carry
+ * (sdpvibe/limbops.py:86-152, :221-239) fused with mpf.normalize / shift_round
+ * (sdpvibe/mpf.py:440-450, sdpvibe/mpmat.py:205-236, sdpvibe/batched.py:200-231). */
+int sdpv_rns_crt(const int16_t* acc, int64_t n, int64_t ld_g, int64_t rows, int64_t N,
+                 int64_t j0, int64_t w, const int32_t* mods, const double* minv,
+                 const int32_t* inv, const int8_t* Ct, const int32_t* Md, int64_t E, int64_t npad,
+                 int32_t mode, int32_t* W, int64_t* bitlen, int32_t* outm, int64_t* oute,
+                 int64_t L, int64_t plane, int64_t ldo, int64_t jo, int64_t Mb, const int64_t* erow,
+                 const int64_t* ecol, const int64_t* scol, int32_t* flag, void* stream);
+
+/* ---------------------------------------------------------------- diagnostics */
+
+/* D = A * B through one wave of v_mfma_i32_32x32x32_i8 using the same fragment mapping as the
+ * GEMM kernels. A: (32, 32) int8 row-major, B: (32, 32) int8 row-major (k, n), D: (32, 32) int32.
+ * No counterpart in the reference: a self-test of the matrix-core operand layout. */
+int sdpv_mfma_selftest(const int8_t* A, const int8_t* B, int32_t* D, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SDPVIBE_HIP_H */

@@ -1,0 +1,436 @@
+/*
+ * oracle.c -- TEST INFRASTRUCTURE ONLY. A CPU restatement of the AntoineTilloy/SDPviBe
+ * multiprecision semantics with GMP mpz integers, used by tests/, the smoke check and the CPU
+ * baseline leg of bench.py as the checker of the HIP library. Nothing in sdpvibe_amd/ links,
+ * loads or calls it.
+ *
+ * Every function works on host arrays in the library's layouts (limb planes: limb k of entry e at
+ * X[k*plane + e], int32 two's complement; exponents int64; zero has e = EZ) and restates one
+ * reference function, cited at its definition. The results are exact integers followed by the
+ * reference's rounding rules; the algorithms (mpz arithmetic, direct CRT) are deliberately
+ * different from the GPU ones (RNS on INT8 matrix cores).
+ */
+#include <gmp.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define EZ (-(1LL << 40))
+
+typedef int64_t i64;
+typedef uint32_t u32;
+
+/* ---------------------------------------------------------------- mpz <-> limbs */
+static void from_words(mpz_t x, const int32_t* X, i64 n, i64 stride) {
+    u32* w = (u32*)malloc(sizeof(u32) * (size_t)n);
+    for (i64 k = 0; k < n; k++) w[k] = (u32)X[k * stride];
+    mpz_import(x, (size_t)n, -1, 4, 0, 0, w);
+    if (w[n - 1] >> 31) {
+        mpz_t t;
+        mpz_init(t);
+        mpz_ui_pow_ui(t, 2, (unsigned long)(32 * n));
+        mpz_sub(x, x, t);
+        mpz_clear(t);
+    }
+    free(w);
+}
+
+/* words 0..n-1 of x mod 2^(32n) */
+static void to_words(const mpz_t x, int32_t* out, i64 n, i64 stride) {
+    mpz_t r;
+    mpz_init(r);
+    mpz_fdiv_r_2exp(r, x, (mp_bitcnt_t)(32 * n));
+    u32* w = (u32*)calloc((size_t)n + 2, sizeof(u32));
+    size_t cnt = 0;
+    mpz_export(w, &cnt, -1, 4, 0, 0, r);
+    for (i64 k = 0; k < n; k++) out[k * stride] = (int32_t)w[k];
+    free(w);
+    mpz_clear(r);
+}
+
+static u32 word_of(const mpz_t x, i64 k) { /* word k of the infinite two's complement of x */
+    mpz_t t;
+    mpz_init(t);
+    mpz_fdiv_q_2exp(t, x, (mp_bitcnt_t)(32 * k));
+    mpz_fdiv_r_2exp(t, t, 32);
+    u32 v = (u32)mpz_get_ui(t);
+    mpz_clear(t);
+    return v;
+}
+
+/* bit length of x, of ~x = -x-1 when x < 0 (limbops.py:142-151, 197-210) */
+static i64 bitlen_tc(const mpz_t x) {
+    if (mpz_sgn(x) >= 0) return mpz_sgn(x) == 0 ? 0 : (i64)mpz_sizeinbase(x, 2);
+    mpz_t t;
+    mpz_init(t);
+    mpz_com(t, x);
+    i64 r = mpz_sgn(t) == 0 ? 0 : (i64)mpz_sizeinbase(t, 2);
+    mpz_clear(t);
+    return r;
+}
+
+static i64 bitlen_abs(const mpz_t x) { return mpz_sgn(x) == 0 ? 0 : (i64)mpz_sizeinbase(x, 2); }
+
+/* y = floor((x + 2^(s-1)) / 2^s) for s > 0 (half toward +inf), x * 2^-s for s <= 0.
+ * |s| is clamped where the result no longer depends on it (see callers). */
+static void round_shift(mpz_t y, const mpz_t x, i64 s, i64 clamp_right, i64 clamp_left) {
+    if (s > 0) {
+        if (s > clamp_right) { /* |x| < 2^(s-1): result 0 */
+            mpz_set_ui(y, 0);
+            return;
+        }
+        mpz_t h;
+        mpz_init(h);
+        mpz_setbit(h, (mp_bitcnt_t)(s - 1));
+        mpz_add(y, x, h);
+        mpz_fdiv_q_2exp(y, y, (mp_bitcnt_t)s);
+        mpz_clear(h);
+    } else {
+        i64 t = -s;
+        if (t > clamp_left) t = clamp_left; /* every kept word is 0 either way */
+        mpz_mul_2exp(y, x, (mp_bitcnt_t)t);
+    }
+}
+
+/* ---------------------------------------------------------------- limbops */
+
+/* limbops.shift_round / _shift_round_kernel (sdpvibe/limbops.py:173-185, :32-62) */
+void orc_shift_round(const int32_t* X, i64 Lin, i64 B, const int64_t* s, int32_t* out, i64 Lout,
+                     int32_t* flag) {
+    #pragma omp parallel
+    {
+        mpz_t x, y;
+        mpz_init(x);
+        mpz_init(y);
+        #pragma omp for schedule(static)
+        for (i64 e = 0; e < B; e++) {
+            from_words(x, X + e, Lin, B);
+            round_shift(y, x, s[e], 32 * Lin + 64, 32 * (Lout + 2));
+            to_words(y, out + e, Lout, B);
+            u32 top = word_of(y, Lout), prev = word_of(y, Lout - 1);
+            u32 want = (prev >> 31) ? 0xffffffffu : 0u;
+            if (flag) flag[e] = top != want;
+        }
+        mpz_clear(x);
+        mpz_clear(y);
+    }
+}
+
+/* limbops.add_same (sdpvibe/limbops.py:188-193, :66-81): A + sgn*B mod 2^(32L) */
+void orc_add_same(const int32_t* A, const int32_t* Bm, int32_t* out, i64 L, i64 B, int32_t sgn) {
+    mpz_t a, b;
+    mpz_init(a);
+    mpz_init(b);
+    for (i64 e = 0; e < B; e++) {
+        from_words(a, A + e, L, B);
+        from_words(b, Bm + e, L, B);
+        if (sgn < 0) mpz_sub(a, a, b); else mpz_add(a, a, b);
+        to_words(a, out + e, L, B);
+    }
+    mpz_clear(a);
+    mpz_clear(b);
+}
+
+/* limbops.bitlen_words (sdpvibe/limbops.py:213-218) */
+void orc_bitlen(const int32_t* X, i64 L, i64 B, int64_t* out) {
+    mpz_t x;
+    mpz_init(x);
+    for (i64 e = 0; e < B; e++) {
+        from_words(x, X + e, L, B);
+        out[e] = bitlen_tc(x);
+    }
+    mpz_clear(x);
+}
+
+/* batched.badd(..., 'align') / _badd_align_kernel (sdpvibe/batched.py:56-92) */
+void orc_badd_align(const int32_t* A, const int32_t* Bm, const int64_t* EA, const int64_t* EB, int32_t* out,
+                    int64_t* EO, i64 L, i64 nb, i64 rows, i64 cols, int32_t sgn) {
+    i64 entries = nb * rows * cols;
+    mpz_t a, b, ra, rb;
+    mpz_init(a); mpz_init(b); mpz_init(ra); mpz_init(rb);
+    for (i64 e = 0; e < entries; e++) {
+        i64 col = (e / (rows * cols)) * cols + e % cols;
+        i64 ex = (EA[col] > EB[col] ? EA[col] : EB[col]) + 1;
+        from_words(a, A + e, L, entries);
+        from_words(b, Bm + e, L, entries);
+        round_shift(ra, a, ex - EA[col], 32 * L + 64, 0);
+        round_shift(rb, b, ex - EB[col], 32 * L + 64, 0);
+        if (sgn < 0) mpz_sub(ra, ra, rb); else mpz_add(ra, ra, rb);
+        to_words(ra, out + e, L, entries);
+        EO[col] = ex;
+    }
+    mpz_clear(a); mpz_clear(b); mpz_clear(ra); mpz_clear(rb);
+}
+
+/* ---------------------------------------------------------------- MPF (sdpvibe/mpf.py) */
+
+/* mpf._top (sdpvibe/mpf.py:348-351) */
+void orc_mpf_top(const int32_t* m, const int64_t* e, i64 L, i64 n, int64_t* top) {
+    mpz_t x;
+    mpz_init(x);
+    for (i64 i = 0; i < n; i++) {
+        if (e[i] == EZ) { top[i] = EZ; continue; }
+        from_words(x, m + i, L, n);
+        top[i] = e[i] + bitlen_tc(x);
+    }
+    mpz_clear(x);
+}
+
+/* Normalise the exact integer v (exponent ein) to L limbs: shift_mag of the magnitude
+ * (half up), sign restored (mpf.py:36-56 with the mpf_norm / mpf_mul tails :84-88, :133-139). */
+static void norm_value(const mpz_t v, i64 ein, i64 L, int32_t* om, i64 stride, int64_t* oe) {
+    if (mpz_sgn(v) == 0) {
+        for (i64 k = 0; k < L; k++) om[k * stride] = 0;
+        *oe = EZ;
+        return;
+    }
+    mpz_t mag, r;
+    mpz_init(mag);
+    mpz_init(r);
+    mpz_abs(mag, v);
+    i64 NB = 32 * L - 2;
+    i64 s = bitlen_abs(mag) - NB;
+    round_shift(r, mag, s, bitlen_abs(mag) + 64, 32 * (L + 2));
+    mpz_fdiv_r_2exp(r, r, (mp_bitcnt_t)(32 * L)); /* words above L dropped (shift_mag nd = L) */
+    if (mpz_sgn(v) < 0) mpz_neg(r, r);
+    to_words(r, om, L, stride);
+    *oe = ein + s;
+    mpz_clear(mag);
+    mpz_clear(r);
+}
+
+/* mpf.normalize / mpf_norm (sdpvibe/mpf.py:307-313, :127-140) */
+void orc_mpf_norm(const int32_t* X, const int64_t* eI, i64 LI, int32_t* O, int64_t* eO, i64 L, i64 n) {
+    mpz_t x;
+    mpz_init(x);
+    for (i64 i = 0; i < n; i++) {
+        from_words(x, X + i, LI, n);
+        norm_value(x, eI[i], L, O + i, n, eO + i);
+    }
+    mpz_clear(x);
+}
+
+/* mpf.mul / mpf_mul (sdpvibe/mpf.py:316-323, :58-89) */
+void orc_mpf_mul(const int32_t* A, const int64_t* eA, const int32_t* Bm, const int64_t* eB, int32_t* O,
+                 int64_t* eO, i64 L, i64 n, int32_t bscalar) {
+    mpz_t a, b;
+    mpz_init(a);
+    mpz_init(b);
+    for (i64 i = 0; i < n; i++) {
+        i64 ib = bscalar ? 0 : i, nb = bscalar ? 1 : n;
+        from_words(a, A + i, L, n);
+        from_words(b, Bm + ib, L, nb);
+        mpz_mul(a, a, b);
+        norm_value(a, eA[i] + eB[ib], L, O + i, n, eO + i);
+    }
+    mpz_clear(a);
+    mpz_clear(b);
+}
+
+/* shift_mag(src, ns, s, dst, nd) of a magnitude, as an integer mod 2^(32 nd) (mpf.py:36-56) */
+static void shift_mag_z(mpz_t y, const mpz_t mag, i64 s, i64 nd) {
+    round_shift(y, mag, s, bitlen_abs(mag) + 64, 32 * (nd + 2));
+    mpz_fdiv_r_2exp(y, y, (mp_bitcnt_t)(32 * nd));
+}
+
+/* mpf.add / mpf_add (sdpvibe/mpf.py:326-331, :92-124) */
+void orc_mpf_add(const int32_t* A, const int64_t* eA, const int32_t* Bm, const int64_t* eB, int32_t* O,
+                 int64_t* eO, i64 L, i64 n, int32_t sgn) {
+    mpz_t a, b, wa, wb, mod;
+    mpz_init(a); mpz_init(b); mpz_init(wa); mpz_init(wb); mpz_init(mod);
+    i64 W = L + 2;
+    mpz_ui_pow_ui(mod, 2, (unsigned long)(32 * W));
+    for (i64 i = 0; i < n; i++) {
+        from_words(a, A + i, L, n);
+        from_words(b, Bm + i, L, n);
+        int na = mpz_sgn(a) < 0, nbg = (mpz_sgn(b) < 0) ^ (sgn < 0);
+        mpz_abs(a, a);
+        mpz_abs(b, b);
+        i64 ea = eA[i], eb = eB[i];
+        if (mpz_sgn(a) == 0) ea = EZ;
+        if (mpz_sgn(b) == 0) eb = EZ;
+        if (ea == EZ && eb == EZ) {
+            for (i64 k = 0; k < L; k++) O[k * n + i] = 0;
+            eO[i] = EZ;
+            continue;
+        }
+        i64 t = (ea > eb ? ea : eb) - 32;
+        i64 da = ea - t, db = eb - t, cl = 32 * (L + 3);
+        i64 sa = da < -cl ? cl : -da, sb = db < -cl ? cl : -db;
+        if (ea == EZ) mpz_set_ui(wa, 0); else shift_mag_z(wa, a, sa, W);
+        if (eb == EZ) mpz_set_ui(wb, 0); else shift_mag_z(wb, b, sb, W);
+        if (na) mpz_neg(wa, wa);
+        if (nbg) mpz_neg(wb, wb);
+        mpz_add(wa, wa, wb);
+        mpz_fdiv_r(wa, wa, mod); /* W-word two's complement */
+        if (mpz_tstbit(wa, (mp_bitcnt_t)(32 * W - 1))) mpz_sub(wa, wa, mod);
+        norm_value(wa, t, L, O + i, n, eO + i);
+    }
+    mpz_clear(a); mpz_clear(b); mpz_clear(wa); mpz_clear(wb); mpz_clear(mod);
+}
+
+/* ---------------------------------------------------------------- RNS (sdpvibe/rns.py) */
+
+/* RNS.residues (sdpvibe/rns.py:196-214): centred x mod m of the entries X[:, r, k0 + k] of an
+ * (L, rows, K) limb-plane matrix; out (G, rows, kc) int32 = r in [-(m-1)/2, (m-1)/2]. */
+void orc_residues(const int32_t* X, i64 L, i64 rows, i64 K, i64 k0, i64 kc, const int32_t* mods, i64 G,
+                  int32_t* out) {
+    #pragma omp parallel
+    {
+        mpz_t x;
+        mpz_init(x);
+        #pragma omp for schedule(static)
+        for (i64 r = 0; r < rows; r++)
+            for (i64 k = 0; k < kc; k++) {
+                from_words(x, X + r * K + k0 + k, L, rows * K);
+                for (i64 g = 0; g < G; g++) {
+                    long m = mods[g];
+                    long v = (long)mpz_fdiv_ui(x, (unsigned long)m);
+                    if (v > m / 2) v -= m;
+                    out[(g * rows + r) * kc + k] = (int32_t)v;
+                }
+            }
+        mpz_clear(x);
+    }
+}
+
+/* RNS.crt_block (sdpvibe/rns.py:227-245): the integer in (-M/2, M/2] congruent to acc[g*B + q]
+ * modulo every m_g, as nw words W[t*B + q] and its bit length (of ~v for v < 0). Direct CRT
+ * with mpz (sum of r_g * (M/m_g) * ((M/m_g)^-1 mod m_g), reduced mod M, centred). */
+void orc_crt(const int16_t* acc, const int32_t* mods, i64 n, i64 B, int32_t* W, i64 nw, int64_t* bitlen) {
+    mpz_t M, Mi, c, v, half;
+    mpz_init_set_ui(M, 1);
+    mpz_init(Mi); mpz_init(c); mpz_init(v); mpz_init(half);
+    for (i64 g = 0; g < n; g++) mpz_mul_ui(M, M, (unsigned long)mods[g]);
+    mpz_t* coef = (mpz_t*)malloc(sizeof(mpz_t) * (size_t)n);
+    for (i64 g = 0; g < n; g++) {
+        mpz_init(coef[g]);
+        mpz_divexact_ui(Mi, M, (unsigned long)mods[g]);
+        mpz_set_ui(c, (unsigned long)mods[g]);
+        mpz_invert(coef[g], Mi, c);
+        mpz_mul(coef[g], coef[g], Mi);
+    }
+    mpz_fdiv_q_2exp(half, M, 1);
+    for (i64 q = 0; q < B; q++) {
+        mpz_set_ui(v, 0);
+        for (i64 g = 0; g < n; g++) {
+            long r = acc[g * B + q] % mods[g];
+            if (r < 0) r += mods[g];
+            mpz_addmul_ui(v, coef[g], (unsigned long)r);
+        }
+        mpz_fdiv_r(v, v, M);
+        if (mpz_cmp(v, half) > 0) mpz_sub(v, v, M);
+        to_words(v, W + q, nw, B);
+        if (bitlen) bitlen[q] = bitlen_tc(v);
+    }
+    for (i64 g = 0; g < n; g++) mpz_clear(coef[g]);
+    free(coef);
+    mpz_clear(M); mpz_clear(Mi); mpz_clear(c); mpz_clear(v); mpz_clear(half);
+}
+
+/* ---------------------------------------------------------------- GEMM */
+
+/* Operand geometry: entry (b, r, k) at b*ldb + r*ldr + k*ldk of every limb plane and of e. */
+typedef struct { i64 nb, rows, K, ldb, ldr, ldk, plane; } geom_t;
+
+/* mpf._align_rows (sdpvibe/mpf.py:390-398): per row, E = max top - NB (0 for a zero row) and the
+ * entries shift-rounded to L limbs at 2^E, returned as exact integers a[(b*rows + r)*K + k]. */
+static void align_rows(const int32_t* m, const int64_t* e, i64 L, geom_t g, mpz_t* a, int64_t* Erow) {
+    i64 NB = 32 * L - 2;
+    #pragma omp parallel
+    {
+        mpz_t x, y;
+        mpz_init(x);
+        mpz_init(y);
+        int32_t* tmp = (int32_t*)malloc(sizeof(int32_t) * (size_t)L);
+        #pragma omp for schedule(static)
+        for (i64 R = 0; R < g.nb * g.rows; R++) {
+            i64 b = R / g.rows, r = R % g.rows, base = b * g.ldb + r * g.ldr;
+            i64 mx = EZ;
+            for (i64 k = 0; k < g.K; k++) {
+                i64 off = base + k * g.ldk;
+                if (e[off] == EZ) continue;
+                from_words(x, m + off, L, g.plane);
+                i64 t = e[off] + bitlen_tc(x);
+                if (t > mx) mx = t;
+            }
+            i64 E = mx - NB;
+            if (E < EZ / 2) E = 0;
+            Erow[R] = E;
+            for (i64 k = 0; k < g.K; k++) {
+                i64 off = base + k * g.ldk;
+                i64 s = (e[off] == EZ) ? (1LL << 30) : E - e[off];
+                from_words(x, m + off, L, g.plane);
+                round_shift(y, x, s, 32 * L + 64, 32 * (L + 2));
+                /* wrap to L words (two's complement), as the L-limb shift_round output */
+                to_words(y, tmp, L, 1);
+                from_words(a[R * g.K + k], tmp, L, 1);
+            }
+        }
+        free(tmp);
+        mpz_clear(x);
+        mpz_clear(y);
+    }
+}
+
+/* mpf.gemm (sdpvibe/mpf.py:401-452): C[b] = op(A)[b] op(B)[b], rows of op(A) and columns of op(B)
+ * aligned (first rounding), exact products, then normalised per entry (second rounding).
+ * ga describes op(A) (rows M), gb describes op(B)^T (rows N); out (L, nb, M, N), oe (nb, M, N). */
+void orc_mpf_gemm(const int32_t* Am, const int64_t* Ae, i64 L, i64 nb, i64 M, i64 N, i64 K, i64 aldb, i64 aldr,
+                  i64 aldk, i64 aplane, const int32_t* Bm, const int64_t* Be, i64 bldb, i64 bldr, i64 bldk,
+                  i64 bplane, int32_t* out, int64_t* oe) {
+    geom_t ga = {nb, M, K, aldb, aldr, aldk, aplane}, gb = {nb, N, K, bldb, bldr, bldk, bplane};
+    mpz_t* a = (mpz_t*)malloc(sizeof(mpz_t) * (size_t)(nb * M * K));
+    mpz_t* b = (mpz_t*)malloc(sizeof(mpz_t) * (size_t)(nb * N * K));
+    for (i64 i = 0; i < nb * M * K; i++) mpz_init(a[i]);
+    for (i64 i = 0; i < nb * N * K; i++) mpz_init(b[i]);
+    int64_t* EA = (int64_t*)malloc(sizeof(int64_t) * (size_t)(nb * M));
+    int64_t* EB = (int64_t*)malloc(sizeof(int64_t) * (size_t)(nb * N));
+    align_rows(Am, Ae, L, ga, a, EA);
+    align_rows(Bm, Be, L, gb, b, EB);
+    i64 plane = nb * M * N;
+    #pragma omp parallel
+    {
+        mpz_t c;
+        mpz_init(c);
+        #pragma omp for schedule(dynamic, 16)
+        for (i64 q = 0; q < nb * M * N; q++) {
+            i64 bb = q / (M * N), i = (q / N) % M, j = q % N;
+            mpz_set_ui(c, 0);
+            const mpz_t* ar = a + (bb * M + i) * K;
+            const mpz_t* br = b + (bb * N + j) * K;
+            for (i64 k = 0; k < K; k++) mpz_addmul(c, ar[k], br[k]);
+            norm_value(c, EA[bb * M + i] + EB[bb * N + j], L, out + q, plane, oe + q);
+        }
+        mpz_clear(c);
+    }
+    for (i64 i = 0; i < nb * M * K; i++) mpz_clear(a[i]);
+    for (i64 i = 0; i < nb * N * K; i++) mpz_clear(b[i]);
+    free(a); free(b); free(EA); free(EB);
+}
+
+/* Exact product of fixed-point limb matrices: C = A Bt^T with A (M, K), Bt (N, K) limb planes
+ * (L limbs), as nw words per entry (W[t*M*N + i*N + j]); the exact core of mpmat.gemm_nt
+ * (sdpvibe/mpmat.py:169-202) before its rounding. */
+void orc_gemm_exact(const int32_t* A, const int32_t* Bt, i64 L, i64 M, i64 N, i64 K, int32_t* W, i64 nw) {
+    mpz_t* a = (mpz_t*)malloc(sizeof(mpz_t) * (size_t)(M * K));
+    mpz_t* b = (mpz_t*)malloc(sizeof(mpz_t) * (size_t)(N * K));
+    for (i64 i = 0; i < M * K; i++) { mpz_init(a[i]); from_words(a[i], A + i, L, M * K); }
+    for (i64 i = 0; i < N * K; i++) { mpz_init(b[i]); from_words(b[i], Bt + i, L, N * K); }
+    #pragma omp parallel
+    {
+        mpz_t c;
+        mpz_init(c);
+        #pragma omp for schedule(dynamic, 16)
+        for (i64 q = 0; q < M * N; q++) {
+            i64 i = q / N, j = q % N;
+            mpz_set_ui(c, 0);
+            for (i64 k = 0; k < K; k++) mpz_addmul(c, a[i * K + k], b[j * K + k]);
+            to_words(c, W + q, nw, M * N);
+        }
+        mpz_clear(c);
+    }
+    for (i64 i = 0; i < M * K; i++) mpz_clear(a[i]);
+    for (i64 i = 0; i < N * K; i++) mpz_clear(b[i]);
+    free(a); free(b);
+}

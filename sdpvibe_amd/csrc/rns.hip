@@ -1,0 +1,596 @@
+// Exact multiprecision GEMM on CDNA4 INT8 matrix cores: residue split, modular products, CRT.
+//
+// Residues are centred, |r| <= (m-1)/2 <= 8128 (primes m <= 16257), and split r = 128*h + l with
+// l in [-64, 63], h in [-64, 64]; then s = h + l also fits int8, so a residue product needs three
+// INT8 products instead of four:  ra*rb = 16256*HH + 128*SS - 127*LL  (HH = ha.hb, SS = sa.sb,
+// LL = la.lb).  All matrix work runs on v_mfma_i32_32x32x32_i8 (wave64, 16 bytes per lane).
+//
+// MFMA fragment mapping used throughout (32x32x32, one wave):
+//   A operand: lane l supplies row (l % 32), bytes [16*(l/32), +16) of the 32-byte k slice;
+//   B operand: lane l supplies column (l % 32), the same k bytes;
+//   D:         lane l, register j holds (row 8*(j/4) + 4*(l/32) + j%4, column l % 32).
+// A and B use the same k-slot assignment, so the dot product is exact whatever order the
+// hardware pairs the bytes in.
+#include "common.hpp"
+
+namespace {
+
+struct PlaneSrc {
+    const i32* p;
+    i64 stride;
+    __device__ __forceinline__ u32 operator()(int k) const { return (u32)p[(i64)k * stride]; }
+};
+
+struct LdsWords {
+    const u32* p;
+    __device__ __forceinline__ u32 operator()(int k) const { return p[k]; }
+};
+
+__device__ __forceinline__ v16i zero16() {
+    v16i z;
+#pragma unroll
+    for (int j = 0; j < 16; j++) z[j] = 0;
+    return z;
+}
+__device__ __forceinline__ v4i zero4() {
+    v4i z;
+#pragma unroll
+    for (int j = 0; j < 4; j++) z[j] = 0;
+    return z;
+}
+__device__ __forceinline__ v16i mfma(v4i a, v4i b, v16i c) {
+    return __builtin_amdgcn_mfma_i32_32x32x32_i8(a, b, c, 0, 0, 0);
+}
+
+// ================================================================== residue split
+// One workgroup: EPW entries (TR rows x TK consecutive k). Phase 1 shift-rounds each entry into
+// its Lout limbs and stores the limb bytes (as u - 128) in LDS; phase 2 computes, per group of 32
+// primes, the two byte dot products T_hi . x, T_lo . x on MFMA (primes = rows, entries = columns),
+// reduces modulo m, splits, stages the int8 planes in LDS and writes 16-byte chunks.
+__global__ void __launch_bounds__(256) k_rns_split(
+    const i32* __restrict__ X, const i64* __restrict__ e, const i64* __restrict__ Erow, int L, int Lout, i64 plane,
+    i64 nb, i64 rows, i64 ldb, i64 ldr, i64 ldk, i64 k0, i64 kc, i64 kp, const int8_t* __restrict__ Thi,
+    const int8_t* __restrict__ Tlo, int NBp, const i32* __restrict__ C1, const i32* __restrict__ C2,
+    const i32* __restrict__ mods, const double* __restrict__ minv, int g0, int G, int8_t* __restrict__ Rh,
+    int8_t* __restrict__ Rl, i32* __restrict__ flag, int EPW, int TK, i64 tiles_k) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int XS = NBp + 16;
+    unsigned char* xs = smem;                           // EPW x XS bytes
+    unsigned char* negs = xs + (size_t)EPW * XS;        // EPW sign flags
+    int8_t* stg = (int8_t*)(negs + EPW);                // 2 planes x 32 primes x EPW
+    const int t = threadIdx.x;
+    const int TR = EPW / TK;
+    const i64 tr = blockIdx.x / tiles_k, tk = blockIdx.x % tiles_k;
+    const i64 nrows = nb * rows;
+
+    if (t < EPW) {
+        const i64 R = tr * TR + t / TK, k = tk * TK + t % TK;
+        u32* row = (u32*)(xs + (size_t)t * XS);
+        int neg = 0;
+        if (R < nrows && k < kc) {
+            const i64 b = R / rows, r = R % rows;
+            const i64 off = b * ldb + r * ldr + (k0 + k) * ldk;
+            i64 s = 0;
+            if (e) {
+                const i64 ex = e[off];
+                s = (ex == EZV) ? (i64)(1 << 30) : Erow[R] - ex;
+            }
+            PlaneSrc src{X + off, plane};
+            const int bad = shr_round(src, L, s, Lout, [&](int w, u32 v) {
+                row[w] = v ^ 0x80808080u;
+                if (w == Lout - 1) neg = (int)(v >> 31);
+            });
+            if (bad && flag) atomicOr(flag, 1);
+        } else {
+            for (int w = 0; w < Lout; w++) row[w] = 0u;
+        }
+        for (int w = Lout; w < NBp / 4; w++) row[w] = 0u;
+        negs[t] = (unsigned char)neg;
+    }
+    __syncthreads();
+
+    const int wave = t >> 6, lane = t & 63, l32 = lane & 31, hsel = lane >> 5;
+    const int nEB = EPW / 32, nPB = (G + 31) / 32, KS = NBp / 32;
+    for (int pb = 0; pb < nPB; pb++) {
+        for (int eb = wave; eb < nEB; eb += 4) {
+            v16i ah = zero16(), al = zero16();
+            const int prow = pb * 32 + l32;
+            const bool pok = prow < G;
+            const int8_t* th = Thi + (i64)(g0 + (pok ? prow : 0)) * NBp;
+            const int8_t* tl = Tlo + (i64)(g0 + (pok ? prow : 0)) * NBp;
+            const unsigned char* xrow = xs + (size_t)(eb * 32 + l32) * XS;
+            for (int ks = 0; ks < KS; ks++) {
+                const int boff = ks * 32 + 16 * hsel;
+                const v4i fah = pok ? *(const v4i*)(th + boff) : zero4();
+                const v4i fal = pok ? *(const v4i*)(tl + boff) : zero4();
+                const v4i fb = *(const v4i*)(xrow + boff);
+                ah = mfma(fah, fb, ah);
+                al = mfma(fal, fb, al);
+            }
+            const int c = eb * 32 + l32;
+            const int cneg = negs[c];
+            const i64 R = tr * TR + c / TK, k = tk * TK + c % TK;
+            const bool cvalid = (R < nrows) && (k < kc);
+#pragma unroll
+            for (int j = 0; j < 16; j++) {
+                const int pr = 8 * (j >> 2) + 4 * hsel + (j & 3);
+                const int gi = pb * 32 + pr;
+                int hi = 0, lo = 0;
+                if (gi < G && cvalid) {
+                    const int g = g0 + gi;
+                    const int m = mods[g];
+                    const int x = ah[j] * 128 + al[j] + C1[g] - (cneg ? C2[g] : 0);
+                    split_centred(mod_m(x, m, minv[g]), m, hi, lo);
+                }
+                stg[(size_t)pr * EPW + c] = (int8_t)hi;
+                stg[(size_t)(32 + pr) * EPW + c] = (int8_t)lo;
+            }
+        }
+        __syncthreads();
+        const int npr = (G - pb * 32) < 32 ? (G - pb * 32) : 32;
+        const int cpr = TK / 16;
+        const int total = 2 * npr * TR * cpr;
+        for (int idx = t; idx < total; idx += 256) {
+            const int cc = idx % cpr;
+            int rest = idx / cpr;
+            const int rr = rest % TR;
+            rest /= TR;
+            const int pr = rest % npr, pl = rest / npr;
+            const i64 R = tr * TR + rr, kb = tk * TK + cc * 16;
+            if (R < nrows && kb < kp) {
+                const v4i val = *(const v4i*)(stg + (size_t)(pl * 32 + pr) * EPW + rr * TK + cc * 16);
+                int8_t* dst = (pl ? Rl : Rh) + ((i64)(pb * 32 + pr) * nrows + R) * kp + kb;
+                *(v4i*)dst = val;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// ================================================================== modular products
+// WM x WN waves, each BLM x BLN blocks of 32x32; BK = 64 bytes per stage, double-buffered LDS
+// holding the h, l and s = h + l planes of both operands (row stride 80 B: conflict-free b128).
+template <int WM, int WN, int BLM, int BLN>
+__global__ void __launch_bounds__(WM* WN * 64) k_rns_gemm(
+    const int8_t* __restrict__ Ah, const int8_t* __restrict__ Al, const int8_t* __restrict__ Bh,
+    const int8_t* __restrict__ Bl, int G, int nb, i64 M, i64 N, i64 kp, int16_t* __restrict__ acc, i64 ld_g,
+    const i32* __restrict__ mods, const double* __restrict__ minv, const i32* __restrict__ kcoef, int g0, int mode,
+    i64 tiles_m, i64 tiles_n) {
+    constexpr int NT = WM * WN * 64;
+    constexpr int BM = WM * BLM * 32, BN = WN * BLN * 32;
+    constexpr int RS = 80;
+    constexpr int ASZ = BM * RS, BSZ = BN * RS;
+    extern __shared__ __align__(16) int8_t smem8[];
+    // buffer layout: [stage][plane h,l,s][A tile | B tile]
+    auto sA = [&](int st, int p) { return smem8 + (size_t)(st * 3 + p) * (ASZ + BSZ); };
+    auto sB = [&](int st, int p) { return smem8 + (size_t)(st * 3 + p) * (ASZ + BSZ) + ASZ; };
+
+    const i64 total = tiles_m * tiles_n * (i64)G * nb;
+    const i64 lg = xcd_remap(blockIdx.x, total);
+    const i64 tn = lg % tiles_n;
+    i64 rest = lg / tiles_n;
+    const i64 tm = rest % tiles_m;
+    rest /= tiles_m;
+    const int gi = (int)(rest % G);
+    const i64 b = rest / G;
+    const bool syrk = (mode & SDPV_GEMM_SYRK_LOWER) != 0;
+    if (syrk && tn > tm) return;
+
+    const int t = threadIdx.x, wave = t / 64, lane = t & 63, l32 = lane & 31, hsel = lane >> 5;
+    const int wm = wave / WN, wn = wave % WN;
+    const i64 rowsA = (i64)nb * M, rowsB = (i64)nb * N;
+    const int8_t* pAh = Ah + ((i64)gi * rowsA + b * M) * kp;
+    const int8_t* pAl = Al + ((i64)gi * rowsA + b * M) * kp;
+    const int8_t* pBh = Bh + ((i64)gi * rowsB + b * N) * kp;
+    const int8_t* pBl = Bl + ((i64)gi * rowsB + b * N) * kp;
+    const i64 r0 = tm * BM, c0 = tn * BN;
+
+    constexpr int PA = BM * 4, PB = BN * 4;  // (row, 16-byte chunk) pairs per stage
+    constexpr int NA = (PA + NT - 1) / NT, NBr = (PB + NT - 1) / NT;
+    v4i ra_h[NA], ra_l[NA], rb_h[NBr], rb_l[NBr];
+
+    auto gload = [&](i64 kk) {
+#pragma unroll
+        for (int u = 0; u < NA; u++) {
+            const int idx = t + u * NT;
+            const int row = idx >> 2, ch = idx & 3;
+            const i64 gr = r0 + row;
+            if (idx < PA && gr < M) {
+                ra_h[u] = *(const v4i*)(pAh + gr * kp + kk + ch * 16);
+                ra_l[u] = *(const v4i*)(pAl + gr * kp + kk + ch * 16);
+            } else {
+                ra_h[u] = zero4();
+                ra_l[u] = zero4();
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < NBr; u++) {
+            const int idx = t + u * NT;
+            const int row = idx >> 2, ch = idx & 3;
+            const i64 gc = c0 + row;
+            if (idx < PB && gc < N) {
+                rb_h[u] = *(const v4i*)(pBh + gc * kp + kk + ch * 16);
+                rb_l[u] = *(const v4i*)(pBl + gc * kp + kk + ch * 16);
+            } else {
+                rb_h[u] = zero4();
+                rb_l[u] = zero4();
+            }
+        }
+    };
+    auto lstore = [&](int st) {
+#pragma unroll
+        for (int u = 0; u < NA; u++) {
+            const int idx = t + u * NT;
+            if (idx < PA) {
+                const int row = idx >> 2, ch = idx & 3;
+                v4i s;
+#pragma unroll
+                for (int q = 0; q < 4; q++) s[q] = (int)add_bytes((u32)ra_h[u][q], (u32)ra_l[u][q]);
+                *(v4i*)(sA(st, 0) + row * RS + ch * 16) = ra_h[u];
+                *(v4i*)(sA(st, 1) + row * RS + ch * 16) = ra_l[u];
+                *(v4i*)(sA(st, 2) + row * RS + ch * 16) = s;
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < NBr; u++) {
+            const int idx = t + u * NT;
+            if (idx < PB) {
+                const int row = idx >> 2, ch = idx & 3;
+                v4i s;
+#pragma unroll
+                for (int q = 0; q < 4; q++) s[q] = (int)add_bytes((u32)rb_h[u][q], (u32)rb_l[u][q]);
+                *(v4i*)(sB(st, 0) + row * RS + ch * 16) = rb_h[u];
+                *(v4i*)(sB(st, 1) + row * RS + ch * 16) = rb_l[u];
+                *(v4i*)(sB(st, 2) + row * RS + ch * 16) = s;
+            }
+        }
+    };
+
+    v16i cHH[BLM][BLN], cSS[BLM][BLN], cLL[BLM][BLN];
+#pragma unroll
+    for (int i = 0; i < BLM; i++)
+#pragma unroll
+        for (int j = 0; j < BLN; j++) {
+            cHH[i][j] = zero16();
+            cSS[i][j] = zero16();
+            cLL[i][j] = zero16();
+        }
+
+    const i64 nst = kp / 64;
+    gload(0);
+    lstore(0);
+    __syncthreads();
+    for (i64 st = 0; st < nst; st++) {
+        const int cur = (int)(st & 1);
+        if (st + 1 < nst) gload((st + 1) * 64);
+#pragma unroll
+        for (int ks = 0; ks < 2; ks++) {
+            const int koff = ks * 32 + 16 * hsel;
+            v4i fa[BLM][3], fb[BLN][3];
+#pragma unroll
+            for (int i = 0; i < BLM; i++) {
+                const int row = (wm * BLM + i) * 32 + l32;
+#pragma unroll
+                for (int p = 0; p < 3; p++) fa[i][p] = *(const v4i*)(sA(cur, p) + row * RS + koff);
+            }
+#pragma unroll
+            for (int j = 0; j < BLN; j++) {
+                const int col = (wn * BLN + j) * 32 + l32;
+#pragma unroll
+                for (int p = 0; p < 3; p++) fb[j][p] = *(const v4i*)(sB(cur, p) + col * RS + koff);
+            }
+#pragma unroll
+            for (int i = 0; i < BLM; i++)
+#pragma unroll
+                for (int j = 0; j < BLN; j++) {
+                    cHH[i][j] = mfma(fa[i][0], fb[j][0], cHH[i][j]);
+                    cLL[i][j] = mfma(fa[i][1], fb[j][1], cLL[i][j]);
+                    cSS[i][j] = mfma(fa[i][2], fb[j][2], cSS[i][j]);
+                }
+        }
+        if (st + 1 < nst) lstore(cur ^ 1);
+        __syncthreads();
+    }
+
+    const int g = g0 + gi;
+    const int m = mods[g];
+    const double mi = minv[g];
+    const int k0c = kcoef[3 * g], k1c = kcoef[3 * g + 1], k2c = kcoef[3 * g + 2];
+    const bool accum = (mode & SDPV_GEMM_ACCUMULATE) != 0;
+    int16_t* out = acc + (i64)g * ld_g + b * M * N;
+#pragma unroll
+    for (int i = 0; i < BLM; i++)
+#pragma unroll
+        for (int j = 0; j < BLN; j++) {
+            const i64 col = c0 + (wn * BLN + j) * 32 + l32;
+#pragma unroll
+            for (int q = 0; q < 16; q++) {
+                const i64 row = r0 + (wm * BLM + i) * 32 + 8 * (q >> 2) + 4 * hsel + (q & 3);
+                if (row < M && col < N) {
+                    int v = k0c * mod_m(cHH[i][j][q], m, mi) + k1c * mod_m(cSS[i][j][q], m, mi) +
+                            k2c * mod_m(cLL[i][j][q], m, mi);
+                    if (accum) v += out[row * N + col];
+                    out[row * N + col] = (int16_t)mod_m(v, m, mi);
+                }
+            }
+        }
+}
+
+template <int WM, int WN, int BLM, int BLN>
+int launch_gemm(const int8_t* Ah, const int8_t* Al, const int8_t* Bh, const int8_t* Bl, int G, int nb, i64 M, i64 N,
+                i64 kp, int16_t* acc, i64 ld_g, const i32* mods, const double* minv, const i32* kcoef, int g0,
+                int mode, hipStream_t st) {
+    constexpr int BM = WM * BLM * 32, BN = WN * BLN * 32;
+    const i64 tiles_m = (M + BM - 1) / BM, tiles_n = (N + BN - 1) / BN;
+    const i64 total = tiles_m * tiles_n * (i64)G * nb;
+    if (total == 0) return SDPV_OK;
+    if (total > 0x7fffffffLL) return SDPV_ESIZE;
+    const size_t lds = (size_t)2 * 3 * (BM + BN) * 80;
+    static bool attr_set = false;
+    if (!attr_set) {
+        (void)hipFuncSetAttribute((const void*)k_rns_gemm<WM, WN, BLM, BLN>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)lds);
+        attr_set = true;
+    }
+    hipLaunchKernelGGL((k_rns_gemm<WM, WN, BLM, BLN>), dim3((unsigned)total), dim3(WM * WN * 64), lds, st, Ah, Al,
+                       Bh, Bl, G, nb, M, N, kp, acc, ld_g, mods, minv, kcoef, g0, mode, tiles_m, tiles_n);
+    SDPV_TRY_LAUNCH();
+    return SDPV_OK;
+}
+
+// ================================================================== CRT reconstruction
+// QW entries per workgroup. Phase 1: y_g = centred(acc_g * inv_g mod m_g), split to int8 planes
+// in LDS, k = round(sum y_g / m_g) (|sum - k| < 1/4 since |value| < M/4, so double suffices).
+// Phase 2 (rounds of DB digit blocks of 32): Z = Y . Ct^T on MFMA (entries x digits), then one
+// thread per entry carries sum_d (Z_d - k Md_d) 256^d into 32-bit words.
+// Phase 3: words -> W + bitlen, or normalised MPF limbs, or a fixed-exponent rounding.
+__global__ void __launch_bounds__(256) k_rns_crt(
+    const int16_t* __restrict__ acc, int n, i64 ld_g, i64 rows, i64 N, i64 j0, i64 w, const i32* __restrict__ mods,
+    const double* __restrict__ minv, const i32* __restrict__ inv, const int8_t* __restrict__ Ct,
+    const i32* __restrict__ Md, int E, int npad, int mode, i32* __restrict__ W, i64* __restrict__ bitlen,
+    i32* __restrict__ outm, i64* __restrict__ oute, int L, i64 plane, i64 ldo, i64 jo, i64 Mb,
+    const i64* __restrict__ erow,
+    const i64* __restrict__ ecol, const i64* __restrict__ scol, i32* __restrict__ flag, int QW, int DB) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int YS = npad + 16;
+    const int NW = E / 4;
+    int8_t* Yh = (int8_t*)smem;
+    int8_t* Yl = Yh + (size_t)QW * YS;
+    double* ksum = (double*)(Yl + (size_t)QW * YS);  // [8][QW]
+    i32* kq = (i32*)(ksum + 8 * QW);                  // [QW]
+    i64* Z = (i64*)(kq + QW);                         // [QW][DB*32]
+    u32* Wl = (u32*)(Z + (size_t)QW * DB * 32);       // [QW][NW+1] (modes 1, 2)
+    const i64 Bq = rows * w;
+    const i64 q0 = blockIdx.x * (i64)QW;
+    const int t = threadIdx.x;
+
+    {
+        const int ql = t % QW, grp = t / QW, ngrp = 256 / QW;
+        const i64 q = q0 + ql;
+        const bool qok = q < Bq;
+        i64 off = 0;
+        if (qok) off = (q / w) * N + j0 + q % w;
+        double s = 0.0;
+        for (int g = grp; g < npad; g += ngrp) {
+            int hi = 0, lo = 0;
+            if (g < n && qok) {
+                const int m = mods[g];
+                int y = mod_m((int)acc[(i64)g * ld_g + off] * inv[g], m, minv[g]);
+                if (y > (m >> 1)) y -= m;
+                s += (double)y * minv[g];
+                lo = ((y + 64) & 127) - 64;
+                hi = (y - lo) >> 7;
+            }
+            Yh[(size_t)ql * YS + g] = (int8_t)hi;
+            Yl[(size_t)ql * YS + g] = (int8_t)lo;
+        }
+        ksum[grp * QW + ql] = s;
+    }
+    __syncthreads();
+    if (t < QW) {
+        double s = 0.0;
+        for (int gr = 0; gr < 256 / QW; gr++) s += ksum[gr * QW + t];
+        kq[t] = (int)floor(s + 0.5);
+    }
+    __syncthreads();
+
+    const int wave = t >> 6, lane = t & 63, l32 = lane & 31, hsel = lane >> 5;
+    const int nEB = QW / 32;
+    const int nDB = (E + 31) / 32;
+    const int ZS = DB * 32;
+    const int kqt = (t < QW) ? kq[t] : 0;
+    const i64 qt = q0 + t;
+    const bool qtok = (t < QW) && (qt < Bq);
+    i64 carry = 0;
+    int last0 = -1, last1 = -1;
+    u32 w0 = 0, w1 = 0, lastw = 0;
+    for (int db0 = 0; db0 < nDB; db0 += DB) {
+        {
+            const int eb = wave % nEB, dbl = wave / nEB, db = db0 + dbl;
+            if (dbl < DB && db < nDB) {
+                v16i zh = zero16(), zl = zero16();
+                const int8_t* yh = Yh + (size_t)(eb * 32 + l32) * YS;
+                const int8_t* yl = Yl + (size_t)(eb * 32 + l32) * YS;
+                const int8_t* crow = Ct + (i64)(db * 32 + l32) * npad;
+                for (int ks = 0; ks < npad / 32; ks++) {
+                    const int off = ks * 32 + 16 * hsel;
+                    const v4i bfr = *(const v4i*)(crow + off);
+                    zh = mfma(*(const v4i*)(yh + off), bfr, zh);
+                    zl = mfma(*(const v4i*)(yl + off), bfr, zl);
+                }
+#pragma unroll
+                for (int j = 0; j < 16; j++) {
+                    const int row = eb * 32 + 8 * (j >> 2) + 4 * hsel + (j & 3);
+                    Z[(size_t)row * ZS + dbl * 32 + l32] = 128 * (i64)zh[j] + (i64)zl[j];
+                }
+            }
+        }
+        __syncthreads();
+        if (t < QW) {
+            const int nd = ((nDB - db0) < DB ? (nDB - db0) : DB) * 32;
+            for (int d4 = 0; d4 < nd; d4 += 4) {
+                i64 v = carry;
+#pragma unroll
+                for (int u = 0; u < 4; u++) {
+                    const int d = db0 * 32 + d4 + u;
+                    v += (Z[(size_t)t * ZS + d4 + u] - (i64)kqt * (i64)Md[d]) * ((i64)1 << (8 * u));
+                }
+                const int wi = (db0 * 32 + d4) >> 2;
+                const u32 word = (u32)v;
+                carry = v >> 32;
+                if (wi < NW) {
+                    if (mode == SDPV_CRT_WORDS) {
+                        if (qtok) W[(i64)wi * Bq + qt] = (i32)word;
+                        if (word != 0u) { last0 = wi; w0 = word; }
+                        if (word != 0xffffffffu) { last1 = wi; w1 = ~word; }
+                        lastw = word;
+                    } else {
+                        Wl[(size_t)t * (NW + 1) + wi] = word;
+                    }
+                }
+            }
+        }
+        __syncthreads();
+    }
+    if (!qtok) return;
+    const i64 i = qt / w, jj = qt % w;
+    if (mode == SDPV_CRT_WORDS) {
+        const bool neg = (lastw >> 31) != 0;
+        const int last = neg ? last1 : last0;
+        const u32 top = neg ? w1 : w0;
+        bitlen[qt] = last >= 0 ? 32 * (i64)last + 32 - clz32(top) : 0;
+        return;
+    }
+    const i64 o = i * ldo + jo + jj;
+    const i64 cidx = (i / Mb) * ldo + jo + jj;
+    LdsWords src{Wl + (size_t)t * (NW + 1)};
+    if (mode == SDPV_CRT_NORMALIZE) {
+        MagView<LdsWords> mag(src, NW);
+        const int bl = bitlen_mag(mag, NW);
+        if (bl == 0) {
+            for (int k = 0; k < L; k++) outm[(i64)k * plane + o] = 0;
+            oute[o] = EZV;
+            return;
+        }
+        const int s = bl - (32 * L - 2);
+        NegStream ng(mag.neg);
+        shift_mag(mag, NW, s, L, [&](int k, u32 v) { outm[(i64)k * plane + o] = (i32)ng(v); });
+        const i64 ein = (erow ? erow[i] : 0) + (ecol ? ecol[cidx] : 0);
+        oute[o] = ein + s;
+    } else {
+        const int bad = shr_round(src, NW, scol[cidx], L, [&](int k, u32 v) { outm[(i64)k * plane + o] = (i32)v; });
+        if (bad && flag) atomicOr(flag, 1);
+    }
+}
+
+// ================================================================== MFMA layout self-test
+__global__ void __launch_bounds__(64) k_mfma_selftest(const int8_t* A, const int8_t* B, i32* D) {
+    const int lane = threadIdx.x, l32 = lane & 31, hsel = lane >> 5;
+    v4i fa, fb;
+    int8_t* pa = (int8_t*)&fa;
+    int8_t* pb = (int8_t*)&fb;
+    for (int u = 0; u < 16; u++) {
+        pa[u] = A[l32 * 32 + 16 * hsel + u];
+        pb[u] = B[(16 * hsel + u) * 32 + l32];
+    }
+    v16i d = mfma(fa, fb, zero16());
+    for (int j = 0; j < 16; j++) D[(8 * (j >> 2) + 4 * hsel + (j & 3)) * 32 + l32] = d[j];
+}
+
+inline unsigned nblk(i64 n, int tpb) { return (unsigned)((n + tpb - 1) / tpb); }
+
+}  // namespace
+
+extern "C" {
+
+int sdpv_rns_split(const int32_t* X, const int64_t* e, const int64_t* Erow, int64_t L, int64_t Lout, int64_t plane,
+                   int64_t nb, int64_t rows, int64_t ldb, int64_t ldr, int64_t ldk, int64_t k0, int64_t kc,
+                   int64_t kp, const int8_t* Thi, const int8_t* Tlo, int64_t NBp, const int32_t* C1,
+                   const int32_t* C2, const int32_t* mods, const double* minv, int64_t g0, int64_t G, int8_t* Rh,
+                   int8_t* Rl, int32_t* flag, void* stream) {
+    if (L < 1 || Lout < 1 || nb < 0 || rows < 0 || kc < 0 || kp < kc || G < 0 || g0 < 0) return SDPV_EARG;
+    if (kp % 64 != 0 || NBp != 32 * ((4 * Lout + 31) / 32)) return SDPV_EARG;
+    if (Lout > 128) return SDPV_ESIZE;
+    const i64 nrows = nb * rows;
+    if (nrows == 0 || kp == 0 || G == 0) return SDPV_OK;
+    if (!X || !Thi || !Tlo || !C1 || !C2 || !mods || !minv || !Rh || !Rl) return SDPV_EARG;
+    if (e && !Erow) return SDPV_EARG;
+    const int EPW = (4 * Lout <= 128) ? 256 : ((4 * Lout <= 256) ? 128 : 64);
+    const int TK = (ldk == 1) ? 64 : 16;
+    const int TR = EPW / TK;
+    const i64 tiles_k = kp / TK, tiles_r = (nrows + TR - 1) / TR;
+    const i64 grid = tiles_r * tiles_k;
+    if (grid > 0x7fffffffLL) return SDPV_ESIZE;
+    const size_t lds = (size_t)EPW * (NBp + 16) + EPW + (size_t)2 * 32 * EPW;
+    hipLaunchKernelGGL(k_rns_split, dim3((unsigned)grid), dim3(256), lds, as_stream(stream), X, e, Erow, (int)L,
+                       (int)Lout, (i64)plane, (i64)nb, (i64)rows, (i64)ldb, (i64)ldr, (i64)ldk, (i64)k0, (i64)kc,
+                       (i64)kp, Thi, Tlo, (int)NBp, C1, C2, mods, minv, (int)g0, (int)G, Rh, Rl, flag, EPW, TK,
+                       tiles_k);
+    SDPV_TRY_LAUNCH();
+    return SDPV_OK;
+}
+
+int sdpv_rns_gemm(const int8_t* Ah, const int8_t* Al, const int8_t* Bh, const int8_t* Bl, int64_t G, int64_t nb,
+                  int64_t M, int64_t N, int64_t kp, int16_t* acc, int64_t ld_g, const int32_t* mods,
+                  const double* minv, const int32_t* kcoef, int64_t g0, int32_t mode, void* stream) {
+    if (G < 0 || nb < 0 || M < 0 || N < 0 || kp < 0 || g0 < 0 || (mode & ~3)) return SDPV_EARG;
+    if (kp % 64 != 0 || kp > 65536) return SDPV_ESIZE;
+    if ((mode & SDPV_GEMM_SYRK_LOWER) && M != N) return SDPV_EARG;
+    if (G == 0 || nb == 0 || M == 0 || N == 0) return SDPV_OK;
+    if (!Ah || !Al || !Bh || !Bl || !acc || !mods || !minv || !kcoef) return SDPV_EARG;
+    hipStream_t st = as_stream(stream);
+    if (kp == 0) return SDPV_EARG;
+    const i64 mn = M < N ? M : N;
+    if (mn >= 128)
+        return launch_gemm<2, 2, 2, 2>(Ah, Al, Bh, Bl, (int)G, (int)nb, M, N, kp, acc, ld_g, mods, minv, kcoef,
+                                       (int)g0, mode, st);
+    if (mn >= 64)
+        return launch_gemm<2, 2, 1, 1>(Ah, Al, Bh, Bl, (int)G, (int)nb, M, N, kp, acc, ld_g, mods, minv, kcoef,
+                                       (int)g0, mode, st);
+    return launch_gemm<1, 1, 1, 1>(Ah, Al, Bh, Bl, (int)G, (int)nb, M, N, kp, acc, ld_g, mods, minv, kcoef, (int)g0,
+                                   mode, st);
+}
+
+int sdpv_rns_crt(const int16_t* acc, int64_t n, int64_t ld_g, int64_t rows, int64_t N, int64_t j0, int64_t w,
+                 const int32_t* mods, const double* minv, const int32_t* inv, const int8_t* Ct, const int32_t* Md,
+                 int64_t E, int64_t npad, int32_t mode, int32_t* W, int64_t* bitlen, int32_t* outm, int64_t* oute,
+                 int64_t L, int64_t plane, int64_t ldo, int64_t jo, int64_t Mb, const int64_t* erow,
+                 const int64_t* ecol, const int64_t* scol, int32_t* flag, void* stream) {
+    if (n < 1 || rows < 0 || N < 0 || j0 < 0 || w < 0 || E < 4 || E % 4 != 0 || npad < n || npad % 32 != 0)
+        return SDPV_EARG;
+    if (mode < 0 || mode > 2) return SDPV_EARG;
+    const i64 Bq = rows * w;
+    if (Bq == 0) return SDPV_OK;
+    if (!acc || !mods || !minv || !inv || !Ct || !Md) return SDPV_EARG;
+    if (mode == SDPV_CRT_WORDS && (!W || !bitlen)) return SDPV_EARG;
+    if (mode != SDPV_CRT_WORDS && (!outm || L < 1 || Mb < 1)) return SDPV_EARG;
+    if (mode == SDPV_CRT_NORMALIZE && !oute) return SDPV_EARG;
+    if (mode == SDPV_CRT_ROUND && !scol) return SDPV_EARG;
+    const int QW = (npad <= 352) ? 64 : 32;
+    const int DB = 4 / (QW / 32);
+    const int NW = (int)(E / 4);
+    size_t lds = (size_t)2 * QW * (npad + 16) + (size_t)8 * QW * 8 + (size_t)QW * 4 + (size_t)QW * DB * 32 * 8;
+    if (mode != SDPV_CRT_WORDS) lds += (size_t)QW * (NW + 1) * 4;
+    if (lds > 160 * 1024) return SDPV_ESIZE;
+    const i64 grid = (Bq + QW - 1) / QW;
+    if (grid > 0x7fffffffLL) return SDPV_ESIZE;
+    static bool attr_set = false;
+    if (!attr_set) {
+        (void)hipFuncSetAttribute((const void*)k_rns_crt, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+        attr_set = true;
+    }
+    hipLaunchKernelGGL(k_rns_crt, dim3((unsigned)grid), dim3(256), lds, as_stream(stream), acc, (int)n, (i64)ld_g,
+                       (i64)rows, (i64)N, (i64)j0, (i64)w, mods, minv, inv, Ct, Md, (int)E, (int)npad, (int)mode, W,
+                       bitlen, outm, oute, (int)L, (i64)plane, (i64)ldo, (i64)jo, (i64)Mb, erow, ecol, scol, flag, QW, DB);
+    SDPV_TRY_LAUNCH();
+    return SDPV_OK;
+}
+
+int sdpv_mfma_selftest(const int8_t* A, const int8_t* B, int32_t* D, void* stream) {
+    if (!A || !B || !D) return SDPV_EARG;
+    hipLaunchKernelGGL(k_mfma_selftest, dim3(1), dim3(64), 0, as_stream(stream), A, B, D);
+    SDPV_TRY_LAUNCH();
+    return SDPV_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,380 @@
+"""Multiprecision floating point arrays on the GPU: one exponent per entry (mirror of
+sdpvibe/mpf.py).
+
+MPF: m int32 (L, *shape) limb planes (little endian, two's complement), e int64 (*shape);
+value = m * 2^e. Normalised: |m| has exactly NB = 32L-2 bits (NB+1 after a rounding carry;
+zero: m = 0, e = EZ).
+
+Elementwise ops are gfx950 kernels (one thread per entry); matrix products go through the exact
+RNS GEMM on INT8 matrix cores with operands aligned per row (A) / per column (B) -- the usual
+normwise floating-point error -- and every output entry rounded once.
+"""
+import mpmath as mp
+import numpy as np
+import torch
+
+from ._lib import call, ptr, stream
+from .limbops import bitlen_words, shift_round
+from .mpmat import ACC_BUDGET, ctx, residue_gemm
+from .rns import dev
+
+EZ = -(1 << 40)                                      # exponent of zero
+LMAX_WIDE = 1 << 20                                  # normalize() takes any input width
+
+
+class MPF:
+    __slots__ = ("m", "e")
+
+    def __init__(self, m, e):
+        self.m, self.e = m, e
+
+    @property
+    def L(self):
+        return self.m.shape[0]
+
+    @property
+    def shape(self):
+        return tuple(self.e.shape)
+
+    def __getitem__(self, idx):
+        if not isinstance(idx, tuple):
+            idx = (idx,)
+        return MPF(self.m[(slice(None),) + idx].contiguous(), self.e[idx].contiguous())
+
+    @property
+    def T(self):
+        """transpose of the last two dimensions"""
+        return MPF(self.m.transpose(-1, -2).contiguous(), self.e.transpose(-1, -2).contiguous())
+
+    def reshape(self, *s):
+        return MPF(self.m.reshape(self.L, *s), self.e.reshape(*s))
+
+    def clone(self):
+        return MPF(self.m.clone(), self.e.clone())
+
+    def __neg__(self):
+        return add(zeros(self.shape, self.L), self, -1)
+
+    def __add__(self, o):
+        return add(self, o, 1)
+
+    def __sub__(self, o):
+        return add(self, o, -1)
+
+    def shift(self, k):
+        """multiply by 2^k (k: int or tensor broadcastable to shape) -- exact (mpf.py:194-197)"""
+        e = self.e + k
+        return MPF(self.m, torch.where(self.e == EZ, EZ, e))
+
+    def to_mp(self):
+        """-> numpy object array of mpmath mpf (host; tests / scalars)"""
+        L = self.L
+        a = self.m.reshape(L, -1).t().contiguous().cpu().numpy().astype(np.uint32)
+        ex = self.e.reshape(-1).cpu().numpy()
+        out = np.empty(a.shape[0], dtype=object)
+        for i in range(a.shape[0]):
+            v = int.from_bytes(a[i].tobytes(), "little", signed=True)
+            out[i] = mp.mpf(0) if v == 0 else mp.mpf((v, int(ex[i])))
+        return out.reshape(self.shape)
+
+    def to_float(self):
+        """approximate float64 values (with exponent clamping); host numpy (mpf.py:210-215)"""
+        top = self.m[-1].double() * 2.0 ** 32 + (self.m[-2].double() % 2.0 ** 32) if self.L > 1 else self.m[-1].double()
+        sc = (self.e + 32 * (self.L - 2)).clamp(-1100, 1100).double()
+        v = torch.where(self.e == EZ, torch.zeros_like(top), top * torch.pow(2.0, sc))
+        return v.cpu().numpy()
+
+
+def zeros(shape, L):
+    return MPF(torch.zeros((L, *shape), dtype=torch.int32, device=dev),
+               torch.full(shape, EZ, dtype=torch.int64, device=dev))
+
+
+def NBITS(L):
+    return 32 * L - 2
+
+
+def from_ints(ints, exps, L, shape):
+    """Python ints (mantissas) and exponents (flat lists) -> normalised MPF (mpf.py:226-235)"""
+    n = len(ints)
+    nb = 4 * L
+    buf = bytearray(nb * n)
+    for i, v in enumerate(ints):
+        buf[i * nb:(i + 1) * nb] = v.to_bytes(nb, "little", signed=True)
+    m = torch.from_numpy(np.frombuffer(bytes(buf), dtype=np.int32).reshape(n, L).T.copy()).to(dev)
+    e = torch.tensor(exps, dtype=torch.int64, device=dev)
+    return normalize(m, e, L).reshape(*shape)
+
+
+def _dec_to_bin(t, NBb):
+    """decimal string -> (signed mantissa with <= NBb bits, binary exponent) (mpf.py:238-261)"""
+    t = t.strip()
+    neg = t.startswith("-")
+    if t[0] in "+-":
+        t = t[1:]
+    e10 = 0
+    if "e" in t or "E" in t:
+        t, ex = t.replace("E", "e").split("e")
+        e10 = int(ex)
+    if "." in t:
+        ip, fp = t.split(".")
+        e10 -= len(fp)
+        t = ip + fp
+    man = int(t) if t else 0
+    if man == 0:
+        return 0, EZ
+    if e10 >= 0:
+        v = man * 10 ** e10
+        bl = v.bit_length()
+        if bl <= NBb:
+            return (-v if neg else v), 0
+        sh = bl - NBb
+        v = (v + (1 << (sh - 1))) >> sh
+        return (-v if neg else v), sh
+    den = 10 ** (-e10)
+    sh = NBb + den.bit_length() - man.bit_length() + 1
+    q = ((man << sh) + den // 2) // den
+    ex = -sh
+    while q.bit_length() > NBb:
+        q = (q + 1) >> 1
+        ex += 1
+    return (-q if neg else q), ex
+
+
+def from_strings(strs, L, shape=None):
+    """decimal strings -> MPF (mpf.py:264-270)"""
+    NBb = NBITS(L)
+    ints, exps = [], []
+    for t in strs:
+        m_, e_ = _dec_to_bin(t, NBb)
+        ints.append(m_)
+        exps.append(e_)
+    return from_ints(ints, exps, L, shape or (len(ints),))
+
+
+def strings_to_host(strs, L):
+    """decimal strings -> (limbs int32 (n, L) entry-major numpy, exps int64 numpy) (mpf.py:273-281)"""
+    NBb = NBITS(L)
+    nb = 4 * L
+    n = len(strs)
+    buf = bytearray(nb * n)
+    exps = np.empty(n, dtype=np.int64)
+    for i, t in enumerate(strs):
+        m_, e_ = _dec_to_bin(t, NBb)
+        buf[i * nb:(i + 1) * nb] = m_.to_bytes(nb, "little", signed=True)
+        exps[i] = e_
+    return np.frombuffer(bytes(buf), dtype=np.int32).reshape(n, L), exps
+
+
+def host_to_mpf(limbs, exps, L, shape):
+    """inverse of strings_to_host, on the GPU (normalised) (mpf.py:284-288)"""
+    m = torch.from_numpy(np.ascontiguousarray(limbs.T)).to(dev)
+    e = torch.from_numpy(exps).to(dev)
+    return normalize(m, e, L).reshape(*shape)
+
+
+def from_mp(vals, L, shape=None):
+    """iterable of mpmath numbers / strings -> MPF (rounded to NB bits) (mpf.py:291-304)"""
+    vals = list(vals)
+    NBb = NBITS(L)
+    ints, exps = [], []
+    for v in vals:
+        v = mp.mpf(v)
+        if v == 0:
+            ints.append(0)
+            exps.append(EZ)
+            continue
+        sign, man, ex, bc = v._mpf_
+        man = int(man)
+        if bc > NBb:                                  # round to NB bits (half up on magnitude)
+            sh = bc - NBb
+            man = (man + (1 << (sh - 1))) >> sh
+            ex += sh
+        ints.append(-man if sign else man)
+        exps.append(ex)
+    return from_ints(ints, exps, L, shape or (len(vals),))
+
+
+def normalize(Xw, ein, L):
+    """Xw: (LI, n) int32 wide two's complement, ein (n,) -> normalised MPF (n,) (mpf.py:307-313)"""
+    LI, n = Xw.shape
+    Xw, ein = Xw.contiguous(), ein.contiguous()
+    O = torch.empty((L, n), dtype=torch.int32, device=Xw.device)
+    eO = torch.empty((n,), dtype=torch.int64, device=Xw.device)
+    call("sdpv_mpf_norm", ptr(Xw), ptr(ein), LI, ptr(O), ptr(eO), L, n, stream())
+    return MPF(O, eO)
+
+
+def mul(A, B):
+    """entrywise product; B may be a single entry (broadcast) (mpf.py:316-323)"""
+    L = A.L
+    n = A.e.numel()
+    bs = B.e.numel() == 1
+    assert bs or B.shape == A.shape
+    Am, Ae, Bm, Be = A.m.contiguous(), A.e.contiguous(), B.m.contiguous(), B.e.contiguous()
+    O = torch.empty_like(Am)
+    eO = torch.empty_like(Ae)
+    call("sdpv_mpf_mul", ptr(Am), ptr(Ae), ptr(Bm), ptr(Be), ptr(O), ptr(eO), L, n, int(bs), stream())
+    return MPF(O, eO)
+
+
+def add(A, B, sgn=1):
+    """A + sgn*B, normalised (mpf.py:326-331)"""
+    assert A.shape == B.shape, (A.shape, B.shape)
+    L = A.L
+    n = A.e.numel()
+    Am, Ae, Bm, Be = A.m.contiguous(), A.e.contiguous(), B.m.contiguous(), B.e.contiguous()
+    O = torch.empty_like(Am)
+    eO = torch.empty_like(Ae)
+    call("sdpv_mpf_add", ptr(Am), ptr(Ae), ptr(Bm), ptr(Be), ptr(O), ptr(eO), L, n, int(sgn), stream())
+    return MPF(O, eO)
+
+
+def scalar(v, L):
+    return from_mp([v], L, (1,))
+
+
+def eye(n, L, val=1):
+    A = zeros((n, n), L)
+    d = from_mp([val], L, (1,))
+    idx = torch.arange(n, device=dev)
+    A.m[:, idx, idx] = d.m[:, :1]
+    A.e[idx, idx] = d.e[0]
+    return A
+
+
+# ------------------------------------------------------------------ reductions (host results)
+def _top(A):
+    """per-entry e + bitlen (log2 of magnitude, roughly); EZ for zeros (mpf.py:348-351)"""
+    m, e = A.m.contiguous(), A.e.contiguous()
+    out = torch.empty_like(e)
+    call("sdpv_mpf_top", ptr(m), ptr(e), A.L, e.numel(), ptr(out), stream())
+    return out
+
+
+def total(A):
+    """exact-ish sum of all entries -> mpmath mpf (error ~ 2^-NB * n * max) (mpf.py:354-366)"""
+    L = A.L
+    n = A.e.numel()
+    if n == 0:
+        return mp.mpf(0)
+    t = int(A.e.max()) - 32
+    if t < EZ // 2:
+        return mp.mpf(0)
+    s = (t - A.e.reshape(-1))
+    W = shift_round(A.m.reshape(L, -1), s, L + 2)
+    S = (W.to(torch.int64) & 0xFFFFFFFF).sum(1)
+    top = W[-1].to(torch.int64).sum()
+    S = S.cpu().tolist()
+    top = int(top)
+    v = sum(int(S[k]) << (32 * k) for k in range(L + 1)) + (top << (32 * (L + 1)))
+    return mp.mpf((v, t)) if v else mp.mpf(0)
+
+
+def dot(A, B):
+    return total(mul(A, B))
+
+
+def max_abs(A):
+    """(mpf.py:372-381)"""
+    if A.e.numel() == 0:
+        return mp.mpf(0)
+    tp = _top(A).reshape(-1)
+    t = int(tp.max())
+    if t <= EZ // 2:
+        return mp.mpf(0)
+    cand = torch.nonzero(tp >= t - 1).reshape(-1)
+    Af = A.reshape(-1)
+    vals = MPF(Af.m[:, cand], Af.e[cand]).shift(-t).to_float()
+    i = int(cand[int(np.argmax(np.abs(vals)))])
+    return abs(Af[i:i + 1].to_mp()[0])
+
+
+def trace(A):
+    n = A.shape[0]
+    idx = torch.arange(n, device=dev)
+    return total(MPF(A.m[:, idx, idx], A.e[idx, idx]))
+
+
+# ------------------------------------------------------------------ GEMM
+def _row_exp(m, e, L, plane, nb, rows, K, ldb, ldr, ldk):
+    Erow = torch.empty((nb * rows,), dtype=torch.int64, device=m.device)
+    call("sdpv_row_exp", ptr(m), ptr(e), L, plane, nb, rows, K, ldb, ldr, ldk, ptr(Erow), stream())
+    return Erow
+
+
+def _align_rows(A):
+    """A (..., R, K) -> (mantissas (L, ..., R, K) with a common exponent per row, row exps (..., R))
+    (mpf.py:390-398)"""
+    L = A.L
+    R_, K = A.shape[-2], A.shape[-1]
+    nb = A.e.numel() // max(R_ * K, 1)
+    m, e = A.m.contiguous(), A.e.contiguous()
+    E = _row_exp(m, e, L, e.numel(), nb, R_, K, R_ * K, K, 1).view(*A.shape[:-1])
+    s = (E[..., None] - A.e).reshape(-1)
+    s = torch.where(A.e.reshape(-1) == EZ, torch.full_like(s, 1 << 30), s)
+    return shift_round(A.m.reshape(L, -1), s, L).view(A.m.shape), E
+
+
+def _geometry(shape, trans):
+    """(rows, K, ldb, ldr, ldk) of op(X) rows over a stored (nb, r, c) matrix X; op = X or X^T"""
+    _, r, c = shape
+    if trans:
+        return c, r, r * c, 1, c
+    return r, c, r * c, c, 1
+
+
+def gemm(A, B, transA=False, transB=False):
+    """C = op(A) op(B) (2-d, or batched 3-d (nb, ., .)); exact products, rounded once per entry
+    (operands aligned per row of op(A) / column of op(B): normwise floating-point error)
+    (mpf.py:401-452).  Row alignment and the residue split are fused into one kernel per operand;
+    the product and the CRT + normalisation are one kernel each."""
+    if A.e.dim() == 2:
+        A1 = MPF(A.m[:, None], A.e[None])
+        B1 = MPF(B.m[:, None], B.e[None])
+        C = gemm(A1, B1, transA, transB)
+        return MPF(C.m[:, 0], C.e[0])
+    L = A.L
+    assert B.L == L
+    Am, Ae, Bm, Be = A.m.contiguous(), A.e.contiguous(), B.m.contiguous(), B.e.contiguous()
+    nb = Ae.shape[0]
+    assert Be.shape[0] == nb
+    M, K, aldb, aldr, aldk = _geometry(Ae.shape, transA)
+    N, K2, bldb, bldr, bldk = _geometry(Be.shape, not transB)   # rows of op(B)^T = columns of op(B)
+    assert K == K2, (A.shape, B.shape, transA, transB)
+    R = ctx(L)
+    EA = _row_exp(Am, Ae, L, Ae.numel(), nb, M, K, aldb, aldr, aldk)
+    EB = _row_exp(Bm, Be, L, Be.numel(), nb, N, K, bldb, bldr, bldk)
+    out_m = torch.empty((L, nb, M, N), dtype=torch.int32, device=dev)
+    out_e = torch.empty((nb, M, N), dtype=torch.int64, device=dev)
+    if nb * M * N == 0:
+        return MPF(out_m, out_e)
+
+    def fa(k0, kc, kp, j0, jw):
+        Rh = torch.empty((R.n, nb * M, kp), dtype=torch.int8, device=dev)
+        Rl = torch.empty_like(Rh)
+        R.split(Am, Ae, EA, L, Ae.numel(), nb, M, aldb, aldr, aldk, k0, kc, kp, Rh, Rl)
+        return Rh, Rl
+
+    def fb(k0, kc, kp, j0, jw):
+        Rh = torch.empty((R.n, nb * jw, kp), dtype=torch.int8, device=dev)
+        Rl = torch.empty_like(Rh)
+        if (j0, jw) == (0, N):
+            R.split(Bm, Be, EB, L, Be.numel(), nb, N, bldb, bldr, bldk, k0, kc, kp, Rh, Rl)
+        else:                                  # column panel: rows j0..j0+jw of op(B)^T
+            Eb = EB.view(nb, N)[:, j0:j0 + jw].contiguous()
+            off = j0 * bldr
+            Bms = Bm.view(L, -1)[:, off:].contiguous()
+            Bes = Be.view(-1)[off:].contiguous()
+            R.split(Bms, Bes, Eb, L, Bes.numel(), nb, jw, bldb, bldr, bldk, k0, kc, kp, Rh, Rl)
+        return Rh, Rl
+
+    for j0, jw, acc in residue_gemm(R, fa, fb, nb, M, N, K, ACC_BUDGET):
+        R.crt(acc, nb * M, jw, 0, jw, 1, outm=out_m, oute=out_e, L=L, plane=nb * M * N, ldo=N, jo=j0, Mb=M,
+              erow=EA, ecol=EB)
+    return MPF(out_m, out_e)
+
+
+def symmetrize(A):
+    """(A + A^T)/2"""
+    return add(A, A.T).shift(-1)

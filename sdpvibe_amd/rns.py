@@ -1,0 +1,172 @@
+"""Exact multiprecision GEMM on CDNA4 INT8 matrix cores via a residue number system (mirror of
+sdpvibe/rns.py).
+
+Number format (see mpmat.py): two's-complement integers of L 32-bit limbs in limb planes.
+An exact product C = A @ Bt^T of L-limb integers is computed as:
+   1. split: every entry is reduced modulo n primes m <= 16257 to a centred residue
+      r = 128*h + l (h, l int8), as a byte dot product with a table of 256^j mod m on INT8 MFMA;
+   2. products: per prime, three INT8 GEMMs (HH = h.h, SS = (h+l).(h+l), LL = l.l; h + l fits
+      int8 because |r| <= 8128), combined as 16256*HH + 128*SS - 127*LL mod m;
+   3. CRT: y_g = r_g (M/m_g)^-1 mod m_g, Z = Y . digits(M/m_g) on INT8 MFMA, carry to words.
+Prime count n: the smallest with M > 4 * kmax * 2^(64L-2), so every product with inner dimension
+<= kmax is reconstructed exactly and |C| < M/4.
+
+The prime set differs from the reference's (primes <= 32639, four products per prime) so that each
+prime costs three INT8 products; the reconstructed integers, and hence every rounded result, are
+identical.
+"""
+import math
+
+import numpy as np
+import torch
+
+from ._lib import call, ptr, stream
+
+dev = "cuda"
+PMAX = 16257       # largest modulus with centred residues |r| <= 8128: h + l stays in int8
+PMIN = 1 << 13
+KMAX_LAUNCH = 65536  # inner dimension per product launch (int32 accumulators stay exact)
+
+
+def _primes_desc(hi, lo):
+    """primes in (lo, hi], descending"""
+    s = np.ones(hi + 1, bool)
+    s[:2] = False
+    for p in range(2, int(hi ** 0.5) + 1):
+        if s[p]:
+            s[p * p::p] = False
+    return [int(p) for p in range(hi, lo, -1) if s[p]]
+
+
+def _sdigits(x, base, n):
+    """signed digits in [-base/2, base/2) of a Python int (rns.py:29-35)"""
+    h = base // 2
+    out = []
+    for _ in range(n):
+        r = ((x + h) % base) - h
+        out.append(r)
+        x = (x - r) // base
+    assert x == 0
+    return out
+
+
+def gemm_config(M, N, K):
+    """(BM, BN) output tile of the product kernel for an M x N x K product (fixed heuristic of the
+    library; rns.py:38-54 chose Triton tiles)."""
+    mn = min(M, N)
+    if mn >= 128:
+        return 128, 128
+    if mn >= 64:
+        return 64, 64
+    return 32, 32
+
+
+class RNS:
+    """Moduli and tables for exact products of L-limb integers with inner dimension <= kmax
+    (rns.py:163-191)."""
+    KPAD = 64
+
+    def __init__(self, L, kmax=1 << 17, G=16, device=None):
+        self.L, self.G, self.NB, self.kmax = L, G, 4 * L, kmax
+        device = device if device is not None else torch.device(dev, torch.cuda.current_device())
+        bound = kmax * (1 << (32 * L - 1)) ** 2              # |C| < bound
+        mods, M = [], 1
+        for p in _primes_desc(PMAX, PMIN):
+            if M > 4 * bound:
+                break
+            mods.append(p)
+            M *= p
+        if M <= 4 * bound:
+            raise ValueError(f"L={L}: not enough primes below {PMAX}")
+        self.mods, self.M, self.n = mods, M, len(mods)
+        n = self.n
+        self.npad = -(-n // 32) * 32
+        self.NBp = 32 * (-(-4 * L // 32))
+        t = lambda a, dt: torch.tensor(a, dtype=dt, device=device)
+        self.m32 = t(mods, torch.int32)
+        self.minv = t([1.0 / m for m in mods], torch.float64)
+        cen = lambda v, m: v - m if v > m // 2 else v
+        T = np.zeros((self.npad, self.NBp), np.int64)
+        for i, m in enumerate(mods):
+            T[i, :4 * L] = [cen(pow(256, j, m), m) for j in range(4 * L)]
+        T_lo = ((T + 64) % 128) - 64
+        T_hi = (T - T_lo) // 128
+        self.Thi = t(T_hi, torch.int8).contiguous()
+        self.Tlo = t(T_lo, torch.int8).contiguous()
+        self.C1 = t([(128 * int(T[i].sum())) % m for i, m in enumerate(mods)], torch.int32)
+        self.C2 = t([pow(2, 32 * L, m) for m in mods], torch.int32)
+        self.kcoef = t([[16256 % m, 128 % m, (-127) % m] for m in mods], torch.int32).contiguous()
+        # CRT in base 256
+        self.E = (math.ceil(M.bit_length() / 8) + 2 + 7) // 8 * 8
+        self.Epad = -(-self.E // 32) * 32
+        self.inv32 = t([pow(M // m % m, -1, m) for m in mods], torch.int32)
+        Ct = np.zeros((self.Epad, self.npad), np.int64)
+        for i, m in enumerate(mods):
+            Ct[:self.E, i] = _sdigits(M // m, 256, self.E)
+        self.Ct = t(Ct, torch.int8).contiguous()
+        Md = np.zeros(self.Epad, np.int64)
+        Md[:self.E] = _sdigits(M, 256, self.E)
+        self.Md = t(Md, torch.int32)
+
+    # ---- step 1: residues of rows x kc entries (limbs) -> (G, rows, kp) hi, lo
+    def kp(self, kc):
+        return -(-kc // self.KPAD) * self.KPAD
+
+    def residues(self, X, k0, kc, g0=0, G=None):
+        """X: (L, rows, K) int32 limb planes. Residues of X[:, :, k0:k0+kc] modulo primes
+        g0..g0+G as (G, rows, kp) int8 hi / lo (128*hi + lo = centred residue), inner dimension
+        zero-padded to kp = kc rounded up to a multiple of 64 (rns.py:196-214)."""
+        L, rows, K = X.shape
+        assert L == self.L
+        G = self.n - g0 if G is None else G
+        kp = self.kp(kc)
+        X = X.contiguous()
+        Rh = torch.empty((G, rows, kp), dtype=torch.int8, device=X.device)
+        Rl = torch.empty_like(Rh)
+        self.split(X, None, None, L, rows * K, 1, rows, 0, K, 1, k0, kc, kp, Rh, Rl, g0, G)
+        return Rh, Rl
+
+    def split(self, X, e, Erow, Lin, plane, nb, rows, ldb, ldr, ldk, k0, kc, kp, Rh, Rl, g0=0, G=None,
+              flag=None):
+        """Fused rounding shift + residue split (see sdpv_rns_split): entry (b, r, k0 + k) at
+        b*ldb + r*ldr + (k0+k)*ldk of every limb plane of X (Lin limbs) and of e, shifted by
+        Erow[b*rows + r] - e (none when e is None) into self.L limbs."""
+        G = self.n - g0 if G is None else G
+        call("sdpv_rns_split", ptr(X), ptr(e), ptr(Erow), Lin, self.L, plane, nb, rows, ldb, ldr, ldk, k0, kc,
+             kp, ptr(self.Thi), ptr(self.Tlo), self.NBp, ptr(self.C1), ptr(self.C2), ptr(self.m32),
+             ptr(self.minv), g0, G, ptr(Rh), ptr(Rl), ptr(flag), stream())
+
+    # ---- step 2: accumulate residues of A @ Bt^T
+    def product(self, Ah, Al, Bh, Bl, acc, nb, M, N, ld_g, g0=0, accumulate=False, syrk=False):
+        G, _, kp = Ah.shape
+        mode = (1 if accumulate else 0) | (2 if syrk else 0)
+        call("sdpv_rns_gemm", ptr(Ah), ptr(Al), ptr(Bh), ptr(Bl), G, nb, M, N, kp, ptr(acc), ld_g,
+             ptr(self.m32), ptr(self.minv), ptr(self.kcoef), g0, mode, stream())
+
+    def accumulate(self, Ahi, Alo, Bhi, Blo, acc, g0=0):
+        """acc[g0:g0+G] += residues of A @ B^T for the G moduli whose residues are given
+        (rns.py:217-224)"""
+        G, M, KC = Ahi.shape
+        N = Bhi.shape[1]
+        assert acc.is_contiguous() and acc.shape[1:] == (M, N)
+        self.product(Ahi, Alo, Bhi, Blo, acc, 1, M, N, M * N, g0, accumulate=True)
+
+    # ---- step 3: CRT of a block of entries
+    def crt(self, acc, rows, N, j0, w, mode, ld_g=None, W=None, bitlen=None, outm=None, oute=None, L=None,
+            plane=0, ldo=0, jo=0, Mb=1, erow=None, ecol=None, scol=None, flag=None):
+        ld_g = acc.stride(0) if ld_g is None else ld_g
+        call("sdpv_rns_crt", ptr(acc), self.n, ld_g, rows, N, j0, w, ptr(self.m32), ptr(self.minv),
+             ptr(self.inv32), ptr(self.Ct), ptr(self.Md), self.E, self.npad, mode, ptr(W), ptr(bitlen),
+             ptr(outm), ptr(oute), L or self.L, plane, ldo, jo, Mb, ptr(erow), ptr(ecol), ptr(scol),
+             ptr(flag), stream())
+
+    def crt_block(self, acc, N, j0, w, rows):
+        """acc: (n, rows_total, N) int16 residues in [0, m); block = rows < `rows`, columns
+        j0..j0+w; entries q = i*w + jj. Returns (W (E/4, Bq) int32 words, bitlen (Bq,), Bq)
+        (rns.py:227-245)."""
+        Bq = rows * w
+        acc = acc.contiguous()
+        W = torch.empty((self.E // 4, Bq), dtype=torch.int32, device=acc.device)
+        bl = torch.empty((Bq,), dtype=torch.int64, device=acc.device)
+        self.crt(acc, rows, N, j0, w, 0, W=W, bitlen=bl)
+        return W, bl, Bq

@@ -1,0 +1,74 @@
+"""Python-int helpers shared by the tests: limb planes <-> ints and the reference's closed forms.
+Independent of both the HIP library and the C oracle (plain Python integers)."""
+import random
+
+import numpy as np
+
+EZ = -(1 << 40)
+MASK = (1 << 32) - 1
+
+
+def to_planes(values, L):
+    """list of Python ints -> (L, n) int32 limb planes (two's complement, wraps mod 2^(32L))"""
+    out = np.zeros((L, len(values)), np.int64)
+    for j, v in enumerate(values):
+        u = v % (1 << (32 * L))
+        for k in range(L):
+            out[k, j] = (u >> (32 * k)) & MASK
+    out = np.where(out >= 1 << 31, out - (1 << 32), out)
+    return out.astype(np.int32)
+
+
+def from_planes(X):
+    """(L, n) int32 limb planes -> list of signed Python ints"""
+    X = np.asarray(X).reshape(X.shape[0], -1).astype(np.int64) & MASK
+    L = X.shape[0]
+    vals = []
+    for j in range(X.shape[1]):
+        v = 0
+        for k in reversed(range(L)):
+            v = (v << 32) | int(X[k, j])
+        if v >> (32 * L - 1):
+            v -= 1 << (32 * L)
+        vals.append(v)
+    return vals
+
+
+def bitlen_tc(v):
+    """bit length of v, of ~v for negatives (tests/test_rns_kernels.py:78-80)"""
+    return (v if v >= 0 else ~v).bit_length()
+
+
+def rnd_half_up(v, s):
+    """round(v / 2^s) half toward +inf, s may be <= 0 (tests/test_gemm.py:195-197)"""
+    if s <= 0:
+        return v << (-s)
+    return (v + (1 << (s - 1))) >> s
+
+
+def wrap(v, L):
+    """two's-complement wrap to L limbs"""
+    M = 1 << (32 * L)
+    v %= M
+    return v - M if v >> (32 * L - 1) else v
+
+
+def norm_closed_form(v, NB):
+    """mpf_norm / mpf_mul rounding of an exact integer v (tests/test_mpf_mul.py:131-139):
+    (mantissa, shift); magnitude rounded half up to NB bits; (0, None) for zero"""
+    if v == 0:
+        return 0, None
+    mag = abs(v)
+    shift = mag.bit_length() - NB
+    r = ((mag + (1 << (shift - 1))) >> shift) if shift > 0 else mag << -shift
+    return (r if v > 0 else -r), shift
+
+
+def rand_mpf_ints(rng, n, L, espread=2000):
+    """normalised random MPF entries as (mantissa ints, exponents) incl. edge cases"""
+    NB = 32 * L - 2
+    edge = [0, 1, -1, (1 << NB) - 1, -((1 << NB) - 1), 1 << (NB - 1), -(1 << (NB - 1)), (1 << (NB - 1)) + 1]
+    vals = edge + [rng.randrange(-(1 << NB), 1 << NB) for _ in range(max(0, n - len(edge)))]
+    vals = vals[:n]
+    exps = [rng.randrange(-espread, espread) for _ in vals]
+    return vals, exps

@@ -1,0 +1,278 @@
+"""GPU parity: every HIP entry point against the C oracle on the same seeded inputs (bit-exact).
+Runs only on a gfx950 GPU with libsdpvibe_hip.so built."""
+import random
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import oracle as O
+from sdpvibe_amd import _lib, limbops
+from sdpvibe_amd import mpf as F
+from sdpvibe_amd import mpmat
+from sdpvibe_amd.synthetic import gaussian_mpf
+from tests.intutil import EZ, from_planes, rand_mpf_ints, to_planes
+
+pytestmark = pytest.mark.gpu
+
+
+def cu(a):
+    return torch.from_numpy(np.ascontiguousarray(a)).cuda()
+
+
+def host(t):
+    return t.cpu().numpy()
+
+
+def _eq(a, b, what=""):
+    a, b = np.asarray(a), np.asarray(b)
+    assert a.shape == b.shape, (what, a.shape, b.shape)
+    bad = np.argwhere(a != b)
+    assert bad.size == 0, f"{what}: {len(bad)} mismatches, first at {bad[:5].tolist()}"
+
+
+def test_mfma_fragment_layout():
+    rng = np.random.default_rng(0)
+    A = rng.integers(-128, 128, (32, 32)).astype(np.int8)
+    B = rng.integers(-128, 128, (32, 32)).astype(np.int8)
+    A[0, :] = -128
+    B[:, 0] = -128
+    dA, dB = cu(A), cu(B)
+    D = torch.empty((32, 32), dtype=torch.int32, device="cuda")
+    _lib.call("sdpv_mfma_selftest", _lib.ptr(dA), _lib.ptr(dB), _lib.ptr(D), _lib.stream())
+    _eq(host(D), A.astype(np.int64) @ B.astype(np.int64), "mfma")
+
+
+def _rand_words(rng, L, n):
+    lim = 1 << (32 * L - 1)
+    vals = [0, 1, -1, lim - 1, -lim] + [rng.randrange(-lim, lim) >> rng.randrange(0, 32 * L) for _ in range(n - 5)]
+    return vals
+
+
+@pytest.mark.parametrize("Lin,Lout", [(1, 1), (3, 2), (5, 7), (24, 24), (24, 25), (33, 34)])
+def test_shift_round(Lin, Lout):
+    rng = random.Random(Lin * 100 + Lout)
+    n = 3000
+    X = to_planes(_rand_words(rng, Lin, n), Lin)
+    s = np.array([rng.choice([0, 1, -1, 31, 32, 33, 1 << 30, -(1 << 20)]) if i % 5 == 0 else
+                  rng.randrange(-40 * Lout, 40 * Lin) for i in range(n)], np.int64)
+    ref, rflag = O.shift_round(X, s, Lout)
+    dX, ds = cu(X), cu(s)
+    out = torch.empty((Lout, n), dtype=torch.int32, device="cuda")
+    flag = torch.zeros(n, dtype=torch.int32, device="cuda")
+    _lib.call("sdpv_shift_round", _lib.ptr(dX), Lin, n, _lib.ptr(ds), _lib.ptr(out), Lout, _lib.ptr(flag), 1,
+              _lib.stream())
+    _eq(host(out), ref, "shift_round")
+    _eq(host(flag), rflag, "overflow flags")
+    _eq(host(limbops.shift_round(dX, ds, Lout)), ref, "shift_round wrapper")
+
+
+@pytest.mark.parametrize("L", [1, 4, 24, 64])
+def test_add_same_bitlen(L):
+    rng = random.Random(L)
+    a, b = _rand_words(rng, L, 2000), _rand_words(rng, L, 2000)
+    A, B = to_planes(a, L), to_planes(b, L)
+    for sgn in (1, -1):
+        _eq(host(limbops.add_same(cu(A), cu(B), sgn)), O.add_same(A, B, sgn), f"add_same {sgn}")
+    _eq(host(limbops.bitlen_words(cu(A))), O.bitlen(A), "bitlen")
+
+
+@pytest.mark.parametrize("L", [1, 4, 28, 34])
+@pytest.mark.parametrize("uniform", [False, True])
+def test_badd_align(L, uniform):
+    rng = random.Random(19 + L)
+    nb, rows, cols = 3, 5, 10
+    lim = 1 << (32 * L - 1)
+    a = [rng.randrange(-lim, lim) for _ in range(nb * rows * cols)]
+    b = list(reversed(a))
+    a[:5] = [0, 1, -1, lim - 1, -lim]
+    A = to_planes(a, L).reshape(L, nb, rows, cols)
+    B = to_planes(b, L).reshape(L, nb, rows, cols)
+    ea = np.full((nb, cols), -100, np.int64)
+    eb = ea.copy() if uniform else ea + np.array([0, 1, 31, 32, 33, 63, 64, 65, -10000, 10000])[None, :]
+    for sgn in (1, -1):
+        ref, rEO = O.badd_align(A, B, ea, eb, sgn)
+        out = torch.empty((L, nb, rows, cols), dtype=torch.int32, device="cuda")
+        EO = torch.empty((nb, cols), dtype=torch.int64, device="cuda")
+        dA, dB, dea, deb = cu(A), cu(B), cu(ea), cu(eb)
+        _lib.call("sdpv_badd_align", _lib.ptr(dA), _lib.ptr(dB), _lib.ptr(dea), _lib.ptr(deb), _lib.ptr(out),
+                  _lib.ptr(EO), L, nb, rows, cols, sgn, _lib.stream())
+        _eq(host(out), ref, "badd")
+        _eq(host(EO), rEO, "badd exps")
+
+
+@pytest.mark.parametrize("L,LI", [(1, 3), (2, 2), (10, 24), (24, 54), (26, 160)])
+def test_mpf_norm(L, LI):
+    rng = random.Random(L * 7 + LI)
+    vals = _rand_words(rng, LI, 3000)
+    X = to_planes(vals, LI)
+    ein = np.array([rng.randrange(-5000, 5000) for _ in vals], np.int64)
+    rm, re_ = O.mpf_norm(X, ein, L)
+    r = F.normalize(cu(X), cu(ein), L)
+    _eq(host(r.m), rm, "norm m")
+    _eq(host(r.e), re_, "norm e")
+
+
+@pytest.mark.parametrize("L", [1, 2, 10, 24, 26, 34, 64])
+def test_mpf_mul_add_top(L):
+    rng = random.Random(71 + L)
+    n = 2000
+    a, ea = rand_mpf_ints(rng, n, L)
+    b, eb = rand_mpf_ints(rng, n, L)
+    b[30], eb[30] = -a[30], ea[30]
+    A, B = to_planes(a, L), to_planes(b, L)
+    ea, eb = np.array(ea, np.int64), np.array(eb, np.int64)
+    ea[7] = EZ
+    dA, dB = F.MPF(cu(A), cu(ea)), F.MPF(cu(B), cu(eb))
+    rm, re_ = O.mpf_mul(A, ea, B, eb)
+    r = F.mul(dA, dB)
+    _eq(host(r.m), rm, "mul m")
+    _eq(host(r.e), re_, "mul e")
+    rs = F.mul(dA, F.MPF(cu(B[:, 3:4]), cu(eb[3:4])))
+    rm, re_ = O.mpf_mul(A, ea, B[:, 3:4], eb[3:4])
+    _eq(host(rs.m), rm, "mul scalar")
+    _eq(host(rs.e), re_, "mul scalar e")
+    for sgn in (1, -1):
+        rm, re_ = O.mpf_add(A, ea, B, eb, sgn)
+        r = F.add(dA, dB, sgn)
+        _eq(host(r.m), rm, f"add m {sgn}")
+        _eq(host(r.e), re_, f"add e {sgn}")
+    _eq(host(F._top(dA)), O.mpf_top(A, ea), "top")
+
+
+@pytest.mark.parametrize("L", [1, 10, 28, 34])
+@pytest.mark.parametrize("K", [17, 65])
+def test_residues(L, K):
+    # tests/test_rns_kernels.py:28-52 shapes, against the oracle
+    rng = random.Random(123 + L + K)
+    lim = 1 << (32 * L - 1)
+    rows = 3
+    values = [[rng.randrange(-lim, lim) for _ in range(K)] for _ in range(rows)]
+    values[0][3:8] = [0, 1, -1, -lim, lim - 1]
+    X = np.stack([to_planes(r, L) for r in values], axis=1)
+    R = mpmat.ctx(L)
+    for g0, G in ((0, R.n), (R.n - 16, 16), (5, 40)):
+        hi, lo = R.residues(cu(X), 3, K - 5, g0, G)
+        got = 128 * host(hi).astype(np.int32) + host(lo)
+        ref = O.residues(X, 3, K - 5, np.array(R.mods[g0:g0 + G]))
+        _eq(got[:, :, :K - 5], ref, f"residues g0={g0}")
+        assert (got[:, :, K - 5:] == 0).all()
+
+
+@pytest.mark.parametrize("L", [1, 10, 24, 34, 130])
+@pytest.mark.parametrize("width", [1, 5, 11])
+def test_crt_block(L, width):
+    # tests/test_rns_kernels.py:55-80 (L=130: n > 512 primes, the wide digit path)
+    R = mpmat.ctx(L)
+    rng = random.Random(41)
+    rows, N, start = 5, 13, 2
+    limit = R.M // 4
+    vals = [[rng.randrange(-limit + 1, limit) for _ in range(N)] for _ in range(rows)]
+    vals[0][2:7] = [0, 1, -1, limit - 1, -limit + 1]
+    vals[1][2:7] = [1 << b for b in (0, 31, 32, 63, 64)]
+    vals[2][2:7] = [-(1 << b) for b in (0, 31, 32, 63, 64)]
+    acc = np.array([[[v % p for v in row] for row in vals] for p in R.mods], np.int16)
+    W, bl, Bq = R.crt_block(cu(acc), N, start, width, rows)
+    assert Bq == rows * width
+    exp = [v for row in vals for v in row[start:start + width]]
+    sel = acc[:, :, start:start + width].reshape(R.n, -1)
+    rW, rbl = O.crt(sel, np.array(R.mods), R.E // 4)
+    _eq(host(W), rW, "crt words")
+    _eq(host(bl), rbl, "crt bitlen")
+    assert from_planes(host(W)) == exp
+
+
+def test_product_accumulate():
+    """RNS.accumulate (three INT8 products + modular combine) against numpy int64"""
+    L = 24
+    R = mpmat.ctx(L)
+    rng = np.random.default_rng(5)
+    for (M, N, K) in ((7, 5, 30), (64, 64, 64), (130, 70, 200), (300, 257, 1000)):
+        G = R.n
+        m = np.array(R.mods, np.int64)[:, None, None]
+        ra = rng.integers(-8128, 8129, (G, M, K))
+        rb = rng.integers(-8128, 8129, (G, N, K))
+        ra = np.where(np.abs(ra) <= (m - 1) // 2, ra, 0)
+        rb = np.where(np.abs(rb) <= (m - 1) // 2, rb, 0)
+        kp = R.kp(K)
+        pad = lambda r: np.pad(r, ((0, 0), (0, 0), (0, kp - K)))
+        sp = lambda r: (((r + 64) & 127) - 64, (r - (((r + 64) & 127) - 64)) >> 7)
+        la, ha = sp(pad(ra))
+        lb, hb = sp(pad(rb))
+        acc0 = rng.integers(0, 1 << 14, (G, M, N)) % m
+        acc = cu(acc0.astype(np.int16))
+        R.accumulate(cu(ha.astype(np.int8)), cu(la.astype(np.int8)), cu(hb.astype(np.int8)),
+                     cu(lb.astype(np.int8)), acc)
+        ref = (acc0 + np.einsum("gik,gjk->gij", ra, rb)) % m
+        _eq(host(acc).astype(np.int64), ref, f"product {M}x{N}x{K}")
+
+
+def _gemm_case(L, nb, M, K, N, transA, transB, seed):
+    rng = random.Random(seed)
+    shpA = (nb, K, M) if transA else (nb, M, K)
+    shpB = (nb, N, K) if transB else (nb, K, N)
+    a, ea = rand_mpf_ints(rng, int(np.prod(shpA)), L, 60)
+    b, eb = rand_mpf_ints(rng, int(np.prod(shpB)), L, 60)
+    rng.shuffle(a)
+    Am = to_planes(a, L).reshape((L,) + shpA)
+    Bm = to_planes(b, L).reshape((L,) + shpB)
+    Ae = np.array(ea, np.int64).reshape(shpA)
+    Be = np.array(eb, np.int64).reshape(shpB)
+    Ae[np.asarray(a).reshape(shpA) == 0] = EZ
+    Be[np.asarray(b).reshape(shpB) == 0] = EZ
+    Ae.reshape(-1)[3] = EZ                            # an EZ entry with nonzero limbs: ignored
+    if M > 2:
+        Am[:, 0, 1] = 0                               # an all-zero row of op(A) when not transposed
+        Ae[0, 1] = EZ
+    return Am, Ae, Bm, Be
+
+
+@pytest.mark.parametrize("L,nb,M,K,N", [(1, 1, 3, 4, 5), (2, 1, 33, 70, 31), (10, 2, 40, 33, 70),
+                                         (24, 1, 130, 200, 129), (26, 3, 17, 64, 65), (34, 1, 64, 65, 64)])
+@pytest.mark.parametrize("transA,transB", [(False, False), (True, False), (False, True), (True, True)])
+def test_mpf_gemm(L, nb, M, K, N, transA, transB):
+    Am, Ae, Bm, Be = _gemm_case(L, nb, M, K, N, transA, transB, L * 7919 + M + 2 * transA + transB)
+    rm, re_ = O.mpf_gemm(Am, Ae, Bm, Be, transA, transB)
+    C = F.gemm(F.MPF(cu(Am), cu(Ae)), F.MPF(cu(Bm), cu(Be)), transA, transB)
+    _eq(host(C.m), rm, "gemm m")
+    _eq(host(C.e), re_, "gemm e")
+    if nb == 1:
+        C2 = F.gemm(F.MPF(cu(Am[:, 0]), cu(Ae[0])), F.MPF(cu(Bm[:, 0]), cu(Be[0])), transA, transB)
+        _eq(host(C2.m), rm[:, 0], "gemm 2d m")
+
+
+def test_mpf_gemm_512_L24():
+    """The 512 x 512 x 512 product at 768 bits (L = 24) on Gaussian inputs, bit-exact."""
+    L, n = 24, 512
+    Am, Ae = gaussian_mpf((1, n, n), L, seed=0)
+    Bm, Be = gaussian_mpf((1, n, n), L, seed=1)
+    C = F.gemm(F.MPF(cu(Am), cu(Ae)), F.MPF(cu(Bm), cu(Be)))
+    rm, re_ = O.mpf_gemm(Am, Ae, Bm, Be)
+    _eq(host(C.e), re_, "gemm512 e")
+    _eq(host(C.m), rm, "gemm512 m")
+
+
+@pytest.mark.parametrize("mode", ["col", "mat", -1000, "fixed"])
+def test_gemm_nt_modes(mode):
+    # tests/test_gemm.py:11-26 shapes at L=34
+    L = 34
+    rng = random.Random(1)
+    for (M, N, K) in [(5, 7, 3), (40, 33, 70), (64, 64, 300), (130, 20, 1000)]:
+        A = [[rng.getrandbits(32 * L - 40) * rng.choice([-1, 1]) for _ in range(K)] for _ in range(M)]
+        B = [[rng.getrandbits(32 * L - 40) * rng.choice([-1, 1]) for _ in range(K)] for _ in range(N)]
+        eA, eB = -1000, -1060
+        Am, Bm = mpmat.from_ints(A, eA, L), mpmat.from_ints(B, eB, L)
+        out_exp = (eA + eB + 700) if mode == "fixed" else mode
+        C = mpmat.gemm_nt(Am, Bm, out_exp=out_exp)
+        Ci, ce = mpmat.to_ints(C)
+        W = O.gemm_exact(host(Am.limbs), host(Bm.limbs), L, (2 * 32 * L + 64) // 32 + 2)
+        exact = [[v for v in from_planes(W.reshape(W.shape[0], -1))[i * N:(i + 1) * N]] for i in range(M)]
+        for i in range(M):
+            for j in range(N):
+                assert exact[i][j] == sum(A[i][k] * B[j][k] for k in range(K))
+                s = ce[j] - (eA + eB)
+                ref = (exact[i][j] + (1 << (s - 1))) >> s if s > 0 else exact[i][j] << -s
+                if mode == "mat":
+                    assert abs(Ci[i][j] - ref) <= 1
+                else:
+                    assert Ci[i][j] == ref, (M, N, K, i, j)
