@@ -36,6 +36,10 @@ extern "C" {
 /* Library version; bumped whenever an entry point changes. */
 int sdpv_version(void);
 
+/* Hash of this header and of the kernel sources the library was built from (the build passes it
+ * in). Loaders compare it with the sources they ship, so a stale binary is refused. */
+uint64_t sdpv_source_hash(void);
+
 /* ---------------------------------------------------------------- limb bookkeeping */
 
 /* out = round_half_up(X / 2^s) per entry as Lout limbs (s <= 0: exact left shift).

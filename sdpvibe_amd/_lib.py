@@ -15,6 +15,7 @@ LIB_PATH = os.path.join(HERE, "libsdpvibe_hip.so")
 _P, _I64, _I32 = ctypes.c_void_p, ctypes.c_int64, ctypes.c_int32
 SIGNATURES = {
     "sdpv_version": ([], _I32),
+    "sdpv_source_hash": ([], ctypes.c_uint64),
     "sdpv_shift_round": ([_P, _I64, _I64, _P, _P, _I64, _P, _I32, _P], _I32),
     "sdpv_add_same": ([_P, _P, _P, _I64, _I64, _I32, _P], _I32),
     "sdpv_bitlen": ([_P, _I64, _I64, _P, _P], _I32),
@@ -54,6 +55,10 @@ def lib():
             fn.argtypes, fn.restype = args, res
         if handle.sdpv_version() != EXPECTED_VERSION:
             raise HipLibraryError("libsdpvibe_hip.so is stale: rebuild it")
+        from .build import source_hash
+        if handle.sdpv_source_hash() != source_hash():
+            raise HipLibraryError("libsdpvibe_hip.so was built from other sources than the ones shipped "
+                                  "with it: rebuild it with `python -m sdpvibe_amd.build`")
         _lib = handle
     return _lib
 

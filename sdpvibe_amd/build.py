@@ -6,6 +6,7 @@ Every csrc/*.hip file is compiled to an object in parallel, then linked into one
 next to this file. Always recompiles (the build check must exercise the compiler).
 """
 import concurrent.futures as cf
+import hashlib
 import os
 import subprocess
 import sys
@@ -23,8 +24,20 @@ FLAGS = [
 ]
 
 
+def source_hash():
+    """64-bit hash of the C ABI header and every kernel source (the library embeds it)."""
+    h = hashlib.sha256()
+    files = [os.path.join(os.path.dirname(HERE), "include", "sdpvibe_hip.h")]
+    files += sorted(os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith((".hip", ".hpp")))
+    for f in files:
+        h.update(os.path.basename(f).encode())
+        with open(f, "rb") as fh:
+            h.update(fh.read())
+    return int(h.hexdigest()[:16], 16)
+
+
 def _compile(src, obj):
-    cmd = [HIPCC, *FLAGS, "-c", src, "-o", obj]
+    cmd = [HIPCC, *FLAGS, f"-DSDPV_SOURCE_HASH={source_hash():#x}ULL", "-c", src, "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     err = "\n".join(l for l in (r.stderr or "").splitlines() if "packed-fp32-ops" not in l)
     if r.returncode != 0:

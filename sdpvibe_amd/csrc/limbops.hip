@@ -293,6 +293,11 @@ extern "C" {
 
 int sdpv_version(void) { return 1; }
 
+#ifndef SDPV_SOURCE_HASH
+#error "build with -DSDPV_SOURCE_HASH=<hash of the sources> (sdpvibe_amd/build.py does)"
+#endif
+uint64_t sdpv_source_hash(void) { return (uint64_t)SDPV_SOURCE_HASH; }
+
 int sdpv_shift_round(const int32_t* X, int64_t Lin, int64_t B, const int64_t* s, int32_t* out, int64_t Lout,
                      int32_t* flag, int32_t per_entry, void* stream) {
     if (B < 0 || Lin < 1 || Lout < 1) return SDPV_EARG;

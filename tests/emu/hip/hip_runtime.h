@@ -1,0 +1,58 @@
+// Host emulation of the small HIP surface used by sdpvibe_amd/csrc (CPU testing only).
+// Each kernel block runs as blockDim std::threads with real barriers; MFMA and cross-lane
+// shuffles are emulated per 64-lane wave. Built with AddressSanitizer, it turns an
+// out-of-bounds global access of a kernel into a host report instead of a GPU fault.
+#pragma once
+#include <barrier>
+#include <cmath>
+#include <cstddef>
+#include <cstdint>
+#include <cstring>
+#include <functional>
+#include <thread>
+#include <vector>
+
+#define __global__
+#define __device__
+#define __host__
+#define __forceinline__ inline
+#define __launch_bounds__(...)
+#define __shared__
+#define __align__(x) __attribute__((aligned(x)))
+
+struct dim3 {
+    unsigned x, y, z;
+    dim3(unsigned a = 1, unsigned b = 1, unsigned c = 1) : x(a), y(b), z(c) {}
+};
+typedef void* hipStream_t;
+typedef int hipError_t;
+#define hipSuccess 0
+#define hipFuncAttributeMaxDynamicSharedMemorySize 8
+inline hipError_t hipGetLastError() { return hipSuccess; }
+inline hipError_t hipFuncSetAttribute(const void*, int, int) { return hipSuccess; }
+
+extern thread_local dim3 threadIdx, blockIdx, blockDim, gridDim;
+void emu_syncthreads();
+inline void __syncthreads() { emu_syncthreads(); }
+inline int __clz(int x) { return x == 0 ? 32 : __builtin_clz((unsigned)x); }
+inline int atomicOr(int* p, int v) { return __atomic_fetch_or(p, v, __ATOMIC_SEQ_CST); }
+long long emu_shfl_xor64(long long v, int mask);
+inline long long __shfl_xor(long long v, int mask, int width = 64) { (void)width; return emu_shfl_xor64(v, mask); }
+inline long __shfl_xor(long v, int mask, int width = 64) { (void)width; return (long)emu_shfl_xor64(v, mask); }
+
+typedef int emu_v4i __attribute__((ext_vector_type(4)));
+typedef int emu_v16i __attribute__((ext_vector_type(16)));
+emu_v16i emu_mfma_i32_32x32x32_i8(emu_v4i a, emu_v4i b, emu_v16i c, int, int, int);
+#define __builtin_amdgcn_mfma_i32_32x32x32_i8 emu_mfma_i32_32x32x32_i8
+
+// Dynamic shared memory of the block being run. The kernels declare their extern arrays inside
+// an anonymous namespace, so each translation unit gets its own definitions here.
+namespace {
+alignas(64) unsigned char smem[160 * 1024];
+alignas(64) int8_t smem8[160 * 1024];
+alignas(64) uint32_t lds[40 * 1024];
+}  // namespace
+void emu_run_grid(dim3 grid, dim3 block, size_t lds_bytes, const std::function<void()>& body);
+
+#define hipLaunchKernelGGL(kernel, grid, block, ldsb, stream, ...) \
+    emu_run_grid((grid), (block), (ldsb), [&]() { kernel(__VA_ARGS__); })

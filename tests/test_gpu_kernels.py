@@ -151,7 +151,8 @@ def test_residues(L, K):
     values[0][3:8] = [0, 1, -1, -lim, lim - 1]
     X = np.stack([to_planes(r, L) for r in values], axis=1)
     R = mpmat.ctx(L)
-    for g0, G in ((0, R.n), (R.n - 16, 16), (5, 40)):
+    groups = {(0, R.n), (max(0, R.n - 16), min(16, R.n)), (min(5, R.n - 1), min(40, R.n - min(5, R.n - 1)))}
+    for g0, G in sorted(groups):
         hi, lo = R.residues(cu(X), 3, K - 5, g0, G)
         got = 128 * host(hi).astype(np.int32) + host(lo)
         ref = O.residues(X, 3, K - 5, np.array(R.mods[g0:g0 + G]))
