@@ -1,0 +1,167 @@
+"""Benchmark: exact multiprecision GEMM (mpf.gemm) 512 x 512 x 512 at 768 bits (L = 24 limbs) on one
+MI355X -- the product of the plain-kernel configuration (Gaussian mantissas, exponents in
+[-16, 16], seed 0). Prints one JSON line.
+
+    python bench.py [--steps K] [--warmup W] [--gpus 1]
+
+value      : INT8 operations of the whole mp-GEMM (split + products + CRT, as executed by this
+             algorithm, padding excluded) per second, as % of MI355X dense INT8 MFMA peak
+roofline   : the dominant kernel of one step, timed with HIP events on its launch stream
+cpu_baseline: the reference's own CPU path (cpu_baseline/mpf_baseline.c, GMP mpf naive loops,
+             compiled from source into oracle/_ref) on the same 512 x 512 x 512 768-bit shape
+"""
+import argparse
+import json
+import os
+import re
+import subprocess
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_INT8_TOPS = 5000.0       # MI355X dense INT8 MFMA, TOP/s
+PEAK_HBM_GBS = 8000.0         # MI355X HBM3E, GB/s
+METRIC = "s/IPM iteration @768-bit N=3000 SDP (batched vs plain); mp-GEMM % INT8 MFMA peak"
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--n", type=int, default=512)
+    ap.add_argument("--L", type=int, default=24)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
+    return ap.parse_args()
+
+
+def cpu_baseline(n, threads, L):
+    """The reference's CPU path on the same shape: mpf_baseline <prec> <n> <nrhs> <threads>
+    (GMP mpf at 32L bits; its GEMM line is an n x n x n product)."""
+    exe = os.path.join(ROOT, "oracle", "_ref", "mpf_baseline")
+    if not os.path.exists(exe):
+        return None
+    prec = 32 * L
+    t0 = time.time()
+    r = subprocess.run([exe, str(prec), str(n), str(n), str(threads)], capture_output=True, text=True,
+                       timeout=600)
+    wall = time.time() - t0
+    m = re.search(r"gemm\s+([\d.]+) s\s+([\d.]+) M mp-MAC/s", r.stdout)
+    if r.returncode != 0 or not m:
+        return None
+    return {"value": float(m.group(2)) * 1e6, "unit": "mp-MAC/s (768-bit GEMM)", "cores": threads,
+            "kind": "reference",
+            "sample": f"cpu_baseline/mpf_baseline.c {prec} {n} {n} {threads}: GMP mpf naive GEMM "
+                      f"{n}x{n}x{n} ({float(m.group(1)):.2f} s; Cholesky+TRSM of the same run not counted), "
+                      f"{wall:.1f} s wall in total"}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (no multi-GPU path "
+              f"in this build)", file=sys.stderr)
+        sys.exit(2)
+    import torch
+    from sdpvibe_amd import mpf as F
+    from sdpvibe_amd import mpmat
+    from sdpvibe_amd.synthetic import gaussian_mpf
+
+    torch.cuda.set_device(0)
+    n, L = args.n, args.L
+    M = N = K = n
+    Am, Ae = gaussian_mpf((1, n, n), L, seed=0)
+    Bm, Be = gaussian_mpf((1, n, n), L, seed=1)
+    A = F.MPF(torch.from_numpy(Am).cuda(), torch.from_numpy(Ae).cuda())
+    B = F.MPF(torch.from_numpy(Bm).cuda(), torch.from_numpy(Be).cuda())
+    R = mpmat.ctx(L)
+    for _ in range(args.warmup):
+        F.gemm(A, B)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        C = F.gemm(A, B)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / args.steps
+    del C
+
+    # ---- per-phase times with HIP events on the launch stream (torch's current stream)
+    st = torch.cuda.current_stream()
+    nb = 1
+    kp = R.kp(K)
+    phases = ["row_exp", "split_A", "split_B", "product", "crt"]
+    acc_ms = {p: 0.0 for p in phases}
+    reps = max(5, args.steps // 2)
+    for rep in range(reps + 2):
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(len(phases) + 1)]
+        ev[0].record(st)
+        EA = F._row_exp(A.m, A.e, L, A.e.numel(), 1, M, K, M * K, K, 1)
+        EB = F._row_exp(B.m, B.e, L, B.e.numel(), 1, N, K, K * N, 1, N)
+        ev[1].record(st)
+        RhA = torch.empty((R.n, M, kp), dtype=torch.int8, device="cuda")
+        RlA = torch.empty_like(RhA)
+        R.split(A.m, A.e, EA, L, A.e.numel(), 1, M, M * K, K, 1, 0, K, kp, RhA, RlA)
+        ev[2].record(st)
+        RhB = torch.empty((R.n, N, kp), dtype=torch.int8, device="cuda")
+        RlB = torch.empty_like(RhB)
+        R.split(B.m, B.e, EB, L, B.e.numel(), 1, N, K * N, 1, N, 0, K, kp, RhB, RlB)
+        ev[3].record(st)
+        acc = torch.empty((R.n, M, N), dtype=torch.int16, device="cuda")
+        R.product(RhA, RlA, RhB, RlB, acc, 1, M, N, M * N)
+        ev[4].record(st)
+        om = torch.empty((L, 1, M, N), dtype=torch.int32, device="cuda")
+        oe = torch.empty((1, M, N), dtype=torch.int64, device="cuda")
+        R.crt(acc, M, N, 0, N, 1, outm=om, oute=oe, L=L, plane=M * N, ldo=N, jo=0, Mb=M, erow=EA, ecol=EB)
+        ev[5].record(st)
+        torch.cuda.synchronize()
+        if rep >= 2:
+            for i, p in enumerate(phases):
+                acc_ms[p] += ev[i].elapsed_time(ev[i + 1])
+    ph = {p: acc_ms[p] / reps for p in phases}
+
+    # ---- algorithmic work (this algorithm: 3 INT8 products per prime; padding excluded)
+    nP, E = R.n, R.E
+    ops_split = 2 * (2 * nP * 4 * L * (M + N) * K)       # T_hi and T_lo byte dots per entry and prime
+    ops_prod = 2 * (3 * nP * M * N * K)
+    ops_crt = 2 * (2 * E * nP * M * N)                     # hi and lo digit products
+    ops_total = ops_split + ops_prod + ops_crt
+    ref_ops = 2 * (4 * 112 * M * N * K + 2 * 112 * 4 * L * (M + N) * K + 2 * 216 * 112 * M * N)
+    value = 100.0 * ops_total / dt / (PEAK_INT8_TOPS * 1e12)
+
+    dom = max(phases, key=lambda p: ph[p])
+    t_dom = ph[dom] * 1e-3
+    if dom == "product":
+        ach = ops_prod / t_dom / 1e12
+        roof = {"bound": "mfma", "achieved": ach, "peak": PEAK_INT8_TOPS, "unit": "TOP/s",
+                "frac": ach / PEAK_INT8_TOPS, "traffic": None}
+    else:
+        per_entry = {"split_A": (4 * L + 8 + 2 * nP) * M * K, "split_B": (4 * L + 8 + 2 * nP) * N * K,
+                     "crt": (2 * nP + 4 * L + 8) * M * N, "row_exp": 16 * (M + N) * K}[dom]
+        ach = per_entry / t_dom / 1e9
+        roof = {"bound": "hbm", "achieved": ach, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                "frac": ach / PEAK_HBM_GBS, "traffic": None}
+    roof["kernel"] = {"product": "k_rns_gemm", "split_A": "k_rns_split", "split_B": "k_rns_split",
+                      "crt": "k_rns_crt", "row_exp": "k_row_exp_*"}[dom]
+    roof["avg_launch_ms"] = ph[dom]
+
+    cpu = None if args.no_cpu_baseline else cpu_baseline(n, args.cpu_threads, L)
+    line = {
+        "metric": METRIC, "value": round(value, 3), "unit": "% of dense INT8 MFMA peak (mp-GEMM)",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": dt * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "int8 (exact 768-bit)",
+        "data": "synthetic (host PCG64 Gaussian mantissas, exponents in [-16, 16], seed 0/1)",
+        "config": {"workload": f"mpf.gemm {M}x{K}x{N} at {32 * L}-bit (L={L}), plain-kernel configuration",
+                   "L": L, "primes": nP, "crt_digits": E},
+        "mp_mac_per_s": M * N * K / dt, "int8_ops_per_step": ops_total,
+        "pct_peak_reference_op_count": round(100.0 * ref_ops / dt / (PEAK_INT8_TOPS * 1e12), 3),
+        "phase_ms": {p: round(v, 4) for p, v in ph.items()},
+        "roofline": roof, "cpu_baseline": cpu,
+    }
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

@@ -16,6 +16,11 @@ typedef int v4i __attribute__((ext_vector_type(4)));
 typedef int v16i __attribute__((ext_vector_type(16)));
 
 #define EZV (-(1LL << 40))
+
+// Dynamic LDS of a kernel (the host emulator in tests/emu supplies its own definition).
+#ifndef SDPV_DYNAMIC_LDS
+#define SDPV_DYNAMIC_LDS(type, name) extern __shared__ __align__(16) type name[]
+#endif
 #define SDPV_TRY_LAUNCH()                                   \
     do {                                                    \
         hipError_t err__ = hipGetLastError();               \
@@ -49,10 +54,11 @@ __device__ __forceinline__ int shr_round(const Src& src, int ns, i64 s, int Lout
         carry = (ext_word(src, ns, ext, sb >> 5) >> (int)(sb & 31)) & 1u;
     }
     u32 prev = 0;
+    u64 lo = ext_word(src, ns, ext, q);
     for (int k = 0; k <= Lout; k++) {
-        const u64 lo = ext_word(src, ns, ext, q + k);
-        const u64 hi = ext_word(src, ns, ext, q + k + 1);
+        const u64 hi = ext_word(src, ns, ext, q + k + 1);   // each source word is read once
         const u64 v = (((lo | (hi << 32)) >> r) & 0xffffffffull) + carry;
+        lo = hi;
         carry = v >> 32;
         const u32 w = (u32)v;
         if (k < Lout) {

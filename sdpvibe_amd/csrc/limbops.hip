@@ -128,7 +128,7 @@ __global__ void __launch_bounds__(MUL_TPB) k_mpf_mul(const i32* __restrict__ A, 
                                                      const i32* __restrict__ Bm, const i64* __restrict__ eB,
                                                      i32* __restrict__ O, i64* __restrict__ eO, int L, i64 n,
                                                      int bscalar) {
-    extern __shared__ u32 lds[];
+    SDPV_DYNAMIC_LDS(u32, lds);
     const int t = threadIdx.x;
     const i64 i = blockIdx.x * (i64)MUL_TPB + t;
     if (i >= n) return;
@@ -177,7 +177,7 @@ __global__ void __launch_bounds__(ADD_TPB) k_mpf_add(const i32* __restrict__ A, 
                                                      const i32* __restrict__ Bm, const i64* __restrict__ eB,
                                                      i32* __restrict__ O, i64* __restrict__ eO, int L, i64 n,
                                                      int sgn) {
-    extern __shared__ u32 lds[];
+    SDPV_DYNAMIC_LDS(u32, lds);
     const int t = threadIdx.x;
     const i64 i = blockIdx.x * (i64)ADD_TPB + t;
     if (i >= n) return;
@@ -261,6 +261,33 @@ __global__ void __launch_bounds__(256) k_row_exp_wave(const i32* __restrict__ X,
         mx = v > mx ? v : mx;
     }
     if (lane == 0) {
+        i64 E = mx - (32 * (i64)L - 2);
+        if (E < EZV / 2) E = 0;
+        Erow[R] = E;
+    }
+}
+
+// rows consecutive in memory (ldr == 1): 64 rows (lanes) x 16 groups of k per block, LDS max
+__global__ void __launch_bounds__(1024) k_row_exp_cols(const i32* __restrict__ X, const i64* __restrict__ e, int L,
+                                                       i64 plane, i64 nrows, i64 rows, i64 K, i64 ldb, i64 ldk,
+                                                       i64* __restrict__ Erow) {
+    __shared__ i64 part[16][64];
+    const int lane = threadIdx.x & 63, kg = threadIdx.x >> 6;
+    const i64 R = blockIdx.x * (i64)64 + lane;
+    i64 mx = EZV;
+    if (R < nrows) {
+        const i64 b = R / rows, r = R % rows;
+        const i64 base = b * ldb + r;
+#pragma unroll 4
+        for (i64 k = kg; k < K; k += 16) {
+            const i64 t = entry_top(X, e, L, plane, base + k * ldk);
+            mx = t > mx ? t : mx;
+        }
+    }
+    part[kg][lane] = mx;
+    __syncthreads();
+    if (kg == 0 && R < nrows) {
+        for (int g = 1; g < 16; g++) mx = part[g][lane] > mx ? part[g][lane] : mx;
         i64 E = mx - (32 * (i64)L - 2);
         if (E < EZV / 2) E = 0;
         Erow[R] = E;
@@ -396,6 +423,9 @@ int sdpv_row_exp(const int32_t* X, const int64_t* e, int64_t L, int64_t plane, i
     if (ldk == 1) {
         hipLaunchKernelGGL(k_row_exp_wave, dim3(nblocks(nrows, 4)), dim3(256), 0, as_stream(stream), X, e, (int)L,
                            (i64)plane, nrows, (i64)rows, (i64)K, (i64)ldb, (i64)ldr, Erow);
+    } else if (ldr == 1) {
+        hipLaunchKernelGGL(k_row_exp_cols, dim3(nblocks(nrows, 64)), dim3(1024), 0, as_stream(stream), X, e,
+                           (int)L, (i64)plane, nrows, (i64)rows, (i64)K, (i64)ldb, (i64)ldk, Erow);
     } else {
         hipLaunchKernelGGL(k_row_exp_thread, dim3(nblocks(nrows, 256)), dim3(256), 0, as_stream(stream), X, e,
                            (int)L, (i64)plane, nrows, (i64)rows, (i64)K, (i64)ldb, (i64)ldr, (i64)ldk, Erow);

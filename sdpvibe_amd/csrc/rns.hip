@@ -53,7 +53,7 @@ __global__ void __launch_bounds__(256) k_rns_split(
     const int8_t* __restrict__ Tlo, int NBp, const i32* __restrict__ C1, const i32* __restrict__ C2,
     const i32* __restrict__ mods, const double* __restrict__ minv, int g0, int G, int8_t* __restrict__ Rh,
     int8_t* __restrict__ Rl, i32* __restrict__ flag, int EPW, int TK, i64 tiles_k) {
-    extern __shared__ __align__(16) unsigned char smem[];
+    SDPV_DYNAMIC_LDS(unsigned char, smem);
     const int XS = NBp + 16;
     unsigned char* xs = smem;                           // EPW x XS bytes
     unsigned char* negs = xs + (size_t)EPW * XS;        // EPW sign flags
@@ -160,7 +160,7 @@ __global__ void __launch_bounds__(WM* WN * 64) k_rns_gemm(
     constexpr int BM = WM * BLM * 32, BN = WN * BLN * 32;
     constexpr int RS = 80;
     constexpr int ASZ = BM * RS, BSZ = BN * RS;
-    extern __shared__ __align__(16) int8_t smem8[];
+    SDPV_DYNAMIC_LDS(int8_t, smem8);
     // buffer layout: [stage][plane h,l,s][A tile | B tile]
     auto sA = [&](int st, int p) { return smem8 + (size_t)(st * 3 + p) * (ASZ + BSZ); };
     auto sB = [&](int st, int p) { return smem8 + (size_t)(st * 3 + p) * (ASZ + BSZ) + ASZ; };
@@ -292,10 +292,11 @@ __global__ void __launch_bounds__(WM* WN * 64) k_rns_gemm(
         __syncthreads();
     }
 
+    // Epilogue: v = k0*HH + k1*SS + k2*LL (+ old) is exact in double (|v| < 2^47), reduced once.
     const int g = g0 + gi;
     const int m = mods[g];
-    const double mi = minv[g];
-    const int k0c = kcoef[3 * g], k1c = kcoef[3 * g + 1], k2c = kcoef[3 * g + 2];
+    const double mi = minv[g], md = (double)m;
+    const double k0c = (double)kcoef[3 * g], k1c = (double)kcoef[3 * g + 1], k2c = (double)kcoef[3 * g + 2];
     const bool accum = (mode & SDPV_GEMM_ACCUMULATE) != 0;
     int16_t* out = acc + (i64)g * ld_g + b * M * N;
 #pragma unroll
@@ -307,10 +308,13 @@ __global__ void __launch_bounds__(WM* WN * 64) k_rns_gemm(
             for (int q = 0; q < 16; q++) {
                 const i64 row = r0 + (wm * BLM + i) * 32 + 8 * (q >> 2) + 4 * hsel + (q & 3);
                 if (row < M && col < N) {
-                    int v = k0c * mod_m(cHH[i][j][q], m, mi) + k1c * mod_m(cSS[i][j][q], m, mi) +
-                            k2c * mod_m(cLL[i][j][q], m, mi);
-                    if (accum) v += out[row * N + col];
-                    out[row * N + col] = (int16_t)mod_m(v, m, mi);
+                    double v = fma(k0c, (double)cHH[i][j][q], fma(k1c, (double)cSS[i][j][q], k2c * (double)cLL[i][j][q]));
+                    if (accum) v += (double)out[row * N + col];
+                    const double qt = floor(v * mi);
+                    int r = (int)fma(-qt, md, v);
+                    if (r < 0) r += m;
+                    if (r >= m) r -= m;
+                    out[row * N + col] = (int16_t)r;
                 }
             }
         }
@@ -351,19 +355,23 @@ __global__ void __launch_bounds__(256) k_rns_crt(
     i32* __restrict__ outm, i64* __restrict__ oute, int L, i64 plane, i64 ldo, i64 jo, i64 Mb,
     const i64* __restrict__ erow,
     const i64* __restrict__ ecol, const i64* __restrict__ scol, i32* __restrict__ flag, int QW, int DB) {
-    extern __shared__ __align__(16) unsigned char smem[];
+    SDPV_DYNAMIC_LDS(unsigned char, smem);
     const int YS = npad + 16;
     const int NW = E / 4;
+    const int QWp = QW + 1;                           // odd row stride: conflict-free both ways
+    const int ZD = DB * 32;
     int8_t* Yh = (int8_t*)smem;
     int8_t* Yl = Yh + (size_t)QW * YS;
     double* ksum = (double*)(Yl + (size_t)QW * YS);  // [8][QW]
     i32* kq = (i32*)(ksum + 8 * QW);                  // [QW]
-    i64* Z = (i64*)(kq + QW);                         // [QW][DB*32]
-    u32* Wl = (u32*)(Z + (size_t)QW * DB * 32);       // [QW][NW+1] (modes 1, 2)
+    i32* Mds = kq + QW;                               // [Epad] digits of M
+    i32* Zt = Mds + ((E + 31) / 32) * 32;             // [ZD][QWp]: Z of digit d, entry q
+    u32* Wl = (u32*)(Zt + (size_t)ZD * QWp);          // [QW][NW+1] (modes 1, 2)
     const i64 Bq = rows * w;
     const i64 q0 = blockIdx.x * (i64)QW;
     const int t = threadIdx.x;
 
+    for (int d = t; d < ((E + 31) / 32) * 32; d += 256) Mds[d] = Md[d];
     {
         const int ql = t % QW, grp = t / QW, ngrp = 256 / QW;
         const i64 q = q0 + ql;
@@ -397,7 +405,6 @@ __global__ void __launch_bounds__(256) k_rns_crt(
     const int wave = t >> 6, lane = t & 63, l32 = lane & 31, hsel = lane >> 5;
     const int nEB = QW / 32;
     const int nDB = (E + 31) / 32;
-    const int ZS = DB * 32;
     const int kqt = (t < QW) ? kq[t] : 0;
     const i64 qt = q0 + t;
     const bool qtok = (t < QW) && (qt < Bq);
@@ -418,11 +425,9 @@ __global__ void __launch_bounds__(256) k_rns_crt(
                     zh = mfma(*(const v4i*)(yh + off), bfr, zh);
                     zl = mfma(*(const v4i*)(yl + off), bfr, zl);
                 }
+                i32* zcol = Zt + (size_t)(dbl * 32 + l32) * QWp + eb * 32;
 #pragma unroll
-                for (int j = 0; j < 16; j++) {
-                    const int row = eb * 32 + 8 * (j >> 2) + 4 * hsel + (j & 3);
-                    Z[(size_t)row * ZS + dbl * 32 + l32] = 128 * (i64)zh[j] + (i64)zl[j];
-                }
+                for (int j = 0; j < 16; j++) zcol[8 * (j >> 2) + 4 * hsel + (j & 3)] = 128 * zh[j] + zl[j];
             }
         }
         __syncthreads();
@@ -433,7 +438,7 @@ __global__ void __launch_bounds__(256) k_rns_crt(
 #pragma unroll
                 for (int u = 0; u < 4; u++) {
                     const int d = db0 * 32 + d4 + u;
-                    v += (Z[(size_t)t * ZS + d4 + u] - (i64)kqt * (i64)Md[d]) * ((i64)1 << (8 * u));
+                    v += ((i64)Zt[(size_t)(d4 + u) * QWp + t] - (i64)kqt * (i64)Mds[d]) * ((i64)1 << (8 * u));
                 }
                 const int wi = (db0 * 32 + d4) >> 2;
                 const u32 word = (u32)v;
@@ -566,10 +571,13 @@ int sdpv_rns_crt(const int16_t* acc, int64_t n, int64_t ld_g, int64_t rows, int6
     if (mode != SDPV_CRT_WORDS && (!outm || L < 1 || Mb < 1)) return SDPV_EARG;
     if (mode == SDPV_CRT_NORMALIZE && !oute) return SDPV_EARG;
     if (mode == SDPV_CRT_ROUND && !scol) return SDPV_EARG;
+    if (npad > 1024) return SDPV_ESIZE;  // digit sums 128*Zh + Zl stay within int32
     const int QW = (npad <= 352) ? 64 : 32;
     const int DB = 4 / (QW / 32);
     const int NW = (int)(E / 4);
-    size_t lds = (size_t)2 * QW * (npad + 16) + (size_t)8 * QW * 8 + (size_t)QW * 4 + (size_t)QW * DB * 32 * 8;
+    const i64 Epad = 32 * ((E + 31) / 32);
+    size_t lds = (size_t)2 * QW * (npad + 16) + (size_t)8 * QW * 8 + (size_t)QW * 4 + (size_t)Epad * 4 +
+                 (size_t)DB * 32 * (QW + 1) * 4;
     if (mode != SDPV_CRT_WORDS) lds += (size_t)QW * (NW + 1) * 4;
     if (lds > 160 * 1024) return SDPV_ESIZE;
     const i64 grid = (Bq + QW - 1) / QW;

@@ -6,6 +6,8 @@
 
 thread_local dim3 threadIdx, blockIdx, blockDim, gridDim;
 static constexpr size_t LDS_MAX = 160 * 1024;
+alignas(64) static unsigned char g_lds_store[LDS_MAX];
+unsigned char* emu_dynamic_lds = g_lds_store;
 
 namespace {
 struct Wave {
@@ -65,6 +67,7 @@ void emu_run_grid(dim3 grid, dim3 block, size_t lds_bytes, const std::function<v
     for (unsigned bz = 0; bz < grid.z; bz++)
         for (unsigned by = 0; by < grid.y; by++)
             for (unsigned bx = 0; bx < grid.x; bx++) {
+                memset(g_lds_store, 0xA5, sizeof g_lds_store);   // LDS starts as garbage
                 g_block = std::make_unique<std::barrier<>>(nt);
                 unsigned nw = (nt + 63) / 64;
                 g_waves = std::vector<Wave>(nw);

@@ -17,7 +17,7 @@
 #define __host__
 #define __forceinline__ inline
 #define __launch_bounds__(...)
-#define __shared__
+#define __shared__ static  // static LDS arrays: one copy shared by the block being run
 #define __align__(x) __attribute__((aligned(x)))
 
 struct dim3 {
@@ -45,13 +45,9 @@ typedef int emu_v16i __attribute__((ext_vector_type(16)));
 emu_v16i emu_mfma_i32_32x32x32_i8(emu_v4i a, emu_v4i b, emu_v16i c, int, int, int);
 #define __builtin_amdgcn_mfma_i32_32x32x32_i8 emu_mfma_i32_32x32x32_i8
 
-// Dynamic shared memory of the block being run. The kernels declare their extern arrays inside
-// an anonymous namespace, so each translation unit gets its own definitions here.
-namespace {
-alignas(64) unsigned char smem[160 * 1024];
-alignas(64) int8_t smem8[160 * 1024];
-alignas(64) uint32_t lds[40 * 1024];
-}  // namespace
+// Dynamic shared memory of the block being run (blocks run one at a time).
+extern unsigned char* emu_dynamic_lds;
+#define SDPV_DYNAMIC_LDS(type, name) type* name = reinterpret_cast<type*>(emu_dynamic_lds)
 void emu_run_grid(dim3 grid, dim3 block, size_t lds_bytes, const std::function<void()>& body);
 
 #define hipLaunchKernelGGL(kernel, grid, block, ldsb, stream, ...) \

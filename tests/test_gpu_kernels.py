@@ -263,10 +263,10 @@ def test_gemm_nt_modes(mode):
         B = [[rng.getrandbits(32 * L - 40) * rng.choice([-1, 1]) for _ in range(K)] for _ in range(N)]
         eA, eB = -1000, -1060
         Am, Bm = mpmat.from_ints(A, eA, L), mpmat.from_ints(B, eB, L)
-        out_exp = (eA + eB + 700) if mode == "fixed" else mode
+        out_exp = (eA + eB + 1100) if mode == "fixed" else mode   # product fits 34 limbs
         C = mpmat.gemm_nt(Am, Bm, out_exp=out_exp)
         Ci, ce = mpmat.to_ints(C)
-        W = O.gemm_exact(host(Am.limbs), host(Bm.limbs), L, (2 * 32 * L + 64) // 32 + 2)
+        W = O.gemm_exact(host(Am.limbs), host(Bm.limbs), (2 * 32 * L + 64) // 32 + 2)
         exact = [[v for v in from_planes(W.reshape(W.shape[0], -1))[i * N:(i + 1) * N]] for i in range(M)]
         for i in range(M):
             for j in range(N):
