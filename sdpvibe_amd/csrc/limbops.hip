@@ -235,24 +235,39 @@ __global__ void __launch_bounds__(ADD_TPB) k_mpf_add(const i32* __restrict__ A, 
 }
 
 // ------------------------------------------------------------------ row alignment exponents
-// Erow = max_k top(entry) - NB (0 for an all-zero row), mpf.py:390-395.
+// Erow = max_k top(entry) - NB over entries with e != EZ (mpf.py:390-395), computed as a K-split
+// reduction: a fill pass, then blocks over (rows, k slices) combine with 64-bit atomicMax.
+// Rows without such entries keep SDPV_EZ (the reference then uses exponent 0; no product
+// depends on it because every entry of such a row is zero).
 __device__ __forceinline__ i64 entry_top(const i32* X, const i64* e, int L, i64 plane, i64 off) {
     const i64 ex = e[off];
     if (ex == EZV) return EZV;
     return ex + bitlen_tc(PlaneSrc{X + off, plane}, L);
 }
 
-// rows contiguous in k (ldk == 1): one wave per row
+__global__ void __launch_bounds__(256) k_fill_i64(i64* __restrict__ p, i64 n, i64 v) {
+    const i64 i = blockIdx.x * (i64)256 + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+__device__ __forceinline__ void row_max_commit(i64* Erow, i64 R, i64 mx, int L) {
+    if (mx != EZV) atomicMax((long long*)(Erow + R), (long long)(mx - (32 * (i64)L - 2)));
+}
+
+// rows contiguous in k (ldk == 1): one wave per (row, slice of RK_WAVE entries)
+constexpr int RK_WAVE = 512;
 __global__ void __launch_bounds__(256) k_row_exp_wave(const i32* __restrict__ X, const i64* __restrict__ e, int L,
                                                       i64 plane, i64 nrows, i64 rows, i64 K, i64 ldb, i64 ldr,
                                                       i64* __restrict__ Erow) {
-    const i64 R = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const i64 R = blockIdx.x * (i64)4 + (threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
     if (R >= nrows) return;
     const i64 b = R / rows, r = R % rows;
     const i64 base = b * ldb + r * ldr;
+    const i64 k0 = blockIdx.y * (i64)RK_WAVE, k1 = (k0 + RK_WAVE < K) ? k0 + RK_WAVE : K;
     i64 mx = EZV;
-    for (i64 k = lane; k < K; k += 64) {
+#pragma unroll 4
+    for (i64 k = k0 + lane; k < k1; k += 64) {
         const i64 t = entry_top(X, e, L, plane, base + k);
         mx = t > mx ? t : mx;
     }
@@ -260,26 +275,24 @@ __global__ void __launch_bounds__(256) k_row_exp_wave(const i32* __restrict__ X,
         const i64 v = __shfl_xor(mx, o, 64);
         mx = v > mx ? v : mx;
     }
-    if (lane == 0) {
-        i64 E = mx - (32 * (i64)L - 2);
-        if (E < EZV / 2) E = 0;
-        Erow[R] = E;
-    }
+    if (lane == 0) row_max_commit(Erow, R, mx, L);
 }
 
-// rows consecutive in memory (ldr == 1): 64 rows (lanes) x 16 groups of k per block, LDS max
-__global__ void __launch_bounds__(1024) k_row_exp_cols(const i32* __restrict__ X, const i64* __restrict__ e, int L,
-                                                       i64 plane, i64 nrows, i64 rows, i64 K, i64 ldb, i64 ldk,
-                                                       i64* __restrict__ Erow) {
-    __shared__ i64 part[16][64];
+// other layouts: 64 rows (lanes) x 4 k-groups per block, one slice of RK_COLS entries per block
+constexpr int RK_COLS = 32;
+__global__ void __launch_bounds__(256) k_row_exp_cols(const i32* __restrict__ X, const i64* __restrict__ e, int L,
+                                                      i64 plane, i64 nrows, i64 rows, i64 K, i64 ldb, i64 ldr,
+                                                      i64 ldk, i64* __restrict__ Erow) {
+    __shared__ i64 part[4][64];
     const int lane = threadIdx.x & 63, kg = threadIdx.x >> 6;
     const i64 R = blockIdx.x * (i64)64 + lane;
+    const i64 k0 = blockIdx.y * (i64)RK_COLS, k1 = (k0 + RK_COLS < K) ? k0 + RK_COLS : K;
     i64 mx = EZV;
     if (R < nrows) {
         const i64 b = R / rows, r = R % rows;
-        const i64 base = b * ldb + r;
+        const i64 base = b * ldb + r * ldr;
 #pragma unroll 4
-        for (i64 k = kg; k < K; k += 16) {
+        for (i64 k = k0 + kg; k < k1; k += 4) {
             const i64 t = entry_top(X, e, L, plane, base + k * ldk);
             mx = t > mx ? t : mx;
         }
@@ -287,29 +300,9 @@ __global__ void __launch_bounds__(1024) k_row_exp_cols(const i32* __restrict__ X
     part[kg][lane] = mx;
     __syncthreads();
     if (kg == 0 && R < nrows) {
-        for (int g = 1; g < 16; g++) mx = part[g][lane] > mx ? part[g][lane] : mx;
-        i64 E = mx - (32 * (i64)L - 2);
-        if (E < EZV / 2) E = 0;
-        Erow[R] = E;
+        for (int g = 1; g < 4; g++) mx = part[g][lane] > mx ? part[g][lane] : mx;
+        row_max_commit(Erow, R, mx, L);
     }
-}
-
-// general strides: one thread per row
-__global__ void __launch_bounds__(256) k_row_exp_thread(const i32* __restrict__ X, const i64* __restrict__ e, int L,
-                                                        i64 plane, i64 nrows, i64 rows, i64 K, i64 ldb, i64 ldr,
-                                                        i64 ldk, i64* __restrict__ Erow) {
-    const i64 R = blockIdx.x * (i64)256 + threadIdx.x;
-    if (R >= nrows) return;
-    const i64 b = R / rows, r = R % rows;
-    const i64 base = b * ldb + r * ldr;
-    i64 mx = EZV;
-    for (i64 k = 0; k < K; k++) {
-        const i64 t = entry_top(X, e, L, plane, base + k * ldk);
-        mx = t > mx ? t : mx;
-    }
-    i64 E = mx - (32 * (i64)L - 2);
-    if (E < EZV / 2) E = 0;
-    Erow[R] = E;
 }
 
 inline unsigned nblocks(i64 n, int tpb) { return (unsigned)((n + tpb - 1) / tpb); }
@@ -420,15 +413,20 @@ int sdpv_row_exp(const int32_t* X, const int64_t* e, int64_t L, int64_t plane, i
     const i64 nrows = nb * rows;
     if (nrows == 0) return SDPV_OK;
     if (!X || !e || !Erow) return SDPV_EARG;
+    hipStream_t st = as_stream(stream);
+    hipLaunchKernelGGL(k_fill_i64, dim3(nblocks(nrows, 256)), dim3(256), 0, st, Erow, nrows, (i64)EZV);
+    SDPV_TRY_LAUNCH();
+    if (K == 0) return SDPV_OK;
     if (ldk == 1) {
-        hipLaunchKernelGGL(k_row_exp_wave, dim3(nblocks(nrows, 4)), dim3(256), 0, as_stream(stream), X, e, (int)L,
+        const i64 ns = (K + RK_WAVE - 1) / RK_WAVE;
+        if (ns > 65535) return SDPV_ESIZE;
+        hipLaunchKernelGGL(k_row_exp_wave, dim3(nblocks(nrows, 4), (unsigned)ns), dim3(256), 0, st, X, e, (int)L,
                            (i64)plane, nrows, (i64)rows, (i64)K, (i64)ldb, (i64)ldr, Erow);
-    } else if (ldr == 1) {
-        hipLaunchKernelGGL(k_row_exp_cols, dim3(nblocks(nrows, 64)), dim3(1024), 0, as_stream(stream), X, e,
-                           (int)L, (i64)plane, nrows, (i64)rows, (i64)K, (i64)ldb, (i64)ldk, Erow);
     } else {
-        hipLaunchKernelGGL(k_row_exp_thread, dim3(nblocks(nrows, 256)), dim3(256), 0, as_stream(stream), X, e,
-                           (int)L, (i64)plane, nrows, (i64)rows, (i64)K, (i64)ldb, (i64)ldr, (i64)ldk, Erow);
+        const i64 ns = (K + RK_COLS - 1) / RK_COLS;
+        if (ns > 65535) return SDPV_ESIZE;
+        hipLaunchKernelGGL(k_row_exp_cols, dim3(nblocks(nrows, 64), (unsigned)ns), dim3(256), 0, st, X, e, (int)L,
+                           (i64)plane, nrows, (i64)rows, (i64)K, (i64)ldb, (i64)ldr, (i64)ldk, Erow);
     }
     SDPV_TRY_LAUNCH();
     return SDPV_OK;

@@ -43,10 +43,18 @@ __device__ __forceinline__ v16i mfma(v4i a, v4i b, v16i c) {
 }
 
 // ================================================================== residue split
-// One workgroup: EPW entries (TR rows x TK consecutive k). Phase 1 shift-rounds each entry into
-// its Lout limbs and stores the limb bytes (as u - 128) in LDS; phase 2 computes, per group of 32
-// primes, the two byte dot products T_hi . x, T_lo . x on MFMA (primes = rows, entries = columns),
-// reduces modulo m, splits, stages the int8 planes in LDS and writes 16-byte chunks.
+// One workgroup: EPW entries (TR rows x TK consecutive k).
+// Phase 1: the entries' Lin limbs are copied into LDS with independent (unrolled) loads, then each
+//   thread shift-rounds its entry into Lout limbs and stores the limb bytes (as u - 128) in LDS.
+// Phase 2: per group of 32 primes, the byte dot products T_hi . x and T_lo . x run on MFMA (primes
+//   = rows, entries = columns); the reduction modulo m uses per-block prime constants in LDS; the
+//   int8 planes are staged in LDS and written as 16-byte chunks.
+struct LdsStrided {
+    const u32* p;
+    int stride;
+    __device__ __forceinline__ u32 operator()(int k) const { return p[k * stride]; }
+};
+
 __global__ void __launch_bounds__(256) k_rns_split(
     const i32* __restrict__ X, const i64* __restrict__ e, const i64* __restrict__ Erow, int L, int Lout, i64 plane,
     i64 nb, i64 rows, i64 ldb, i64 ldr, i64 ldk, i64 k0, i64 kc, i64 kp, const int8_t* __restrict__ Thi,
@@ -55,9 +63,15 @@ __global__ void __launch_bounds__(256) k_rns_split(
     int8_t* __restrict__ Rl, i32* __restrict__ flag, int EPW, int TK, i64 tiles_k) {
     SDPV_DYNAMIC_LDS(unsigned char, smem);
     const int XS = NBp + 16;
-    unsigned char* xs = smem;                           // EPW x XS bytes
+    unsigned char* xs = smem;                           // EPW x XS limb bytes
     unsigned char* negs = xs + (size_t)EPW * XS;        // EPW sign flags
-    int8_t* stg = (int8_t*)(negs + EPW);                // 2 planes x 32 primes x EPW
+    double* cmi = (double*)(negs + EPW);                // 32 prime constants of the current group
+    i32* cm = (i32*)(cmi + 32);
+    i32* cc1 = cm + 32;
+    i32* cc2 = cc1 + 32;
+    unsigned char* uni = (unsigned char*)(cc2 + 32);    // phase 1: raw limbs [L][EPW]; phase 2: staging
+    u32* raw = (u32*)uni;
+    int8_t* stg = (int8_t*)uni;                         // 2 planes x 32 primes x EPW
     const int t = threadIdx.x;
     const int TR = EPW / TK;
     const i64 tr = blockIdx.x / tiles_k, tk = blockIdx.x % tiles_k;
@@ -70,15 +84,24 @@ __global__ void __launch_bounds__(256) k_rns_split(
         if (R < nrows && k < kc) {
             const i64 b = R / rows, r = R % rows;
             const i64 off = b * ldb + r * ldr + (k0 + k) * ldk;
+            const i32* src = X + off;
+            int w = 0;
+            for (; w + 8 <= L; w += 8) {
+                u32 v[8];
+#pragma unroll
+                for (int u = 0; u < 8; u++) v[u] = (u32)src[(i64)(w + u) * plane];
+#pragma unroll
+                for (int u = 0; u < 8; u++) raw[(w + u) * EPW + t] = v[u];
+            }
+            for (; w < L; w++) raw[w * EPW + t] = (u32)src[(i64)w * plane];
             i64 s = 0;
             if (e) {
                 const i64 ex = e[off];
                 s = (ex == EZV) ? (i64)(1 << 30) : Erow[R] - ex;
             }
-            PlaneSrc src{X + off, plane};
-            const int bad = shr_round(src, L, s, Lout, [&](int w, u32 v) {
-                row[w] = v ^ 0x80808080u;
-                if (w == Lout - 1) neg = (int)(v >> 31);
+            const int bad = shr_round(LdsStrided{raw + t, EPW}, L, s, Lout, [&](int wi, u32 v) {
+                row[wi] = v ^ 0x80808080u;
+                if (wi == Lout - 1) neg = (int)(v >> 31);
             });
             if (bad && flag) atomicOr(flag, 1);
         } else {
@@ -87,11 +110,19 @@ __global__ void __launch_bounds__(256) k_rns_split(
         for (int w = Lout; w < NBp / 4; w++) row[w] = 0u;
         negs[t] = (unsigned char)neg;
     }
-    __syncthreads();
 
     const int wave = t >> 6, lane = t & 63, l32 = lane & 31, hsel = lane >> 5;
     const int nEB = EPW / 32, nPB = (G + 31) / 32, KS = NBp / 32;
     for (int pb = 0; pb < nPB; pb++) {
+        if (t < 32) {
+            const int gi = pb * 32 + t;
+            const int g = g0 + (gi < G ? gi : 0);
+            cm[t] = mods[g];
+            cmi[t] = minv[g];
+            cc1[t] = C1[g];
+            cc2[t] = C2[g];
+        }
+        __syncthreads();
         for (int eb = wave; eb < nEB; eb += 4) {
             v16i ah = zero16(), al = zero16();
             const int prow = pb * 32 + l32;
@@ -99,13 +130,23 @@ __global__ void __launch_bounds__(256) k_rns_split(
             const int8_t* th = Thi + (i64)(g0 + (pok ? prow : 0)) * NBp;
             const int8_t* tl = Tlo + (i64)(g0 + (pok ? prow : 0)) * NBp;
             const unsigned char* xrow = xs + (size_t)(eb * 32 + l32) * XS;
-            for (int ks = 0; ks < KS; ks++) {
-                const int boff = ks * 32 + 16 * hsel;
-                const v4i fah = pok ? *(const v4i*)(th + boff) : zero4();
-                const v4i fal = pok ? *(const v4i*)(tl + boff) : zero4();
-                const v4i fb = *(const v4i*)(xrow + boff);
-                ah = mfma(fah, fb, ah);
-                al = mfma(fal, fb, al);
+            for (int kb = 0; kb < KS; kb += 4) {
+                v4i fah[4], fal[4];
+#pragma unroll
+                for (int u = 0; u < 4; u++) {
+                    const int boff = (kb + u) * 32 + 16 * hsel;
+                    const bool ok = pok && (kb + u < KS);
+                    fah[u] = ok ? *(const v4i*)(th + boff) : zero4();
+                    fal[u] = ok ? *(const v4i*)(tl + boff) : zero4();
+                }
+#pragma unroll
+                for (int u = 0; u < 4; u++) {
+                    if (kb + u < KS) {
+                        const v4i fb = *(const v4i*)(xrow + (kb + u) * 32 + 16 * hsel);
+                        ah = mfma(fah[u], fb, ah);
+                        al = mfma(fal[u], fb, al);
+                    }
+                }
             }
             const int c = eb * 32 + l32;
             const int cneg = negs[c];
@@ -114,13 +155,11 @@ __global__ void __launch_bounds__(256) k_rns_split(
 #pragma unroll
             for (int j = 0; j < 16; j++) {
                 const int pr = 8 * (j >> 2) + 4 * hsel + (j & 3);
-                const int gi = pb * 32 + pr;
                 int hi = 0, lo = 0;
-                if (gi < G && cvalid) {
-                    const int g = g0 + gi;
-                    const int m = mods[g];
-                    const int x = ah[j] * 128 + al[j] + C1[g] - (cneg ? C2[g] : 0);
-                    split_centred(mod_m(x, m, minv[g]), m, hi, lo);
+                if (pb * 32 + pr < G && cvalid) {
+                    const int m = cm[pr];
+                    const int x = ah[j] * 128 + al[j] + cc1[pr] - (cneg ? cc2[pr] : 0);
+                    split_centred(mod_m(x, m, cmi[pr]), m, hi, lo);
                 }
                 stg[(size_t)pr * EPW + c] = (int8_t)hi;
                 stg[(size_t)(32 + pr) * EPW + c] = (int8_t)lo;
@@ -187,20 +226,20 @@ __global__ void __launch_bounds__(WM* WN * 64) k_rns_gemm(
 
     constexpr int PA = BM * 4, PB = BN * 4;  // (row, 16-byte chunk) pairs per stage
     constexpr int NA = (PA + NT - 1) / NT, NBr = (PB + NT - 1) / NT;
-    v4i ra_h[NA], ra_l[NA], rb_h[NBr], rb_l[NBr];
+    v4i ra_h[1][NA], ra_l[1][NA], rb_h[1][NBr], rb_l[1][NBr];
 
-    auto gload = [&](i64 kk) {
+    auto gload = [&](int set, i64 kk) {
 #pragma unroll
         for (int u = 0; u < NA; u++) {
             const int idx = t + u * NT;
             const int row = idx >> 2, ch = idx & 3;
             const i64 gr = r0 + row;
             if (idx < PA && gr < M) {
-                ra_h[u] = *(const v4i*)(pAh + gr * kp + kk + ch * 16);
-                ra_l[u] = *(const v4i*)(pAl + gr * kp + kk + ch * 16);
+                ra_h[set][u] = *(const v4i*)(pAh + gr * kp + kk + ch * 16);
+                ra_l[set][u] = *(const v4i*)(pAl + gr * kp + kk + ch * 16);
             } else {
-                ra_h[u] = zero4();
-                ra_l[u] = zero4();
+                ra_h[set][u] = zero4();
+                ra_l[set][u] = zero4();
             }
         }
 #pragma unroll
@@ -209,15 +248,15 @@ __global__ void __launch_bounds__(WM* WN * 64) k_rns_gemm(
             const int row = idx >> 2, ch = idx & 3;
             const i64 gc = c0 + row;
             if (idx < PB && gc < N) {
-                rb_h[u] = *(const v4i*)(pBh + gc * kp + kk + ch * 16);
-                rb_l[u] = *(const v4i*)(pBl + gc * kp + kk + ch * 16);
+                rb_h[set][u] = *(const v4i*)(pBh + gc * kp + kk + ch * 16);
+                rb_l[set][u] = *(const v4i*)(pBl + gc * kp + kk + ch * 16);
             } else {
-                rb_h[u] = zero4();
-                rb_l[u] = zero4();
+                rb_h[set][u] = zero4();
+                rb_l[set][u] = zero4();
             }
         }
     };
-    auto lstore = [&](int st) {
+    auto lstore = [&](int set, int st) {
 #pragma unroll
         for (int u = 0; u < NA; u++) {
             const int idx = t + u * NT;
@@ -225,9 +264,9 @@ __global__ void __launch_bounds__(WM* WN * 64) k_rns_gemm(
                 const int row = idx >> 2, ch = idx & 3;
                 v4i s;
 #pragma unroll
-                for (int q = 0; q < 4; q++) s[q] = (int)add_bytes((u32)ra_h[u][q], (u32)ra_l[u][q]);
-                *(v4i*)(sA(st, 0) + row * RS + ch * 16) = ra_h[u];
-                *(v4i*)(sA(st, 1) + row * RS + ch * 16) = ra_l[u];
+                for (int q = 0; q < 4; q++) s[q] = (int)add_bytes((u32)ra_h[set][u][q], (u32)ra_l[set][u][q]);
+                *(v4i*)(sA(st, 0) + row * RS + ch * 16) = ra_h[set][u];
+                *(v4i*)(sA(st, 1) + row * RS + ch * 16) = ra_l[set][u];
                 *(v4i*)(sA(st, 2) + row * RS + ch * 16) = s;
             }
         }
@@ -238,9 +277,9 @@ __global__ void __launch_bounds__(WM* WN * 64) k_rns_gemm(
                 const int row = idx >> 2, ch = idx & 3;
                 v4i s;
 #pragma unroll
-                for (int q = 0; q < 4; q++) s[q] = (int)add_bytes((u32)rb_h[u][q], (u32)rb_l[u][q]);
-                *(v4i*)(sB(st, 0) + row * RS + ch * 16) = rb_h[u];
-                *(v4i*)(sB(st, 1) + row * RS + ch * 16) = rb_l[u];
+                for (int q = 0; q < 4; q++) s[q] = (int)add_bytes((u32)rb_h[set][u][q], (u32)rb_l[set][u][q]);
+                *(v4i*)(sB(st, 0) + row * RS + ch * 16) = rb_h[set][u];
+                *(v4i*)(sB(st, 1) + row * RS + ch * 16) = rb_l[set][u];
                 *(v4i*)(sB(st, 2) + row * RS + ch * 16) = s;
             }
         }
@@ -256,13 +295,7 @@ __global__ void __launch_bounds__(WM* WN * 64) k_rns_gemm(
             cLL[i][j] = zero16();
         }
 
-    const i64 nst = kp / 64;
-    gload(0);
-    lstore(0);
-    __syncthreads();
-    for (i64 st = 0; st < nst; st++) {
-        const int cur = (int)(st & 1);
-        if (st + 1 < nst) gload((st + 1) * 64);
+    auto compute = [&](int cur) {
 #pragma unroll
         for (int ks = 0; ks < 2; ks++) {
             const int koff = ks * 32 + 16 * hsel;
@@ -288,7 +321,18 @@ __global__ void __launch_bounds__(WM* WN * 64) k_rns_gemm(
                     cSS[i][j] = mfma(fa[i][2], fb[j][2], cSS[i][j]);
                 }
         }
-        if (st + 1 < nst) lstore(cur ^ 1);
+    };
+
+    // stage s: global loads issued during the compute of stage s-1, then stored to LDS buffer s % 2
+    const i64 nst = kp / 64;
+    gload(0, 0);
+    lstore(0, 0);
+    __syncthreads();
+    for (i64 st = 0; st < nst; st++) {
+        const int cur = (int)(st & 1);
+        if (st + 1 < nst) gload(0, (st + 1) * 64);
+        compute(cur);
+        if (st + 1 < nst) lstore(0, cur ^ 1);
         __syncthreads();
     }
 
@@ -343,68 +387,91 @@ int launch_gemm(const int8_t* Ah, const int8_t* Al, const int8_t* Bh, const int8
 }
 
 // ================================================================== CRT reconstruction
-// QW entries per workgroup. Phase 1: y_g = centred(acc_g * inv_g mod m_g), split to int8 planes
-// in LDS, k = round(sum y_g / m_g) (|sum - k| < 1/4 since |value| < M/4, so double suffices).
-// Phase 2 (rounds of DB digit blocks of 32): Z = Y . Ct^T on MFMA (entries x digits), then one
-// thread per entry carries sum_d (Z_d - k Md_d) 256^d into 32-bit words.
+// QW entries per workgroup (template), 256 threads. Phase 1: y_g = centred(acc_g * inv_g mod m_g)
+// (residues gathered 8 at a time, prime constants in LDS), split to int8 planes in LDS;
+// k = round(sum y_g / m_g) (|sum - k| < 1/4 because |value| < M/4, so double suffices).
+// Phase 2 (rounds of DB digit blocks of 32): Z = Y . Ct^T on MFMA (entries x digits) into a
+// digit-major int32 buffer with an odd row stride, then one thread per entry carries
+// sum_d (Z_d - k Md_d) 256^d into 32-bit words.
 // Phase 3: words -> W + bitlen, or normalised MPF limbs, or a fixed-exponent rounding.
+template <int QW>
 __global__ void __launch_bounds__(256) k_rns_crt(
     const int16_t* __restrict__ acc, int n, i64 ld_g, i64 rows, i64 N, i64 j0, i64 w, const i32* __restrict__ mods,
     const double* __restrict__ minv, const i32* __restrict__ inv, const int8_t* __restrict__ Ct,
     const i32* __restrict__ Md, int E, int npad, int mode, i32* __restrict__ W, i64* __restrict__ bitlen,
     i32* __restrict__ outm, i64* __restrict__ oute, int L, i64 plane, i64 ldo, i64 jo, i64 Mb,
-    const i64* __restrict__ erow,
-    const i64* __restrict__ ecol, const i64* __restrict__ scol, i32* __restrict__ flag, int QW, int DB) {
+    const i64* __restrict__ erow, const i64* __restrict__ ecol, const i64* __restrict__ scol, i32* __restrict__ flag) {
+    constexpr int DB = 4 / (QW / 32);
+    constexpr int NGRP = 256 / QW;
+    constexpr int QWp = QW + 1;                       // odd row stride: conflict-free both ways
+    constexpr int ZD = DB * 32;
     SDPV_DYNAMIC_LDS(unsigned char, smem);
     const int YS = npad + 16;
     const int NW = E / 4;
-    const int QWp = QW + 1;                           // odd row stride: conflict-free both ways
-    const int ZD = DB * 32;
+    const int Epad = ((E + 31) / 32) * 32;
     int8_t* Yh = (int8_t*)smem;
     int8_t* Yl = Yh + (size_t)QW * YS;
-    double* ksum = (double*)(Yl + (size_t)QW * YS);  // [8][QW]
-    i32* kq = (i32*)(ksum + 8 * QW);                  // [QW]
+    double* ksum = (double*)(Yl + (size_t)QW * YS);  // [NGRP][QW]
+    i32* kq = (i32*)(ksum + NGRP * QW);               // [QW]
     i32* Mds = kq + QW;                               // [Epad] digits of M
-    i32* Zt = Mds + ((E + 31) / 32) * 32;             // [ZD][QWp]: Z of digit d, entry q
+    i32* cm = Mds + Epad;                             // [npad] moduli
+    i32* cinv = cm + npad;                            // [npad] (M/m)^-1 mod m
+    double* cmi = (double*)(cinv + npad);             // [npad] 1/m
+    i32* Zt = (i32*)(cmi + npad);                     // [ZD][QWp]: Z of digit d, entry q
     u32* Wl = (u32*)(Zt + (size_t)ZD * QWp);          // [QW][NW+1] (modes 1, 2)
     const i64 Bq = rows * w;
     const i64 q0 = blockIdx.x * (i64)QW;
     const int t = threadIdx.x;
 
-    for (int d = t; d < ((E + 31) / 32) * 32; d += 256) Mds[d] = Md[d];
+    for (int d = t; d < Epad; d += 256) Mds[d] = Md[d];
+    for (int g = t; g < npad; g += 256) {
+        const bool ok = g < n;
+        cm[g] = ok ? mods[g] : 1;
+        cinv[g] = ok ? inv[g] : 0;
+        cmi[g] = ok ? minv[g] : 0.0;
+    }
+    __syncthreads();
     {
-        const int ql = t % QW, grp = t / QW, ngrp = 256 / QW;
+        const int ql = t % QW, grp = t / QW;
         const i64 q = q0 + ql;
         const bool qok = q < Bq;
         i64 off = 0;
         if (qok) off = (q / w) * N + j0 + q % w;
         double s = 0.0;
-        for (int g = grp; g < npad; g += ngrp) {
-            int hi = 0, lo = 0;
-            if (g < n && qok) {
-                const int m = mods[g];
-                int y = mod_m((int)acc[(i64)g * ld_g + off] * inv[g], m, minv[g]);
-                if (y > (m >> 1)) y -= m;
-                s += (double)y * minv[g];
-                lo = ((y + 64) & 127) - 64;
-                hi = (y - lo) >> 7;
+        for (int gb = grp; gb < npad; gb += 32 * NGRP) {
+            int a[32];
+#pragma unroll
+            for (int u = 0; u < 32; u++) {
+                const int g = gb + u * NGRP;
+                a[u] = (g < n && qok) ? (int)acc[(i64)g * ld_g + off] : 0;
             }
-            Yh[(size_t)ql * YS + g] = (int8_t)hi;
-            Yl[(size_t)ql * YS + g] = (int8_t)lo;
+#pragma unroll
+            for (int u = 0; u < 32; u++) {
+                const int g = gb + u * NGRP;
+                if (g < npad) {
+                    const int m = cm[g];
+                    int y = mod_m(a[u] * cinv[g], m, cmi[g]);
+                    if (y > (m >> 1)) y -= m;
+                    s += (double)y * cmi[g];
+                    const int lo = ((y + 64) & 127) - 64;
+                    Yh[(size_t)ql * YS + g] = (int8_t)((y - lo) >> 7);
+                    Yl[(size_t)ql * YS + g] = (int8_t)lo;
+                }
+            }
         }
         ksum[grp * QW + ql] = s;
     }
     __syncthreads();
     if (t < QW) {
         double s = 0.0;
-        for (int gr = 0; gr < 256 / QW; gr++) s += ksum[gr * QW + t];
+        for (int gr = 0; gr < NGRP; gr++) s += ksum[gr * QW + t];
         kq[t] = (int)floor(s + 0.5);
     }
     __syncthreads();
 
     const int wave = t >> 6, lane = t & 63, l32 = lane & 31, hsel = lane >> 5;
-    const int nEB = QW / 32;
-    const int nDB = (E + 31) / 32;
+    constexpr int nEB = QW / 32;
+    const int nDB = Epad / 32;
     const int kqt = (t < QW) ? kq[t] : 0;
     const i64 qt = q0 + t;
     const bool qtok = (t < QW) && (qt < Bq);
@@ -419,11 +486,19 @@ __global__ void __launch_bounds__(256) k_rns_crt(
                 const int8_t* yh = Yh + (size_t)(eb * 32 + l32) * YS;
                 const int8_t* yl = Yl + (size_t)(eb * 32 + l32) * YS;
                 const int8_t* crow = Ct + (i64)(db * 32 + l32) * npad;
-                for (int ks = 0; ks < npad / 32; ks++) {
-                    const int off = ks * 32 + 16 * hsel;
-                    const v4i bfr = *(const v4i*)(crow + off);
-                    zh = mfma(*(const v4i*)(yh + off), bfr, zh);
-                    zl = mfma(*(const v4i*)(yl + off), bfr, zl);
+                for (int kb = 0; kb < npad / 32; kb += 8) {
+                    v4i bfr[8];
+#pragma unroll
+                    for (int u = 0; u < 8; u++)
+                        bfr[u] = (kb + u < npad / 32) ? *(const v4i*)(crow + (kb + u) * 32 + 16 * hsel) : zero4();
+#pragma unroll
+                    for (int u = 0; u < 8; u++) {
+                        if (kb + u < npad / 32) {
+                            const int off = (kb + u) * 32 + 16 * hsel;
+                            zh = mfma(*(const v4i*)(yh + off), bfr[u], zh);
+                            zl = mfma(*(const v4i*)(yl + off), bfr[u], zl);
+                        }
+                    }
                 }
                 i32* zcol = Zt + (size_t)(dbl * 32 + l32) * QWp + eb * 32;
 #pragma unroll
@@ -433,6 +508,7 @@ __global__ void __launch_bounds__(256) k_rns_crt(
         __syncthreads();
         if (t < QW) {
             const int nd = ((nDB - db0) < DB ? (nDB - db0) : DB) * 32;
+#pragma unroll 4
             for (int d4 = 0; d4 < nd; d4 += 4) {
                 i64 v = carry;
 #pragma unroll
@@ -515,7 +591,7 @@ int sdpv_rns_split(const int32_t* X, const int64_t* e, const int64_t* Erow, int6
                    int8_t* Rl, int32_t* flag, void* stream) {
     if (L < 1 || Lout < 1 || nb < 0 || rows < 0 || kc < 0 || kp < kc || G < 0 || g0 < 0) return SDPV_EARG;
     if (kp % 64 != 0 || NBp != 32 * ((4 * Lout + 31) / 32)) return SDPV_EARG;
-    if (Lout > 128) return SDPV_ESIZE;
+    if (Lout > 128 || L > 128) return SDPV_ESIZE;
     const i64 nrows = nb * rows;
     if (nrows == 0 || kp == 0 || G == 0) return SDPV_OK;
     if (!X || !Thi || !Tlo || !C1 || !C2 || !mods || !minv || !Rh || !Rl) return SDPV_EARG;
@@ -526,7 +602,14 @@ int sdpv_rns_split(const int32_t* X, const int64_t* e, const int64_t* Erow, int6
     const i64 tiles_k = kp / TK, tiles_r = (nrows + TR - 1) / TR;
     const i64 grid = tiles_r * tiles_k;
     if (grid > 0x7fffffffLL) return SDPV_ESIZE;
-    const size_t lds = (size_t)EPW * (NBp + 16) + EPW + (size_t)2 * 32 * EPW;
+    const size_t uni = (size_t)EPW * (L * 4 > 64 ? L * 4 : 64);
+    const size_t lds = (size_t)EPW * (NBp + 16) + EPW + 32 * 8 + 3 * 32 * 4 + uni;
+    if (lds > 160 * 1024) return SDPV_ESIZE;
+    static bool attr_set = false;
+    if (!attr_set) {
+        (void)hipFuncSetAttribute((const void*)k_rns_split, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+        attr_set = true;
+    }
     hipLaunchKernelGGL(k_rns_split, dim3((unsigned)grid), dim3(256), lds, as_stream(stream), X, e, Erow, (int)L,
                        (int)Lout, (i64)plane, (i64)nb, (i64)rows, (i64)ldb, (i64)ldr, (i64)ldk, (i64)k0, (i64)kc,
                        (i64)kp, Thi, Tlo, (int)NBp, C1, C2, mods, minv, (int)g0, (int)G, Rh, Rl, flag, EPW, TK,
@@ -576,20 +659,32 @@ int sdpv_rns_crt(const int16_t* acc, int64_t n, int64_t ld_g, int64_t rows, int6
     const int DB = 4 / (QW / 32);
     const int NW = (int)(E / 4);
     const i64 Epad = 32 * ((E + 31) / 32);
-    size_t lds = (size_t)2 * QW * (npad + 16) + (size_t)8 * QW * 8 + (size_t)QW * 4 + (size_t)Epad * 4 +
-                 (size_t)DB * 32 * (QW + 1) * 4;
+    size_t lds = (size_t)2 * QW * (npad + 16) + (size_t)(256 / QW) * QW * 8 + (size_t)QW * 4 + (size_t)Epad * 4 +
+                 (size_t)npad * 16 + (size_t)DB * 32 * (QW + 1) * 4;
     if (mode != SDPV_CRT_WORDS) lds += (size_t)QW * (NW + 1) * 4;
     if (lds > 160 * 1024) return SDPV_ESIZE;
     const i64 grid = (Bq + QW - 1) / QW;
     if (grid > 0x7fffffffLL) return SDPV_ESIZE;
-    static bool attr_set = false;
-    if (!attr_set) {
-        (void)hipFuncSetAttribute((const void*)k_rns_crt, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-        attr_set = true;
+    hipStream_t st = as_stream(stream);
+    if (QW == 64) {
+        static bool attr64 = false;
+        if (!attr64) {
+            (void)hipFuncSetAttribute((const void*)k_rns_crt<64>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+            attr64 = true;
+        }
+        hipLaunchKernelGGL(k_rns_crt<64>, dim3((unsigned)grid), dim3(256), lds, st, acc, (int)n, (i64)ld_g, (i64)rows,
+                           (i64)N, (i64)j0, (i64)w, mods, minv, inv, Ct, Md, (int)E, (int)npad, (int)mode, W, bitlen,
+                           outm, oute, (int)L, (i64)plane, (i64)ldo, (i64)jo, (i64)Mb, erow, ecol, scol, flag);
+    } else {
+        static bool attr32 = false;
+        if (!attr32) {
+            (void)hipFuncSetAttribute((const void*)k_rns_crt<32>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+            attr32 = true;
+        }
+        hipLaunchKernelGGL(k_rns_crt<32>, dim3((unsigned)grid), dim3(256), lds, st, acc, (int)n, (i64)ld_g, (i64)rows,
+                           (i64)N, (i64)j0, (i64)w, mods, minv, inv, Ct, Md, (int)E, (int)npad, (int)mode, W, bitlen,
+                           outm, oute, (int)L, (i64)plane, (i64)ldo, (i64)jo, (i64)Mb, erow, ecol, scol, flag);
     }
-    hipLaunchKernelGGL(k_rns_crt, dim3((unsigned)grid), dim3(256), lds, as_stream(stream), acc, (int)n, (i64)ld_g,
-                       (i64)rows, (i64)N, (i64)j0, (i64)w, mods, minv, inv, Ct, Md, (int)E, (int)npad, (int)mode, W,
-                       bitlen, outm, oute, (int)L, (i64)plane, (i64)ldo, (i64)jo, (i64)Mb, erow, ecol, scol, flag, QW, DB);
     SDPV_TRY_LAUNCH();
     return SDPV_OK;
 }

@@ -311,6 +311,7 @@ def _align_rows(A):
     nb = A.e.numel() // max(R_ * K, 1)
     m, e = A.m.contiguous(), A.e.contiguous()
     E = _row_exp(m, e, L, e.numel(), nb, R_, K, R_ * K, K, 1).view(*A.shape[:-1])
+    E = torch.where(E < EZ // 2, torch.zeros_like(E), E)          # all-zero rows: exponent 0
     s = (E[..., None] - A.e).reshape(-1)
     s = torch.where(A.e.reshape(-1) == EZ, torch.full_like(s, 1 << 30), s)
     return shift_round(A.m.reshape(L, -1), s, L).view(A.m.shape), E

@@ -36,6 +36,12 @@ void emu_syncthreads();
 inline void __syncthreads() { emu_syncthreads(); }
 inline int __clz(int x) { return x == 0 ? 32 : __builtin_clz((unsigned)x); }
 inline int atomicOr(int* p, int v) { return __atomic_fetch_or(p, v, __ATOMIC_SEQ_CST); }
+inline long long atomicMax(long long* p, long long v) {
+    long long cur = __atomic_load_n(p, __ATOMIC_SEQ_CST);
+    while (cur < v && !__atomic_compare_exchange_n(p, &cur, v, false, __ATOMIC_SEQ_CST, __ATOMIC_SEQ_CST)) {
+    }
+    return cur;
+}
 long long emu_shfl_xor64(long long v, int mask);
 inline long long __shfl_xor(long long v, int mask, int width = 64) { (void)width; return emu_shfl_xor64(v, mask); }
 inline long __shfl_xor(long v, int mask, int width = 64) { (void)width; return (long)emu_shfl_xor64(v, mask); }
