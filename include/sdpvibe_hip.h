@@ -157,6 +157,22 @@ int sdpv_rns_crt(const int16_t* acc, int64_t n, int64_t ld_g, int64_t rows, int6
                  int64_t L, int64_t plane, int64_t ldo, int64_t jo, int64_t Mb, const int64_t* erow,
                  const int64_t* ecol, const int64_t* scol, int32_t* flag, void* stream);
 
+/* ---------------------------------------------------------------- tile base case */
+
+/* Batched Cholesky factor and inverse of nb BT x BT tiles (BT <= 64) in wide fixed point.
+ * A: (L, nb, BT, BT) limb planes of equilibrated SPD blocks at exponent -(32L - 40).
+ * The tiles are widened exactly to WL = L + 24 words with F = 32L fractional bits, factored
+ * (pivot d^-1/2 by a double seed and precision-doubling Newton, truncated products) and inverted
+ * row by row, as the reference's kernels do.
+ * Lout: (L, nb, BT, BT) the factor rounded (half up) back to A's exponent, upper triangle zero.
+ * Xw:   (L + 24, nb, BT, BT) the inverse L^-1 in wide fixed point (exponent -32L).
+ * work: device scratch of 3 * nb * BT * BT * (L + 24) uint32 words.
+ * flag: optional int32 set to 1 when the narrowing to L limbs overflows.
+ * Replaces chol_tiles + trinv_tiles (sdpvibe/basecase.py:159-226) and the widening/narrowing of
+ * batched.bchol_tile (sdpvibe/batched.py:243-257). */
+int sdpv_tile_chol(const int32_t* A, int64_t L, int64_t nb, int64_t BT, int32_t* Lout, int32_t* Xw,
+                   uint32_t* work, int32_t* flag, void* stream);
+
 /* ---------------------------------------------------------------- diagnostics */
 
 /* D = A * B through one wave of v_mfma_i32_32x32x32_i8 using the same fragment mapping as the

@@ -14,6 +14,7 @@
 #include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
+#include <math.h>
 
 #define EZ (-(1LL << 40))
 
@@ -433,4 +434,229 @@ void orc_gemm_exact(const int32_t* A, const int32_t* Bt, i64 L, i64 M, i64 N, i6
     for (i64 i = 0; i < M * K; i++) mpz_clear(a[i]);
     for (i64 i = 0; i < N * K; i++) mpz_clear(b[i]);
     free(a); free(b);
+}
+
+/* batched.bgemm_nt + _bcrt_round (sdpvibe/batched.py:165-231): C[b] = A[b] Bt[b]^T exactly (A: (L, nb, M, K),
+ * Bt: (L, nb, N, K) fixed-point limb planes), then rounded to L limbs per (block, column):
+ *   mode 0 (fixed): exponent oexp[b*N + j] given; shift = oexp - e_in
+ *   mode 1 ('col'): shift = max_i bitlen(C[b, i, j]) - (32L - 32); oexp = e_in + shift
+ *   mode 2 ('mat'): 'col', then every column of block b re-rounded to the block's max exponent
+ * e_in (nb, N) is the exponent of the exact product. out (L, nb, M, N), oexp (nb, N) in/out. */
+void orc_bgemm_nt(const int32_t* A, const int32_t* Bt, i64 L, i64 nb, i64 M, i64 N, i64 K, const int64_t* e_in,
+                  int32_t mode, int32_t* out, int64_t* oexp) {
+    mpz_t* c = (mpz_t*)malloc(sizeof(mpz_t) * (size_t)(nb * M * N));
+    i64 plA = nb * M * K, plB = nb * N * K, plC = nb * M * N;
+    #pragma omp parallel
+    {
+        mpz_t a, b;
+        mpz_init(a);
+        mpz_init(b);
+        #pragma omp for schedule(dynamic, 8)
+        for (i64 q = 0; q < plC; q++) {
+            i64 bb = q / (M * N), i = (q / N) % M, j = q % N;
+            mpz_init_set_ui(c[q], 0);
+            for (i64 k = 0; k < K; k++) {
+                from_words(a, A + (bb * M + i) * K + k, L, plA);
+                from_words(b, Bt + (bb * N + j) * K + k, L, plB);
+                mpz_addmul(c[q], a, b);
+            }
+        }
+        mpz_clear(a);
+        mpz_clear(b);
+    }
+    i64 H = 32 * L - 32;
+    for (i64 bb = 0; bb < nb; bb++)
+        for (i64 j = 0; j < N; j++) {
+            i64 ej = bb * N + j;
+            if (mode != 0) {
+                i64 mx = 0;
+                for (i64 i = 0; i < M; i++) {
+                    i64 bl = bitlen_tc(c[(bb * M + i) * N + j]);
+                    if (i == 0 || bl > mx) mx = bl;
+                }
+                oexp[ej] = e_in[ej] + (mx - H);
+            }
+            mpz_t y;
+            mpz_init(y);
+            for (i64 i = 0; i < M; i++) {
+                i64 q = (bb * M + i) * N + j;
+                round_shift(y, c[q], oexp[ej] - e_in[ej], 1LL << 40, 32 * (L + 2));
+                to_words(y, out + q, L, plC);
+            }
+            mpz_clear(y);
+        }
+    if (mode == 2) {
+        mpz_t x, y;
+        mpz_init(x);
+        mpz_init(y);
+        for (i64 bb = 0; bb < nb; bb++) {
+            i64 emax = oexp[bb * N];
+            for (i64 j = 1; j < N; j++) if (oexp[bb * N + j] > emax) emax = oexp[bb * N + j];
+            for (i64 j = 0; j < N; j++) {
+                for (i64 i = 0; i < M; i++) {
+                    i64 q = (bb * M + i) * N + j;
+                    from_words(x, out + q, L, plC);
+                    round_shift(y, x, emax - oexp[bb * N + j], 32 * L + 64, 32 * (L + 2));
+                    to_words(y, out + q, L, plC);
+                }
+                oexp[bb * N + j] = emax;
+            }
+        }
+        mpz_clear(x);
+        mpz_clear(y);
+    }
+    for (i64 q = 0; q < plC; q++) mpz_clear(c[q]);
+    free(c);
+}
+
+/* ---------------------------------------------------------------- tile base case (sdpvibe/basecase.py)
+ * Wide fixed point: WL 32-bit words, two's complement, F = 32*LF fractional bits. Restated from the
+ * CUDA source held in basecase.py:13-246 (read as text). */
+typedef struct { int WL, LF; } wfx_t;
+
+static int w_isneg(const u32* a, int WL) { return a[WL - 1] >> 31; }
+static void w_negate(u32* a, int WL) {
+    uint64_t c = 1;
+    for (int i = 0; i < WL; i++) { uint64_t t = (uint64_t)(~a[i]) + c; a[i] = (u32)t; c = t >> 32; }
+}
+static void w_addto(u32* r, const u32* a, int WL) {
+    uint64_t c = 0;
+    for (int i = 0; i < WL; i++) { uint64_t t = (uint64_t)r[i] + a[i] + c; r[i] = (u32)t; c = t >> 32; }
+}
+static void w_subfrom(u32* r, const u32* a, int WL) {
+    int64_t br = 0;
+    for (int i = 0; i < WL; i++) { int64_t t = (int64_t)r[i] - (int64_t)a[i] + br; r[i] = (u32)t; br = t >> 32; }
+}
+static int w_topl(const u32* a, int WL) { for (int i = WL - 1; i > 0; i--) if (a[i]) return i; return 0; }
+static void w_shr1(u32* a, int WL) {
+    for (int i = 0; i < WL - 1; i++) a[i] = (a[i] >> 1) | (a[i + 1] << 31);
+    a[WL - 1] = (u32)(((int32_t)a[WL - 1]) >> 1);
+}
+
+/* r = (a*b) >> F, signed, truncated; t > 0: only the top t words of |a| and |b| (basecase.py:31-53) */
+static void w_mulfx(u32* r, const u32* a, const u32* b, int t, wfx_t f) {
+    int WL = f.WL, LF = f.LF;
+    u32 x[256], y[256];
+    memcpy(x, a, 4 * (size_t)WL);
+    memcpy(y, b, 4 * (size_t)WL);
+    int sa = w_isneg(x, WL), sb = w_isneg(y, WL);
+    if (sa) w_negate(x, WL);
+    if (sb) w_negate(y, WL);
+    int ta = w_topl(x, WL), tb = w_topl(y, WL);
+    int la = t ? (ta - t + 1 > 0 ? ta - t + 1 : 0) : 0, lb = t ? (tb - t + 1 > 0 ? tb - t + 1 : 0) : 0;
+    int kmin = LF - 2 > la + lb ? LF - 2 : la + lb, kmax = ta + tb;
+    for (int j = 0; j < WL; j++) r[j] = 0;
+    uint64_t carry = 0, hprev = 0;
+    for (int k = kmin; k <= kmax + 2 && k < LF + WL; k++) {
+        uint64_t lo = 0, hi = 0;
+        if (k <= kmax) {
+            int i0 = la > k - tb ? la : k - tb, i1 = ta < k - lb ? ta : k - lb;
+            for (int i = i0; i <= i1; i++) { uint64_t p = (uint64_t)x[i] * y[k - i]; lo += (u32)p; hi += p >> 32; }
+        }
+        uint64_t s = lo + hprev + carry;
+        if (k >= LF) r[k - LF] = (u32)s;
+        carry = s >> 32;
+        hprev = hi;
+    }
+    if (sa ^ sb) w_negate(r, WL);
+}
+
+static double w_to_double(const u32* a, wfx_t f) {                 /* basecase.py:59-68 */
+    int WL = f.WL;
+    u32 x[256];
+    memcpy(x, a, 4 * (size_t)WL);
+    int s = w_isneg(x, WL);
+    if (s) w_negate(x, WL);
+    int t = w_topl(x, WL);
+    double v = (double)x[t];
+    if (t >= 1) v = v * 4294967296.0 + (double)x[t - 1];
+    if (t >= 2) v = v * 4294967296.0 + (double)x[t - 2];
+    int e = 32 * (t - 2 > 0 ? t - 2 : 0) - 32 * f.LF;
+    v = ldexp(v, e);
+    return s ? -v : v;
+}
+
+static void w_from_double(u32* r, double v, wfx_t f) {             /* basecase.py:69-78, v > 0 */
+    int WL = f.WL;
+    for (int i = 0; i < WL; i++) r[i] = 0;
+    if (!(v > 0.0) || !isfinite(v)) return;   /* non-positive pivot: reciprocal 0 (undefined in the reference) */
+    int e;
+    double m = frexp(v, &e);
+    uint64_t mi = (uint64_t)ldexp(m, 53);
+    int pos = e - 53 + 32 * f.LF;
+    if (pos < 0) { mi = (-pos < 64) ? (mi >> (-pos)) : 0; pos = 0; }
+    int li = pos / 32, sh = pos % 32;
+    unsigned __int128 w = ((unsigned __int128)mi) << sh;
+    for (int k = 0; k < 4 && li + k < WL; k++) { r[li + k] = (u32)(w & 0xFFFFFFFFu); w >>= 32; }
+}
+
+/* warp-cooperative d^{-1/2} of chol_tiles (basecase.py:145-156): same products as mulfx */
+static void w_rsqrt(u32* r, const u32* d, wfx_t f) {
+    int WL = f.WL;
+    u32 t1[256], t2[256], one[256];
+    w_from_double(r, 1.0 / sqrt(w_to_double(d, f)), f);
+    for (int i = 0; i < WL; i++) one[i] = 0;
+    one[f.LF] = 1;
+    for (int bits = 100;; bits *= 2) {
+        int t = bits / 32 + 2;
+        if (t >= WL) t = 0;
+        w_mulfx(t1, r, r, t, f);
+        w_mulfx(t2, t1, d, t, f);
+        memcpy(t1, one, 4 * (size_t)WL);
+        w_subfrom(t1, t2, WL);
+        w_mulfx(t2, r, t1, t, f);
+        w_shr1(t2, WL);
+        w_addto(r, t2, WL);
+        if (t == 0) break;
+    }
+}
+
+/* chol_tiles + upper-triangle clearing + trinv_tiles (basecase.py:159-226, batched.py:250-255) on nb
+ * tiles of BT x BT wide entries. W: (nb, BT, BT, WL) entry-major words, factored in place (lower L,
+ * upper cleared); RINV (nb, BT, WL) pivot reciprocals; X (nb, BT, BT, WL) = L^-1. */
+void orc_tile_chol(uint32_t* W, uint32_t* RINV, uint32_t* X, i64 nb, int BT, int WL, int LF) {
+    wfx_t f = {WL, LF};
+    #pragma omp parallel for schedule(dynamic, 1)
+    for (i64 b = 0; b < nb; b++) {
+        u32* T = W + (size_t)b * BT * BT * WL;
+        u32* RI = RINV + (size_t)b * BT * WL;
+        u32* XT = X + (size_t)b * BT * BT * WL;
+        u32 v[256];
+        for (int k = 0; k < BT; k++) {
+            u32* akk = T + ((size_t)k * BT + k) * WL;
+            u32 r[256];
+            w_rsqrt(r, akk, f);
+            w_mulfx(v, akk, r, 0, f);
+            memcpy(akk, v, 4 * (size_t)WL);
+            memcpy(RI + (size_t)k * WL, r, 4 * (size_t)WL);
+            for (int i = k + 1; i < BT; i++) {
+                u32* aik = T + ((size_t)i * BT + k) * WL;
+                w_mulfx(v, aik, RI + (size_t)k * WL, 0, f);
+                memcpy(aik, v, 4 * (size_t)WL);
+            }
+            for (int i = k + 1; i < BT; i++)
+                for (int j = k + 1; j <= i; j++) {
+                    w_mulfx(v, T + ((size_t)i * BT + k) * WL, T + ((size_t)j * BT + k) * WL, 0, f);
+                    w_subfrom(T + ((size_t)i * BT + j) * WL, v, WL);
+                }
+        }
+        for (int i = 0; i < BT; i++)
+            for (int j = i + 1; j < BT; j++) memset(T + ((size_t)i * BT + j) * WL, 0, 4 * (size_t)WL);
+        memset(XT, 0, 4 * (size_t)BT * BT * WL);
+        for (int i = 0; i < BT; i++) {
+            for (int k = 0; k <= i; k++) {
+                u32* xik = XT + ((size_t)i * BT + k) * WL;
+                if (k == i) { memcpy(xik, RI + (size_t)i * WL, 4 * (size_t)WL); continue; }
+                u32 acc[256];
+                memset(acc, 0, 4 * (size_t)WL);
+                for (int j = k; j < i; j++) {
+                    w_mulfx(v, T + ((size_t)i * BT + j) * WL, XT + ((size_t)j * BT + k) * WL, 0, f);
+                    w_addto(acc, v, WL);
+                }
+                w_mulfx(v, acc, RI + (size_t)i * WL, 0, f);
+                w_negate(v, WL);
+                memcpy(xik, v, 4 * (size_t)WL);
+            }
+        }
+    }
 }

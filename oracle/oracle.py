@@ -28,6 +28,8 @@ _SIGS = {
     "orc_mpf_gemm": [_P, _P, _I64, _I64, _I64, _I64, _I64, _I64, _I64, _I64, _I64, _P, _P, _I64, _I64, _I64,
                      _I64, _P, _P],
     "orc_gemm_exact": [_P, _P, _I64, _I64, _I64, _I64, _P, _I64],
+    "orc_bgemm_nt": [_P, _P, _I64, _I64, _I64, _I64, _I64, _P, _I32, _P, _P],
+    "orc_tile_chol": [_P, _P, _P, _I64, _I32, _I32, _I32],
 }
 _lib = None
 
@@ -177,3 +179,29 @@ def gemm_exact(A, Bt, nw):
     W = np.zeros((nw, M, N), np.int32)
     lib().orc_gemm_exact(_p(A), _p(Bt), L, M, N, K, _p(W), nw)
     return W
+
+
+def bgemm_nt(A, Bt, e_in, mode, oexp=None):
+    """batched.bgemm_nt exact product + _bcrt_round. A (L, nb, M, K), Bt (L, nb, N, K) limb planes,
+    e_in (nb, N) exponents of the exact product; mode 0 fixed (oexp (nb, N) given), 1 'col', 2 'mat'.
+    Returns (out (L, nb, M, N), oexp (nb, N))."""
+    A, Bt = _c(A, np.int32), _c(Bt, np.int32)
+    L, nb, M, K = A.shape
+    N = Bt.shape[2]
+    e_in = _c(np.broadcast_to(e_in, (nb, N)), np.int64)
+    oe = np.zeros((nb, N), np.int64) if oexp is None else _c(np.broadcast_to(oexp, (nb, N)), np.int64).copy()
+    out = np.zeros((L, nb, M, N), np.int32)
+    lib().orc_bgemm_nt(_p(A), _p(Bt), L, nb, M, N, K, _p(e_in), mode, _p(out), _p(oe))
+    return out, oe
+
+
+def tile_chol(W, LF):
+    """chol_tiles + clearing + trinv_tiles (basecase.py:159-226) on W (nb, BT, BT, WL) uint32 wide
+    fixed-point words (F = 32*LF fractional bits). Returns (L (nb, BT, BT, WL), RINV (nb, BT, WL),
+    X (nb, BT, BT, WL))."""
+    W = np.ascontiguousarray(W, dtype=np.uint32).copy()
+    nb, BT, _, WL = W.shape
+    RINV = np.zeros((nb, BT, WL), np.uint32)
+    X = np.zeros_like(W)
+    lib().orc_tile_chol(_p(W), _p(RINV), _p(X), nb, BT, WL, LF)
+    return W, RINV, X

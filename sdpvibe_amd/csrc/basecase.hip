@@ -1,0 +1,417 @@
+// Tile base case on gfx950: batched multiprecision Cholesky and triangular inverse of BT x BT
+// tiles in wide fixed point (WL = L + 24 words, F = 32 L fractional bits), the arithmetic of the
+// reference's chol_tiles / trinv_tiles (sdpvibe/basecase.py:13-246). One workgroup per tile.
+//
+// Every product is the reference's truncated product mulfx (columns below LF - 2 dropped, signs
+// applied to magnitudes), so results are bit-identical. Operands are staged as magnitudes into
+// per-thread LDS slots (thread-strided, conflict-free) and the product is streamed out word by
+// word into its consumer (store, subtract, accumulate). The pivot reciprocal square root (double
+// seed, precision-doubling Newton) runs on one wave, its column sums spread over the lanes.
+#include "common.hpp"
+
+namespace {
+
+constexpr int TT = 256;  // threads per tile workgroup
+
+struct Slot {  // thread-strided word array in LDS
+    u32* p;
+    __device__ __forceinline__ u32& operator[](int k) const { return p[k * TT]; }
+};
+
+// copy |v| into slot s (WL words), return the sign of v
+template <class Src>
+__device__ __forceinline__ bool load_mag(Slot s, const Src& v, int WL) {
+    const bool neg = (v(WL - 1) >> 31) != 0;
+    u64 c = 1;
+    for (int k = 0; k < WL; k++) {
+        u32 w = v(k);
+        if (neg) {
+            const u64 t = (u64)(~w) + c;
+            w = (u32)t;
+            c = t >> 32;
+        }
+        s[k] = w;
+    }
+    return neg;
+}
+
+__device__ __forceinline__ int topl_slot(Slot s, int WL) {
+    for (int i = WL - 1; i > 0; i--)
+        if (s[i]) return i;
+    return 0;
+}
+
+// Streams the words j = 0..WL-1 of mulfx(a, b) (sdpvibe/basecase.py:31-53) given magnitudes in x, y
+// and the product sign; out(j, word). Columns kmin..LF-1 only feed the carry chain.
+template <class Out>
+__device__ __forceinline__ void mulfx_stream(Slot x, Slot y, bool neg, int t, int WL, int LF, Out out) {
+    const int ta = topl_slot(x, WL), tb = topl_slot(y, WL);
+    const int la = t ? max(0, ta - t + 1) : 0, lb = t ? max(0, tb - t + 1) : 0;
+    const int kmin = max(LF - 2, la + lb), kmax = ta + tb;
+    u64 carry = 0, hprev = 0, nc = 1;
+    auto column = [&](int k) -> u64 {
+        u64 lo = 0, hi = 0;
+        if (k <= kmax) {
+            const int i0 = max(la, k - tb), i1 = min(ta, k - lb);
+            for (int i = i0; i <= i1; i++) {
+                const u64 p = (u64)x[i] * y[k - i];
+                lo += (u32)p;
+                hi += p >> 32;
+            }
+        }
+        const u64 sm = lo + hprev + carry;
+        carry = sm >> 32;
+        hprev = hi;
+        return sm;
+    };
+    for (int k = kmin; k < LF && k <= kmax + 2; k++) column(k);
+    for (int j = 0; j < WL; j++) {
+        const int k = LF + j;
+        u32 w = 0;
+        if (k >= kmin && k <= kmax + 2) w = (u32)column(k);
+        if (neg) {
+            const u64 tt = (u64)(~w) + nc;
+            w = (u32)tt;
+            nc = tt >> 32;
+        }
+        out(j, w);
+    }
+}
+
+struct GlobalEntry {  // WL consecutive words in global memory
+    const u32* p;
+    __device__ __forceinline__ u32 operator()(int k) const { return p[k]; }
+};
+struct LdsEntry {
+    const u32* p;
+    __device__ __forceinline__ u32 operator()(int k) const { return p[k]; }
+};
+
+// ---------------------------------------------------------------- wave-level pivot arithmetic
+// Arrays of WL words in LDS; called by all TT threads (threads >= 64 only meet the barriers).
+struct Pivot {
+    u32 *d, *r, *t1, *t2, *sx, *sy;
+    u64* col;  // 2 * (2 WL + 4)
+    int WL, LF;
+};
+
+__device__ __forceinline__ bool isneg_w(const u32* a, int WL) { return (a[WL - 1] >> 31) != 0; }
+
+// r = mulfx(a, b, t) with the column sums spread over wave 0, the carry chain on lane 0
+__device__ void wave_mul(const Pivot& P, u32* r, const u32* a, const u32* b, int t) {
+    const int tid = threadIdx.x, WL = P.WL, LF = P.LF;
+    __syncthreads();
+    const bool sa = isneg_w(a, WL), sb = isneg_w(b, WL);
+    if (tid == 0) {
+        u64 ca = 1, cb = 1;
+        for (int k = 0; k < WL; k++) {
+            u32 x = a[k], y = b[k];
+            if (sa) { const u64 q = (u64)(~x) + ca; x = (u32)q; ca = q >> 32; }
+            if (sb) { const u64 q = (u64)(~y) + cb; y = (u32)q; cb = q >> 32; }
+            P.sx[k] = x;
+            P.sy[k] = y;
+        }
+    }
+    __syncthreads();
+    int ta = 0, tb = 0;
+    for (int i = WL - 1; i > 0; i--)
+        if (P.sx[i]) { ta = i; break; }
+    for (int i = WL - 1; i > 0; i--)
+        if (P.sy[i]) { tb = i; break; }
+    const int la = t ? max(0, ta - t + 1) : 0, lb = t ? max(0, tb - t + 1) : 0;
+    const int kmin = max(LF - 2, la + lb), kmax = ta + tb;
+    if (tid < 64) {
+        for (int k = kmin + tid; k <= kmax; k += 64) {
+            u64 lo = 0, hi = 0;
+            const int i0 = max(la, k - tb), i1 = min(ta, k - lb);
+            for (int i = i0; i <= i1; i++) {
+                const u64 p = (u64)P.sx[i] * P.sy[k - i];
+                lo += (u32)p;
+                hi += p >> 32;
+            }
+            P.col[2 * (k - kmin)] = lo;
+            P.col[2 * (k - kmin) + 1] = hi;
+        }
+    }
+    __syncthreads();
+    if (tid == 0) {
+        for (int j = 0; j < WL; j++) r[j] = 0;
+        u64 carry = 0, hprev = 0;
+        for (int k = kmin; k <= kmax + 2 && k < LF + WL; k++) {
+            u64 lo = 0, hi = 0;
+            if (k <= kmax) {
+                lo = P.col[2 * (k - kmin)];
+                hi = P.col[2 * (k - kmin) + 1];
+            }
+            const u64 sm = lo + hprev + carry;
+            if (k >= LF) r[k - LF] = (u32)sm;
+            carry = sm >> 32;
+            hprev = hi;
+        }
+        if (sa ^ sb) {
+            u64 c = 1;
+            for (int j = 0; j < WL; j++) {
+                const u64 q = (u64)(~r[j]) + c;
+                r[j] = (u32)q;
+                c = q >> 32;
+            }
+        }
+    }
+    __syncthreads();
+}
+
+__device__ double to_double_w(const u32* a, int WL, int LF) {  // basecase.py:59-68 (one thread)
+    const bool s = isneg_w(a, WL);
+    auto mag = [&](int k) {  // word k of |a|
+        if (!s) return a[k];
+        u64 c = 1;
+        u32 w = 0;
+        for (int q = 0; q <= k; q++) {
+            const u64 t = (u64)(~a[q]) + c;
+            w = (u32)t;
+            c = t >> 32;
+        }
+        return w;
+    };
+    int t = 0;
+    for (int k = WL - 1; k > 0; k--)
+        if (mag(k)) { t = k; break; }
+    double v = (double)mag(t);
+    if (t >= 1) v = v * 4294967296.0 + (double)mag(t - 1);
+    if (t >= 2) v = v * 4294967296.0 + (double)mag(t - 2);
+    const int e = 32 * (t - 2 > 0 ? t - 2 : 0) - 32 * LF;
+    v = ldexp(v, e);
+    return s ? -v : v;
+}
+
+__device__ void from_double_w(u32* r, double v, int WL, int LF) {  // basecase.py:69-78
+    for (int i = 0; i < WL; i++) r[i] = 0;
+    if (!(v > 0.0) || !isfinite(v)) return;  // non-positive pivot: reciprocal 0
+    int e;
+    const double m = frexp(v, &e);
+    u64 mi = (u64)ldexp(m, 53);
+    int pos = e - 53 + 32 * LF;
+    if (pos < 0) {
+        mi = (-pos < 64) ? (mi >> (-pos)) : 0;
+        pos = 0;
+    }
+    const int li = pos / 32, sh = pos % 32;
+    u64 lo = mi << sh;
+    u64 hi = sh ? (mi >> (64 - sh)) : 0;
+    const u32 wds[4] = {(u32)lo, (u32)(lo >> 32), (u32)hi, (u32)(hi >> 32)};
+    for (int k = 0; k < 4 && li + k < WL; k++) r[li + k] = wds[k];
+}
+
+// r = d^{-1/2}: double seed, Newton r <- r + r (1 - d r^2) / 2 with precision doubling
+// (basecase.py:145-156)
+__device__ void wave_rsqrt(const Pivot& P) {
+    const int tid = threadIdx.x, WL = P.WL, LF = P.LF;
+    if (tid == 0) from_double_w(P.r, 1.0 / sqrt(to_double_w(P.d, WL, LF)), WL, LF);
+    for (int bits = 100;; bits *= 2) {
+        int t = bits / 32 + 2;
+        if (t >= WL) t = 0;
+        wave_mul(P, P.t1, P.r, P.r, t);
+        wave_mul(P, P.t2, P.t1, P.d, t);
+        if (tid == 0) {  // t1 = one - t2
+            i64 br = 0;
+            for (int i = 0; i < WL; i++) {
+                const i64 q = (i64)(i == LF ? 1u : 0u) - (i64)P.t2[i] + br;
+                P.t1[i] = (u32)q;
+                br = q >> 32;
+            }
+        }
+        wave_mul(P, P.t2, P.r, P.t1, t);
+        if (tid == 0) {  // r += t2 >> 1 (arithmetic)
+            u64 c = 0;
+            for (int i = 0; i < WL; i++) {
+                const u32 hw = (i < WL - 1) ? ((P.t2[i] >> 1) | (P.t2[i + 1] << 31)) : (u32)(((int)P.t2[i]) >> 1);
+                const u64 q = (u64)P.r[i] + hw + c;
+                P.r[i] = (u32)q;
+                c = q >> 32;
+            }
+        }
+        __syncthreads();
+        if (t == 0) break;
+    }
+}
+
+// ---------------------------------------------------------------- the tile kernel
+// W, X, S: AoS (nb, BT, BT, WL) words in global memory (W: working copy, factored in place;
+// X: L^-1; S: products of the inverse). LDS: 2 operand slots per thread + pivot area + RINV.
+__global__ void __launch_bounds__(TT) k_tile_factor(u32* __restrict__ Wg, u32* __restrict__ Xg, u32* __restrict__ Sg,
+                                                   int BT, int WL, int LF) {
+    SDPV_DYNAMIC_LDS(u32, lds);
+    const int tid = threadIdx.x;
+    const i64 tile = blockIdx.x;
+    u32* W = Wg + tile * BT * BT * WL;
+    u32* X = Xg + tile * BT * BT * WL;
+    u32* S = Sg + tile * BT * BT * WL;
+    Slot sx{lds + tid}, sy{lds + WL * TT + tid};
+    u32* piv = lds + 2 * WL * TT;
+    Pivot P;
+    P.d = piv;
+    P.r = piv + WL;
+    P.t1 = piv + 2 * WL;
+    P.t2 = piv + 3 * WL;
+    P.sx = piv + 4 * WL;
+    P.sy = piv + 5 * WL;
+    u32* rinv = piv + 6 * WL;                            // BT x WL
+    P.col = (u64*)(rinv + BT * WL + ((BT * WL) & 1));    // 2 (2 WL + 4) u64 (8-byte aligned)
+    P.WL = WL;
+    P.LF = LF;
+    auto ent = [&](u32* base, int i, int j) { return base + ((i64)i * BT + j) * WL; };
+
+    for (int k = 0; k < BT; k++) {
+        // pivot: d = A_kk, r = d^-1/2, A_kk = d r, RINV_k = r
+        for (int q = tid; q < WL; q += TT) P.d[q] = ent(W, k, k)[q];
+        __syncthreads();
+        wave_rsqrt(P);
+        wave_mul(P, P.t1, P.d, P.r, 0);
+        for (int q = tid; q < WL; q += TT) {
+            ent(W, k, k)[q] = P.t1[q];
+            rinv[k * WL + q] = P.r[q];
+        }
+        __syncthreads();
+        // column scale: A_ik = mulfx(A_ik, r)
+        for (int i = k + 1 + tid; i < BT; i += TT) {
+            u32* aik = ent(W, i, k);
+            const bool na = load_mag(sx, GlobalEntry{aik}, WL);
+            load_mag(sy, LdsEntry{rinv + k * WL}, WL);
+            mulfx_stream(sx, sy, na ^ isneg_w(rinv + k * WL, WL), 0, WL, LF, [&](int j, u32 w) { aik[j] = w; });
+        }
+        __syncthreads();
+        // trailing update over lower pairs: A_ij -= mulfx(A_ik, A_jk)
+        const int m = BT - k - 1, npairs = m * (m + 1) / 2;
+        for (int p = tid; p < npairs; p += TT) {
+            int ii = (int)((sqrtf(8.0f * p + 1.0f) - 1.0f) * 0.5f);
+            while ((ii + 1) * (ii + 2) / 2 <= p) ii++;
+            while (ii * (ii + 1) / 2 > p) ii--;
+            const int jj = p - ii * (ii + 1) / 2;
+            const int i = k + 1 + ii, j = k + 1 + jj;
+            const bool na = load_mag(sx, GlobalEntry{ent(W, i, k)}, WL);
+            const bool nb = load_mag(sy, GlobalEntry{ent(W, j, k)}, WL);
+            u32* aij = ent(W, i, j);
+            i64 br = 0;
+            mulfx_stream(sx, sy, na ^ nb, 0, WL, LF, [&](int jw, u32 w) {
+                const i64 q = (i64)aij[jw] - (i64)w + br;
+                aij[jw] = (u32)q;
+                br = q >> 32;
+            });
+        }
+        __syncthreads();
+    }
+    // clear the upper triangle; X = 0
+    for (int e = tid; e < BT * BT; e += TT) {
+        const int i = e / BT, j = e % BT;
+        for (int q = 0; q < WL; q++) {
+            if (j > i) W[(i64)e * WL + q] = 0;
+            X[(i64)e * WL + q] = 0;
+        }
+    }
+    __syncthreads();
+    // X = L^-1 row by row: x_ii = rinv_i, x_ik = -rinv_i * sum_{j=k}^{i-1} l_ij x_jk
+    for (int i = 0; i < BT; i++) {
+        const int np = i * (i + 1) / 2;  // pairs k <= j < i
+        for (int p = tid; p < np; p += TT) {
+            int jj = (int)((sqrtf(8.0f * p + 1.0f) - 1.0f) * 0.5f);
+            while ((jj + 1) * (jj + 2) / 2 <= p) jj++;
+            while (jj * (jj + 1) / 2 > p) jj--;
+            const int kk = p - jj * (jj + 1) / 2, j = jj;
+            const bool na = load_mag(sx, GlobalEntry{ent(W, i, j)}, WL);
+            const bool nb = load_mag(sy, GlobalEntry{ent(X, j, kk)}, WL);
+            u32* s = ent(S, kk, j);
+            mulfx_stream(sx, sy, na ^ nb, 0, WL, LF, [&](int jw, u32 w) { s[jw] = w; });
+        }
+        __syncthreads();
+        for (int kk = tid; kk <= i; kk += TT) {
+            u32* xik = ent(X, i, kk);
+            if (kk == i) {
+                for (int q = 0; q < WL; q++) xik[q] = rinv[i * WL + q];
+                continue;
+            }
+            // acc = sum_j S[kk][j] (exact, wrapping) into slot sx, then x_ik = -(acc * rinv_i)
+            u64 c = 0;
+            for (int q = 0; q < WL; q++) {
+                u64 sm = c;
+                for (int j = kk; j < i; j++) sm += ent(S, kk, j)[q];
+                sy[q] = (u32)sm;  // raw sum words in sy (two's complement)
+                c = sm >> 32;
+            }
+            // magnitude of acc into sx
+            const bool na = (sy[WL - 1] >> 31) != 0;
+            u64 cc = 1;
+            for (int q = 0; q < WL; q++) {
+                u32 w = sy[q];
+                if (na) { const u64 tq = (u64)(~w) + cc; w = (u32)tq; cc = tq >> 32; }
+                sx[q] = w;
+            }
+            load_mag(sy, LdsEntry{rinv + i * WL}, WL);
+            const bool neg = !(na ^ isneg_w(rinv + i * WL, WL));  // negated product
+            mulfx_stream(sx, sy, neg, 0, WL, LF, [&](int jw, u32 w) { xik[jw] = w; });
+        }
+        __syncthreads();
+    }
+}
+
+// planes (L limbs, value at exponent efx) -> AoS wide words (value * 2^40 in WL words)
+__global__ void __launch_bounds__(256) k_tile_widen(const i32* __restrict__ A, i64 plane, int L, int WL,
+                                                    u32* __restrict__ W, i64 n) {
+    const i64 e = blockIdx.x * (i64)256 + threadIdx.x;
+    if (e >= n) return;
+    struct Src {
+        const i32* p;
+        i64 s;
+        __device__ __forceinline__ u32 operator()(int k) const { return (u32)p[(i64)k * s]; }
+    } src{A + e, plane};
+    u32* out = W + e * WL;
+    shr_round(src, L, -40, WL, [&](int k, u32 w) { out[k] = w; });
+}
+
+// AoS wide -> planes: L (rounded, value / 2^40 into L limbs) and X (WL words)
+__global__ void __launch_bounds__(256) k_tile_narrow(const u32* __restrict__ W, const u32* __restrict__ X, int L,
+                                                     int WL, i32* __restrict__ Lout, i32* __restrict__ Xw,
+                                                     i64 plane, i64 n, i32* __restrict__ flag) {
+    const i64 e = blockIdx.x * (i64)256 + threadIdx.x;
+    if (e >= n) return;
+    const u32* w = W + e * WL;
+    const int bad = shr_round([&](int k) { return w[k]; }, WL, 40, L,
+                              [&](int k, u32 v) { Lout[(i64)k * plane + e] = (i32)v; });
+    if (bad && flag) atomicOr(flag, 1);
+    const u32* x = X + e * WL;
+    for (int k = 0; k < WL; k++) Xw[(i64)k * plane + e] = (i32)x[k];
+}
+
+inline unsigned nblk(i64 n, int tpb) { return (unsigned)((n + tpb - 1) / tpb); }
+
+}  // namespace
+
+extern "C" {
+
+int sdpv_tile_chol(const int32_t* A, int64_t L, int64_t nb, int64_t BT, int32_t* Lout, int32_t* Xw, uint32_t* work,
+                   int32_t* flag, void* stream) {
+    if (L < 1 || nb < 0 || BT < 1 || BT > 64) return SDPV_EARG;
+    if (nb == 0) return SDPV_OK;
+    if (!A || !Lout || !Xw || !work) return SDPV_EARG;
+    const int WL = (int)L + 24;
+    const size_t lds = (size_t)2 * WL * TT * 4 + (size_t)(6 * WL + BT * WL + 2) * 4 + (size_t)2 * (2 * WL + 4) * 8;
+    if (lds > 160 * 1024) return SDPV_ESIZE;
+    hipStream_t st = as_stream(stream);
+    const i64 n = nb * BT * BT;
+    u32* W = work;
+    u32* X = work + n * WL;
+    u32* S = work + 2 * n * WL;
+    hipLaunchKernelGGL(k_tile_widen, dim3(nblk(n, 256)), dim3(256), 0, st, A, n, (int)L, WL, W, n);
+    SDPV_TRY_LAUNCH();
+    static bool attr = false;
+    if (!attr) {
+        (void)hipFuncSetAttribute((const void*)k_tile_factor, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+        attr = true;
+    }
+    hipLaunchKernelGGL(k_tile_factor, dim3((unsigned)nb), dim3(TT), lds, st, W, X, S, (int)BT, WL, (int)L);
+    SDPV_TRY_LAUNCH();
+    hipLaunchKernelGGL(k_tile_narrow, dim3(nblk(n, 256)), dim3(256), 0, st, W, X, (int)L, WL, Lout, Xw, n, n, flag);
+    SDPV_TRY_LAUNCH();
+    return SDPV_OK;
+}
+
+}  // extern "C"

@@ -34,6 +34,13 @@ inline hipError_t hipFuncSetAttribute(const void*, int, int) { return hipSuccess
 extern thread_local dim3 threadIdx, blockIdx, blockDim, gridDim;
 void emu_syncthreads();
 inline void __syncthreads() { emu_syncthreads(); }
+using std::isfinite;
+using std::frexp;
+using std::ldexp;
+inline int max(int a, int b) { return a > b ? a : b; }
+inline int min(int a, int b) { return a < b ? a : b; }
+inline long long max(long long a, long long b) { return a > b ? a : b; }
+inline long long min(long long a, long long b) { return a < b ? a : b; }
 inline int __clz(int x) { return x == 0 ? 32 : __builtin_clz((unsigned)x); }
 inline int atomicOr(int* p, int v) { return __atomic_fetch_or(p, v, __ATOMIC_SEQ_CST); }
 inline long long atomicMax(long long* p, long long v) {

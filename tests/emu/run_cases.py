@@ -66,6 +66,61 @@ def gpu_test_gemm_cases():
 CASES["gpu_gemm"] = gpu_test_gemm_cases
 
 
+def bgemm_cases():
+    from sdpvibe_amd import batched as Bm
+    from tests.intutil import to_planes as tp
+    ok = True
+    L, nb, M, N, K = 3, 2, 33, 7, 20
+    rng = random.Random(5)
+    A = tp([rng.randrange(-(1 << 56), 1 << 56) for _ in range(nb * M * K)], L).reshape(L, nb, M, K)
+    Bt = tp([rng.randrange(-(1 << 56), 1 << 56) for _ in range(nb * N * K)], L).reshape(L, nb, N, K)
+    eA, eB = -56, -57
+    e_in = np.full((nb, N), eA + eB, np.int64)
+    for mode, om in (("col", 1), ("mat", 2), (40, 0)):
+        C = Bm.bgemm_nt(Bm.BMP(torch.from_numpy(A), eA), Bm.BMP(torch.from_numpy(Bt), eB), out_exp=mode)
+        ref, rexp = O.bgemm_nt(A, Bt, e_in, om, np.full((nb, N), 40, np.int64) if om == 0 else None)
+        good = np.array_equal(C.limbs.numpy(), ref)
+        if om != 0:
+            good &= np.array_equal(C.exp.numpy(), rexp)
+        print(f"bgemm_nt mode={mode}: {'ok' if good else 'MISMATCH'}", flush=True)
+        ok &= good
+    return ok
+
+
+CASES["bgemm"] = bgemm_cases
+
+
+def tile_case(L, nb, BT, logc, seed):
+    """sdpv_tile_chol against the oracle's chol_tiles/trinv_tiles restatement, bit for bit"""
+    from sdpvibe_amd.basecase import tile_factor
+    from tests.intutil import int_to_words, rnd_half_up, spd_tile, words_to_int, wrap
+    WL = L + 24
+    tiles = [spd_tile(BT, logc, seed + b, 32 * L - 40) for b in range(nb)]
+    flat = [tiles[b][i][j] for b in range(nb) for i in range(BT) for j in range(BT)]
+    A = to_planes(flat, L).reshape(L, nb, BT, BT)
+    W = np.zeros((nb, BT, BT, WL), np.uint32)
+    for b in range(nb):
+        for i in range(BT):
+            for j in range(BT):
+                W[b, i, j] = int_to_words(tiles[b][i][j] << 40, WL)
+    Lw, _, X = O.tile_chol(W, L)
+    Lo, Xw = tile_factor(torch.from_numpy(A), BT)
+    exp_L = to_planes([wrap(rnd_half_up(words_to_int(Lw[b, i, j]), 40), L) for b in range(nb) for i in range(BT)
+                       for j in range(BT)], L).reshape(L, nb, BT, BT)
+    exp_X = np.ascontiguousarray(X.transpose(3, 0, 1, 2)).view(np.int32)
+    okL = np.array_equal(Lo.numpy(), exp_L)
+    okX = np.array_equal(Xw.numpy(), exp_X)
+    print(f"tile_chol L={L} nb={nb} BT={BT} logc={logc}: L {'ok' if okL else 'MISMATCH'}, "
+          f"X {'ok' if okX else 'MISMATCH'}", flush=True)
+    if not okX:
+        bad = np.argwhere(Xw.numpy() != exp_X)
+        print("first X mismatches (word, b, i, j):", bad[:8].tolist(), flush=True)
+    return okL and okX
+
+
+CASES["tile"] = lambda: tile_case(2, 2, 32, 6, 11) and tile_case(4, 1, 20, 12, 3)
+
+
 if __name__ == "__main__":  # noqa: E305
     ok = True
     for c in sys.argv[1:] or list(CASES):

@@ -72,3 +72,32 @@ def rand_mpf_ints(rng, n, L, espread=2000):
     vals = vals[:n]
     exps = [rng.randrange(-espread, espread) for _ in vals]
     return vals, exps
+
+
+def words_to_int(w):
+    """1-d array of uint32/int32 words (little endian, two's complement) -> Python int"""
+    w = [int(x) & MASK for x in w]
+    v = 0
+    for x in reversed(w):
+        v = (v << 32) | x
+    if w and w[-1] >> 31:
+        v -= 1 << (32 * len(w))
+    return v
+
+
+def int_to_words(v, n):
+    u = v % (1 << (32 * n))
+    return [(u >> (32 * k)) & MASK for k in range(n)]
+
+
+def spd_tile(n, logcond, seed, F):
+    """equilibrated SPD matrix (unit diagonal, cond ~ 10^logcond) as Python ints scaled by 2^F"""
+    import mpmath as mpm
+    rng = np.random.default_rng(seed)
+    Qd, _ = np.linalg.qr(rng.standard_normal((n, n)))
+    mpm.mp.prec = 4 * F + 200
+    Q = mpm.matrix(Qd.tolist())
+    D = mpm.diag([mpm.mpf(10) ** (-mpm.mpf(logcond) * k / max(n - 1, 1)) for k in range(n)])
+    A = Q * D * Q.T
+    d = [1 / mpm.sqrt(A[i, i]) for i in range(n)]
+    return [[int(mpm.nint(A[i, j] * d[i] * d[j] * mpm.mpf(2) ** F)) for j in range(n)] for i in range(n)]

@@ -1,0 +1,124 @@
+"""GPU parity of the batched layer (sdpvibe/batched.py) against the oracle."""
+import random
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import oracle as O
+from sdpvibe_amd import batched as Bm
+from tests.intutil import from_planes, to_planes
+
+pytestmark = pytest.mark.gpu
+
+
+def cu(a):
+    return torch.from_numpy(np.ascontiguousarray(a)).cuda()
+
+
+def _eq(a, b, what):
+    a, b = np.asarray(a), np.asarray(b)
+    assert a.shape == b.shape, (what, a.shape, b.shape)
+    bad = np.argwhere(a != b)
+    assert bad.size == 0, f"{what}: {len(bad)} mismatches, first at {bad[:5].tolist()}"
+
+
+def _fx(rng, L, shape, bits):
+    n = int(np.prod(shape))
+    return to_planes([rng.randrange(-(1 << bits), 1 << bits) for _ in range(n)], L).reshape((L,) + shape)
+
+
+@pytest.mark.parametrize("L,nb,M,N,K", [(3, 2, 4, 5, 6), (24, 3, 32, 32, 32), (24, 5, 40, 7, 96), (34, 2, 64, 100, 33)])
+@pytest.mark.parametrize("mode", ["fixed", "col", "mat", "rowexp_col", "rowexp_fixed"])
+def test_bgemm_nt(L, nb, M, N, K, mode):
+    rng = random.Random(L * 100 + M + N + K + len(mode))
+    A = _fx(rng, L, (nb, M, K), 32 * L - 40)
+    Bt = _fx(rng, L, (nb, N, K), 32 * L - 40)
+    eA, eB = -(32 * L - 40), -(32 * L - 41)
+    rowexp = np.array([[rng.randrange(-3000, -2000) for _ in range(N)] for _ in range(nb)], np.int64)
+    use_row = mode.startswith("rowexp")
+    e_in = (eA + rowexp) if use_row else np.full((nb, N), eA + eB, np.int64)
+    Ab = Bm.BMP(cu(A), eA)
+    Bb = Bm.BMP(cu(Bt), eB)
+    if mode in ("fixed", "rowexp_fixed"):
+        oe = int(e_in.max()) + 2 * (32 * L - 40) + 8 - (32 * L - 32)
+        C = Bm.bgemm_nt(Ab, Bb, out_exp=oe, bt_rowexp=cu(rowexp) if use_row else None)
+        ref, rexp = O.bgemm_nt(A, Bt, e_in, 0, np.full((nb, N), oe, np.int64))
+        assert C.exp == oe
+    else:
+        omode = {"col": "col", "mat": "mat", "rowexp_col": "col"}[mode]
+        C = Bm.bgemm_nt(Ab, Bb, out_exp=omode, bt_rowexp=cu(rowexp) if use_row else None)
+        ref, rexp = O.bgemm_nt(A, Bt, e_in, 2 if omode == "mat" else 1)
+        _eq(C.exp.cpu().numpy(), rexp, "bgemm exps")
+    _eq(C.limbs.cpu().numpy(), ref, f"bgemm_nt {mode}")
+
+
+@pytest.mark.parametrize("L", [2, 24])
+def test_brescale_and_badd(L):
+    rng = random.Random(L)
+    nb, R, C = 3, 5, 7
+    A = _fx(rng, L, (nb, R, C), 32 * L - 3)
+    B = _fx(rng, L, (nb, R, C), 32 * L - 3)
+    ea = np.array([[rng.randrange(-500, -400) for _ in range(C)] for _ in range(nb)], np.int64)
+    eb = np.array([[rng.randrange(-500, -400) for _ in range(C)] for _ in range(nb)], np.int64)
+    for sgn in (1, -1):
+        out = Bm.badd(Bm.BMP(cu(A), cu(ea)), Bm.BMP(cu(B), cu(eb)), sgn, out_exp="align")
+        ref, rEO = O.badd_align(A, B, ea, eb, sgn)
+        _eq(out.limbs.cpu().numpy(), ref, "badd")
+        _eq(out.exp.cpu().numpy(), rEO, "badd exp")
+    same = Bm.badd(Bm.BMP(cu(A), -7), Bm.BMP(cu(B), -7), -1, out_exp=-7)
+    _eq(same.limbs.cpu().numpy(), O.add_same(A.reshape(L, -1), B.reshape(L, -1), -1).reshape(A.shape), "badd same")
+    ne = ea + 3
+    r = Bm.brescale(Bm.BMP(cu(A), cu(ea)), cu(ne), L + 1)
+    s = np.broadcast_to((ne - ea)[:, None, :], (nb, R, C)).reshape(-1)
+    ref, _ = O.shift_round(A.reshape(L, -1), s, L + 1)
+    _eq(r.limbs.cpu().numpy().reshape(L + 1, -1), ref, "brescale")
+
+
+def _tile_inputs(L, nb, BT, logc, seed):
+    from tests.intutil import int_to_words, spd_tile
+    WL = L + 24
+    tiles = [spd_tile(BT, logc, seed + b, 32 * L - 40) for b in range(nb)]
+    flat = [tiles[b][i][j] for b in range(nb) for i in range(BT) for j in range(BT)]
+    A = to_planes(flat, L).reshape(L, nb, BT, BT)
+    W = np.zeros((nb, BT, BT, WL), np.uint32)
+    for b in range(nb):
+        for i in range(BT):
+            for j in range(BT):
+                W[b, i, j] = int_to_words(tiles[b][i][j] << 40, WL)
+    return A, W
+
+
+@pytest.mark.parametrize("L,nb,BT,logc", [(2, 2, 32, 6), (5, 1, 17, 10), (8, 3, 32, 30), (24, 4, 32, 60),
+                                          (24, 2, 64, 40), (40, 1, 48, 90)])
+def test_tile_chol(L, nb, BT, logc):
+    """sdpv_tile_chol (widen, factor, invert, narrow) bit for bit against the oracle's
+    chol_tiles/trinv_tiles restatement"""
+    from sdpvibe_amd.basecase import tile_factor
+    from tests.intutil import rnd_half_up, words_to_int, wrap
+    A, W = _tile_inputs(L, nb, BT, logc, 7 * L + BT)
+    Lw, _, X = O.tile_chol(W, L)
+    Lo, Xw = tile_factor(cu(A), BT)
+    exp_L = to_planes([wrap(rnd_half_up(words_to_int(Lw[b, i, j]), 40), L) for b in range(nb) for i in range(BT)
+                       for j in range(BT)], L).reshape(L, nb, BT, BT)
+    _eq(Xw.cpu().numpy(), np.ascontiguousarray(X.transpose(3, 0, 1, 2)).view(np.int32), "tile inverse")
+    _eq(Lo.cpu().numpy(), exp_L, "tile factor")
+
+
+def test_bchol_tile_inverse_exponents():
+    """bchol_tile: X rounded to one exponent per block, top of X at the headroom below 32L bits"""
+    from sdpvibe_amd.mpmat import headroom_bits
+    L, nb, BT = 24, 3, 32
+    A, W = _tile_inputs(L, nb, BT, 40, 5)
+    Lw, _, X = O.tile_chol(W, L)
+    Lt, t = Bm.bchol_tile(Bm.BMP(cu(A), -(32 * L - 40)))
+    ex = t.X.exp.cpu().numpy()
+    assert (ex == ex[:, :1]).all()
+    from tests.intutil import words_to_int
+    for b in range(nb):
+        xs = [words_to_int(X[b, i, j]) for i in range(BT) for j in range(BT)]
+        top = max(abs(v).bit_length() if v >= 0 else (~v).bit_length() for v in xs) - 32 * L
+        assert ex[b, 0] == top - headroom_bits(L)
+        s = np.full(BT * BT, int(ex[b, 0]) + 32 * L, np.int64)
+        ref, _ = O.shift_round(to_planes(xs, L + 24), s, L)
+        _eq(t.X.limbs[:, b].reshape(L, -1).cpu().numpy(), ref, "bchol_tile X")

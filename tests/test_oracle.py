@@ -307,3 +307,73 @@ def test_mpf_gemm_error_in_bits():
             bound = max(bound, mp.mpf(2) ** (2 - NB) * K * rowmax * colmax)
             c = val(got[i * N + j], int(oe[0, i, j]))
             assert abs(c - exact) <= bound
+
+
+@pytest.mark.parametrize("mode", [0, 1, 2])
+def test_bgemm_nt_oracle_against_python(mode):
+    """batched._bcrt_round rules (sdpvibe/batched.py:200-231) on Python integers"""
+    L, nb, M, N, K = 3, 2, 4, 5, 6
+    rng = random.Random(9 + mode)
+    lim = 1 << (32 * L - 40)
+    A = [[[rng.randrange(-lim, lim) for _ in range(K)] for _ in range(M)] for _ in range(nb)]
+    Bt = [[[rng.randrange(-lim, lim) for _ in range(K)] for _ in range(N)] for _ in range(nb)]
+    Bt[1][2] = [0] * K                                   # an all-zero column
+    e_in = np.array([[rng.randrange(-50, 50) for _ in range(N)] for _ in range(nb)], np.int64)
+    oexp0 = e_in + np.array([[rng.randrange(-10, 140) for _ in range(N)] for _ in range(nb)])
+    Ap = to_planes([v for b in A for r in b for v in r], L).reshape(L, nb, M, K)
+    Bp = to_planes([v for b in Bt for r in b for v in r], L).reshape(L, nb, N, K)
+    out, oe = O.bgemm_nt(Ap, Bp, e_in, mode, oexp0 if mode == 0 else None)
+    got = from_planes(out.reshape(L, -1))
+    H = 32 * L - 32
+    for b in range(nb):
+        exact = [[sum(A[b][i][k] * Bt[b][j][k] for k in range(K)) for j in range(N)] for i in range(M)]
+        if mode == 0:
+            ex = [int(oexp0[b, j]) for j in range(N)]
+        else:
+            ex = [int(e_in[b, j]) + max(bitlen_tc(exact[i][j]) for i in range(M)) - H for j in range(N)]
+        vals = [[wrap(rnd_half_up(exact[i][j], ex[j] - int(e_in[b, j])), L) for j in range(N)] for i in range(M)]
+        if mode == 2:
+            emax = max(ex)
+            vals = [[wrap(rnd_half_up(vals[i][j], emax - ex[j]), L) for j in range(N)] for i in range(M)]
+            ex = [emax] * N
+        assert [int(v) for v in oe[b]] == ex
+        for i in range(M):
+            for j in range(N):
+                assert got[(b * M + i) * N + j] == vals[i][j], (b, i, j)
+
+
+@pytest.mark.parametrize("L,logc", [(4, 10), (8, 60), (24, 80)])
+def test_tile_chol_oracle_error_bits(L, logc):
+    """chol_tiles/trinv_tiles restatement: L L^T = A and X L = I to the wide precision, and
+    |L - L_exact| against an mpmath Cholesky at twice the precision (errors in bits)."""
+    from tests.intutil import int_to_words, spd_tile, words_to_int
+    BT, LF = 32, L
+    WL, F = L + 24, 32 * L
+    Ai = spd_tile(BT, logc, L, F)
+    W = np.zeros((1, BT, BT, WL), np.uint32)
+    for i in range(BT):
+        for j in range(BT):
+            W[0, i, j] = int_to_words(Ai[i][j], WL)
+    Lw, RINV, X = O.tile_chol(W, LF)
+    Li = [[words_to_int(Lw[0, i, j]) for j in range(BT)] for i in range(BT)]
+    Xi = [[words_to_int(X[0, i, j]) for j in range(BT)] for i in range(BT)]
+    assert all(Li[i][j] == 0 for i in range(BT) for j in range(i + 1, BT))
+    assert all(Li[i][i] > 0 for i in range(BT))
+    one = 1 << F
+    # backward error of the factor: max |L L^T - A| in units of 2^-F
+    bw = max(abs(sum(Li[i][k] * Li[j][k] for k in range(j + 1)) - (Ai[i][j] << F)) for i in range(BT)
+             for j in range(i + 1))
+    assert bw.bit_length() <= F + 8, bw.bit_length() - F          # |LL^T - A| <= 2^(8-F)
+    # X L = I: error relative to ||X|| ~ cond
+    xl = max(abs(sum(Xi[i][k] * Li[k][j] for k in range(BT)) - (one << F if i == j else 0))
+             for i in range(BT) for j in range(BT))
+    xmax = max(abs(v) for r in Xi for v in r)
+    assert xl.bit_length() <= xmax.bit_length() + F + 8
+    # forward error against mpmath at 2x precision
+    import mpmath as mpm
+    mpm.mp.prec = 2 * F + 200
+    Am = mpm.matrix([[mpm.mpf(v) / mpm.mpf(2) ** F for v in r] for r in Ai])
+    Lr = mpm.cholesky(Am)
+    fe = max(abs(mpm.mpf(Li[i][j]) / mpm.mpf(2) ** F - Lr[i, j]) for i in range(BT) for j in range(i + 1))
+    cond_bits = int(logc * 3.33)
+    assert fe == 0 or float(mpm.log(fe, 2)) <= -F + cond_bits / 2 + 12
