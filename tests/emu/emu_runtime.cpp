@@ -6,8 +6,7 @@
 
 thread_local dim3 threadIdx, blockIdx, blockDim, gridDim;
 static constexpr size_t LDS_MAX = 160 * 1024;
-alignas(64) static unsigned char g_lds_store[LDS_MAX];
-unsigned char* emu_dynamic_lds = g_lds_store;
+unsigned char* emu_dynamic_lds = nullptr;
 
 namespace {
 struct Wave {
@@ -64,10 +63,13 @@ void emu_run_grid(dim3 grid, dim3 block, size_t lds_bytes, const std::function<v
     }
     unsigned nt = block.x * block.y * block.z;
     g_lds_bytes = lds_bytes;
+    // exactly the requested size on the heap, so AddressSanitizer reports any access past it
+    std::unique_ptr<unsigned char[]> lds(new unsigned char[lds_bytes ? lds_bytes : 16]);
+    emu_dynamic_lds = lds.get();
     for (unsigned bz = 0; bz < grid.z; bz++)
         for (unsigned by = 0; by < grid.y; by++)
             for (unsigned bx = 0; bx < grid.x; bx++) {
-                memset(g_lds_store, 0xA5, sizeof g_lds_store);   // LDS starts as garbage
+                memset(lds.get(), 0xA5, lds_bytes);   // LDS starts as garbage
                 g_block = std::make_unique<std::barrier<>>(nt);
                 unsigned nw = (nt + 63) / 64;
                 g_waves = std::vector<Wave>(nw);
