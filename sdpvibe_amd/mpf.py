@@ -9,6 +9,8 @@ Elementwise ops are gfx950 kernels (one thread per entry); matrix products go th
 RNS GEMM on INT8 matrix cores with operands aligned per row (A) / per column (B) -- the usual
 normwise floating-point error -- and every output entry rounded once.
 """
+import os
+
 import mpmath as mp
 import numpy as np
 import torch
@@ -379,3 +381,95 @@ def gemm(A, B, transA=False, transB=False):
 def symmetrize(A):
     """(A + A^T)/2"""
     return add(A, A.T).shift(-1)
+
+
+# ------------------------------------------------------------------ Cholesky (equilibrated, batched)
+def fx_exp(L):
+    """exponent of the equilibrated fixed-point format: |x| <= 1 with 40 bits of integer headroom
+    below the top limb (linalg.py:19)"""
+    return -(32 * L - 40)
+
+
+class Chol:
+    """A = L L^T for one (n, n) or nb batched (nb, n, n) SPD matrices, with L = D^-1 Lt,
+    D = diag(2^-k): Lt is fixed point at fx_exp(L), k integer per row (mpf.py:461-532).
+
+    The equilibrated D A D (unit-order diagonal) is padded with identity to a multiple of bt and
+    factored by the batched recursive Cholesky; solves go through the explicit tile inverses."""
+
+    SOLVE_REFINE = int(os.environ.get("SDPB_GPU_SOLVE_REFINE", "0"))
+
+    def __init__(self, A, bt=32, refine=1):
+        from .batched import BMP, bchol
+        self.two_d = A.e.dim() == 2
+        if self.two_d:
+            A = MPF(A.m[:, None], A.e[None])
+        nb, n, _ = A.shape
+        L = A.L
+        self.nb, self.n, self.L, self.bt, self.refine = nb, n, L, bt, refine
+        d = torch.arange(n, device=A.e.device)
+        top_d = _top(A)[:, d, d]                                    # (nb, n)
+        neg_d = A.m[-1][:, d, d] < 0
+        if bool((neg_d | (top_d <= EZ // 2)).any()):
+            raise ValueError("matrix is not positive definite (diagonal)")
+        k = torch.div(top_d + 1, 2, rounding_mode="floor")          # A_ii < 2^(2 k_i)
+        self.k = k
+        self.npad = npad = max(bt, -(-n // bt) * bt)
+        efx = fx_exp(L)
+        # (D A D)_ij = m_ij 2^(e_ij - k_i - k_j), re-rounded to the fixed-point exponent
+        s = efx - (A.e - k[:, :, None] - k[:, None, :])
+        s = torch.where(A.e == EZ, torch.full_like(s, 1 << 30), s).reshape(-1)
+        Af = shift_round(A.m.reshape(L, -1), s, L).view(L, nb, n, n)
+        P = torch.zeros((L, nb, npad, npad), dtype=torch.int32, device=A.m.device)
+        P[:, :, :n, :n] = Af
+        if npad > n:                                                # identity padding: 1 = 2^-efx units
+            one = 1 << (-efx)
+            pad = torch.arange(n, npad, device=A.m.device)
+            for w in range(L):
+                v = (one >> (32 * w)) & 0xFFFFFFFF
+                P[w][:, pad, pad] = v - (1 << 32) if v >> 31 else v
+        self.Lt, self.tiles = bchol(BMP(P, efx), bt, refine)
+
+    def _to_bmp(self, B):
+        """right-hand sides (nb, n, m) -> fixed point with one exponent per (block, column)"""
+        from .batched import BMP
+        L = self.L
+        Bm, cE = _align_rows(B.T)                                   # (L, nb, m, n), (nb, m)
+        P = torch.zeros((L, self.nb, self.npad, B.shape[-1]), dtype=torch.int32, device=Bm.device)
+        P[:, :, :self.n] = Bm.transpose(2, 3)
+        return BMP(P, cE)
+
+    def _from_bmp(self, P):
+        L, m = self.L, P.shape[2]
+        X = P.limbs[:, :, :self.n].reshape(L, -1).contiguous()
+        ein = P.expt()[:, None, :].expand(self.nb, self.n, m).reshape(-1).contiguous()
+        return normalize(X, ein, L).reshape(self.nb, self.n, m)
+
+    def _io(self, B, f):
+        if self.two_d:
+            r = f(MPF(B.m[:, None], B.e[None]))
+            return MPF(r.m[:, 0], r.e[0])
+        return f(B)
+
+    def solve_lower(self, B):
+        """L^-1 B = Lt^-1 (D B)"""
+        from .batched import btrsm_left
+
+        def f(B):
+            P = btrsm_left(self.Lt, self._to_bmp(B.shift(-self.k[:, :, None])), self.tiles, self.bt,
+                           self.SOLVE_REFINE)
+            return self._from_bmp(P)
+        return self._io(B, f)
+
+    def solve_upper(self, B):
+        """L^-T B = D (Lt^-T B)"""
+        from .batched import btrsm_left_T
+
+        def f(B):
+            P = btrsm_left_T(self.Lt, self._to_bmp(B), self.tiles, self.bt, self.SOLVE_REFINE)
+            return self._from_bmp(P).shift(-self.k[:, :, None])
+        return self._io(B, f)
+
+    def solve(self, B):
+        """A^-1 B"""
+        return self.solve_upper(self.solve_lower(B))

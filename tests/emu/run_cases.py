@@ -121,6 +121,19 @@ def tile_case(L, nb, BT, logc, seed):
 CASES["tile"] = lambda: tile_case(2, 2, 32, 6, 11) and tile_case(4, 1, 20, 12, 3)
 
 
+def chol_cases():
+    from tests.test_gpu_chol import chol_parity
+    import tests.test_gpu_chol as T
+    T.cu = lambda a: torch.from_numpy(np.ascontiguousarray(a))
+    chol_parity(2, 2, 40, 3, 5, seed=1)
+    chol_parity(3, 1, 20, 1, 3, seed=2, two_d=True)
+    print("chol: ok", flush=True)
+    return True
+
+
+CASES["chol"] = chol_cases
+
+
 if __name__ == "__main__":  # noqa: E305
     ok = True
     for c in sys.argv[1:] or list(CASES):

@@ -101,3 +101,43 @@ def spd_tile(n, logcond, seed, F):
     A = Q * D * Q.T
     d = [1 / mpm.sqrt(A[i, i]) for i in range(n)]
     return [[int(mpm.nint(A[i, j] * d[i] * d[j] * mpm.mpf(2) ** F)) for j in range(n)] for i in range(n)]
+
+
+def spd_blocks(nb, n, L, seed, uspread=20, gbits=None):
+    """nb SPD matrices A_b = D (G G^T + n I) D with D = diag(2^u), u uniform in [-uspread, uspread],
+    as normalised MPF host arrays. Returns (m (L, nb, n, n) int32, e (nb, n, n) int64, exact) with
+    exact[b][i][j] = (integer, exponent) of the value before rounding to NB bits."""
+    rng = random.Random(seed)
+    NB = 32 * L - 2
+    gb = gbits or max(8, NB // 2 + 4)
+    m = np.zeros((nb, n, n), dtype=object)
+    e = np.zeros((nb, n, n), np.int64)
+    exact = []
+    for b in range(nb):
+        G = [[rng.randrange(-(1 << gb), 1 << gb) for _ in range(n + 2)] for _ in range(n)]
+        u = [rng.randint(-uspread, uspread) for _ in range(n)]
+        ex_b = []
+        for i in range(n):
+            row = []
+            for j in range(n):
+                v = sum(G[i][t] * G[j][t] for t in range(n + 2)) + (n << (2 * gb) if i == j else 0)
+                ev = u[i] + u[j]
+                row.append((v, ev))
+                r, sh = norm_closed_form(v, NB)
+                m[b, i, j] = r
+                e[b, i, j] = EZ if v == 0 else ev + sh
+            ex_b.append(row)
+        exact.append(ex_b)
+    planes = to_planes([int(x) for x in m.reshape(-1)], L).reshape(L, nb, n, n)
+    return planes, e, exact
+
+
+def mpf_host(vals_exps, L):
+    """list of (integer, exponent) -> normalised (planes (L, n), exps (n,))"""
+    NB = 32 * L - 2
+    ms, es = [], []
+    for v, ev in vals_exps:
+        r, sh = norm_closed_form(v, NB)
+        ms.append(r)
+        es.append(EZ if v == 0 else ev + sh)
+    return to_planes(ms, L), np.array(es, np.int64)
