@@ -387,164 +387,168 @@ int launch_gemm(const int8_t* Ah, const int8_t* Al, const int8_t* Bh, const int8
 }
 
 // ================================================================== CRT reconstruction
-// QW entries per workgroup (template), 256 threads. Phase 1: y_g = centred(acc_g * inv_g mod m_g)
-// (residues gathered 8 at a time, prime constants in LDS), split to int8 planes in LDS;
-// k = round(sum y_g / m_g) (|sum - k| < 1/4 because |value| < M/4, so double suffices).
-// Phase 2 (rounds of DB digit blocks of 32): Z = Y . Ct^T on MFMA (entries x digits) into a
-// digit-major int32 buffer with an odd row stride, then one thread per entry carries
-// sum_d (Z_d - k Md_d) 256^d into 32-bit words.
-// Phase 3: words -> W + bitlen, or normalised MPF limbs, or a fixed-exponent rounding.
-template <int QW>
+// One wavefront reconstructs 32 entries on its own; the waves of a workgroup share only the prime
+// and digit constants in LDS, so there is no workgroup barrier after the prologue.
+// Phase 1 (2 lanes per entry, each over half of the primes): y_g = centred(acc_g * inv_g mod m_g),
+//   split to 128*hi + lo and written as packed bytes to the wave's LDS rows; k = round(sum y_g/m_g)
+//   (|sum - k| < 1/4 because |value| < M/4, so double suffices).
+// Phase 2, per block of 32 base-256 digits: Z = Ct . Y on MFMA with digits as rows and entries as
+//   columns, so lane e holds the digits of words 0, 2, 4, 6 of entry e and lane e + 32 those of
+//   words 1, 3, 5, 7; each lane folds its digits into 4 word partials sum (Z_d - k Md_d) 256^u,
+//   the odd partials move across with one exchange and lane e carries the 8 words in order.
+// Phase 3: words -> W + bitlen (mode 0), or normalised MPF limbs (1), or a fixed-exponent rounding (2).
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 __global__ void __launch_bounds__(256) k_rns_crt(
     const int16_t* __restrict__ acc, int n, i64 ld_g, i64 rows, i64 N, i64 j0, i64 w, const i32* __restrict__ mods,
     const double* __restrict__ minv, const i32* __restrict__ inv, const int8_t* __restrict__ Ct,
     const i32* __restrict__ Md, int E, int npad, int mode, i32* __restrict__ W, i64* __restrict__ bitlen,
     i32* __restrict__ outm, i64* __restrict__ oute, int L, i64 plane, i64 ldo, i64 jo, i64 Mb,
     const i64* __restrict__ erow, const i64* __restrict__ ecol, const i64* __restrict__ scol, i32* __restrict__ flag) {
-    constexpr int DB = 4 / (QW / 32);
-    constexpr int NGRP = 256 / QW;
-    constexpr int QWp = QW + 1;                       // odd row stride: conflict-free both ways
-    constexpr int ZD = DB * 32;
     SDPV_DYNAMIC_LDS(unsigned char, smem);
-    const int YS = npad + 16;
+    const int YS = npad + 16;                         // bytes per entry row (multiple of 16)
     const int NW = E / 4;
     const int Epad = ((E + 31) / 32) * 32;
-    int8_t* Yh = (int8_t*)smem;
-    int8_t* Yl = Yh + (size_t)QW * YS;
-    double* ksum = (double*)(Yl + (size_t)QW * YS);  // [NGRP][QW]
-    i32* kq = (i32*)(ksum + NGRP * QW);               // [QW]
-    i32* Mds = kq + QW;                               // [Epad] digits of M
-    i32* cm = Mds + Epad;                             // [npad] moduli
+    const int nthreads = blockDim.x;
+    i32* Mds = (i32*)smem;                            // [Epad] digits of M
+    i32* cm = Mds + Epad;                             // [npad] moduli (1 beyond n)
     i32* cinv = cm + npad;                            // [npad] (M/m)^-1 mod m
     double* cmi = (double*)(cinv + npad);             // [npad] 1/m
-    i32* Zt = (i32*)(cmi + npad);                     // [ZD][QWp]: Z of digit d, entry q
-    u32* Wl = (u32*)(Zt + (size_t)ZD * QWp);          // [QW][NW+1] (modes 1, 2)
-    const i64 Bq = rows * w;
-    const i64 q0 = blockIdx.x * (i64)QW;
-    const int t = threadIdx.x;
+    const size_t wbytes = (size_t)64 * YS + (mode != SDPV_CRT_WORDS ? (size_t)128 * (NW + 1) : 0);
+    const int t = threadIdx.x, wave = t >> 6, lane = t & 63, e = lane & 31, half = lane >> 5;
+    unsigned char* wbase = (unsigned char*)(cmi + npad) + (size_t)wave * wbytes;
+    int8_t* Yh = (int8_t*)wbase;                      // [32][YS]
+    int8_t* Yl = Yh + 32 * YS;                        // [32][YS]
+    u32* Wl = (u32*)(Yl + 32 * YS);                   // [32][NW + 1] (modes 1, 2)
 
-    for (int d = t; d < Epad; d += 256) Mds[d] = Md[d];
-    for (int g = t; g < npad; g += 256) {
+    for (int d = t; d < Epad; d += nthreads) Mds[d] = Md[d];
+    for (int g = t; g < npad; g += nthreads) {
         const bool ok = g < n;
         cm[g] = ok ? mods[g] : 1;
         cinv[g] = ok ? inv[g] : 0;
         cmi[g] = ok ? minv[g] : 0.0;
     }
     __syncthreads();
-    {
-        const int ql = t % QW, grp = t / QW;
-        const i64 q = q0 + ql;
-        const bool qok = q < Bq;
-        i64 off = 0;
-        if (qok) off = (q / w) * N + j0 + q % w;
-        double s = 0.0;
-        for (int gb = grp; gb < npad; gb += 32 * NGRP) {
-            int a[32];
-#pragma unroll
-            for (int u = 0; u < 32; u++) {
-                const int g = gb + u * NGRP;
-                a[u] = (g < n && qok) ? (int)acc[(i64)g * ld_g + off] : 0;
-            }
-#pragma unroll
-            for (int u = 0; u < 32; u++) {
-                const int g = gb + u * NGRP;
-                if (g < npad) {
-                    const int m = cm[g];
-                    int y = mod_m(a[u] * cinv[g], m, cmi[g]);
-                    if (y > (m >> 1)) y -= m;
-                    s += (double)y * cmi[g];
-                    const int lo = ((y + 64) & 127) - 64;
-                    Yh[(size_t)ql * YS + g] = (int8_t)((y - lo) >> 7);
-                    Yl[(size_t)ql * YS + g] = (int8_t)lo;
-                }
-            }
-        }
-        ksum[grp * QW + ql] = s;
-    }
-    __syncthreads();
-    if (t < QW) {
-        double s = 0.0;
-        for (int gr = 0; gr < NGRP; gr++) s += ksum[gr * QW + t];
-        kq[t] = (int)floor(s + 0.5);
-    }
-    __syncthreads();
 
-    const int wave = t >> 6, lane = t & 63, l32 = lane & 31, hsel = lane >> 5;
-    constexpr int nEB = QW / 32;
-    const int nDB = Epad / 32;
-    const int kqt = (t < QW) ? kq[t] : 0;
-    const i64 qt = q0 + t;
-    const bool qtok = (t < QW) && (qt < Bq);
+    const i64 Bq = rows * w;
+    const i64 q0 = ((i64)blockIdx.x * (nthreads >> 6) + wave) * 32;
+    if (q0 >= Bq) return;                             // whole wave: no barrier follows
+    const i64 q = q0 + e;
+    const bool qok = q < Bq;
+    const i64 off = qok ? (q / w) * N + j0 + q % w : 0;
+
+    // ---- phase 1
+    const int hp = npad >> 1;                         // primes per lane half (multiple of 16)
+    double s = 0.0;
+    for (int gc = 0; gc < hp; gc += 16) {
+        const int gb = half * hp + gc;
+        int a[16];
+#pragma unroll
+        for (int u = 0; u < 16; u++) {
+            const int g = gb + u;
+            a[u] = (g < n && qok) ? (int)acc[(i64)g * ld_g + off] : 0;
+        }
+        u32 ph[4] = {0u, 0u, 0u, 0u}, pl[4] = {0u, 0u, 0u, 0u};
+#pragma unroll
+        for (int u = 0; u < 16; u++) {
+            const int g = gb + u;
+            const int m = cm[g];
+            const double mi = cmi[g];
+            int y = mod_m(a[u] * cinv[g], m, mi);
+            if (y > (m >> 1)) y -= m;
+            s += (double)y * mi;
+            const int lo = ((y + 64) & 127) - 64;
+            const int hi = (y - lo) >> 7;
+            ph[u >> 2] |= (u32)(hi & 255) << (8 * (u & 3));
+            pl[u >> 2] |= (u32)(lo & 255) << (8 * (u & 3));
+        }
+        v4i vh, vl;
+#pragma unroll
+        for (int c = 0; c < 4; c++) {
+            vh[c] = (int)ph[c];
+            vl[c] = (int)pl[c];
+        }
+        *(v4i*)(Yh + (size_t)e * YS + gb) = vh;
+        *(v4i*)(Yl + (size_t)e * YS + gb) = vl;
+    }
+    s += __shfl_xor(s, 32);
+    const i64 kq = (i64)floor(s + 0.5);
+    wave_sync();
+
+    // ---- phase 2
+    const int nDB = Epad / 32, nK = npad / 32;
+    const int8_t* yh = Yh + (size_t)e * YS + 16 * half;
+    const int8_t* yl = Yl + (size_t)e * YS + 16 * half;
     i64 carry = 0;
     int last0 = -1, last1 = -1;
     u32 w0 = 0, w1 = 0, lastw = 0;
-    for (int db0 = 0; db0 < nDB; db0 += DB) {
-        {
-            const int eb = wave % nEB, dbl = wave / nEB, db = db0 + dbl;
-            if (dbl < DB && db < nDB) {
-                v16i zh = zero16(), zl = zero16();
-                const int8_t* yh = Yh + (size_t)(eb * 32 + l32) * YS;
-                const int8_t* yl = Yl + (size_t)(eb * 32 + l32) * YS;
-                const int8_t* crow = Ct + (i64)(db * 32 + l32) * npad;
-                for (int kb = 0; kb < npad / 32; kb += 8) {
-                    v4i bfr[8];
+    u32* wrow = Wl + (size_t)e * (NW + 1);
+    for (int db = 0; db < nDB; db++) {
+        v16i zh = zero16(), zl = zero16();
+        const int8_t* crow = Ct + (i64)(db * 32 + e) * npad + 16 * half;
+        for (int kb = 0; kb < nK; kb += 8) {
+            v4i cf[8];
 #pragma unroll
-                    for (int u = 0; u < 8; u++)
-                        bfr[u] = (kb + u < npad / 32) ? *(const v4i*)(crow + (kb + u) * 32 + 16 * hsel) : zero4();
+            for (int u = 0; u < 8; u++) cf[u] = (kb + u < nK) ? *(const v4i*)(crow + (kb + u) * 32) : zero4();
 #pragma unroll
-                    for (int u = 0; u < 8; u++) {
-                        if (kb + u < npad / 32) {
-                            const int off = (kb + u) * 32 + 16 * hsel;
-                            zh = mfma(*(const v4i*)(yh + off), bfr[u], zh);
-                            zl = mfma(*(const v4i*)(yl + off), bfr[u], zl);
-                        }
-                    }
+            for (int u = 0; u < 8; u++) {
+                if (kb + u < nK) {
+                    zh = mfma(cf[u], *(const v4i*)(yh + (kb + u) * 32), zh);
+                    zl = mfma(cf[u], *(const v4i*)(yl + (kb + u) * 32), zl);
                 }
-                i32* zcol = Zt + (size_t)(dbl * 32 + l32) * QWp + eb * 32;
-#pragma unroll
-                for (int j = 0; j < 16; j++) zcol[8 * (j >> 2) + 4 * hsel + (j & 3)] = 128 * zh[j] + zl[j];
             }
         }
-        __syncthreads();
-        if (t < QW) {
-            const int nd = ((nDB - db0) < DB ? (nDB - db0) : DB) * 32;
-#pragma unroll 4
-            for (int d4 = 0; d4 < nd; d4 += 4) {
-                i64 v = carry;
+        i64 P[4];
 #pragma unroll
-                for (int u = 0; u < 4; u++) {
-                    const int d = db0 * 32 + d4 + u;
-                    v += ((i64)Zt[(size_t)(d4 + u) * QWp + t] - (i64)kqt * (i64)Mds[d]) * ((i64)1 << (8 * u));
-                }
-                const int wi = (db0 * 32 + d4) >> 2;
+        for (int wq = 0; wq < 4; wq++) {
+            const int d0 = db * 32 + 8 * wq + 4 * half;
+            i64 v = 0;
+#pragma unroll
+            for (int u = 0; u < 4; u++)
+                v += ((i64)(128 * zh[4 * wq + u] + zl[4 * wq + u]) - kq * (i64)Mds[d0 + u]) * ((i64)1 << (8 * u));
+            P[wq] = v;
+        }
+        i64 Q[4];
+#pragma unroll
+        for (int wq = 0; wq < 4; wq++) Q[wq] = __shfl_xor(P[wq], 32);
+        if (half == 0) {
+#pragma unroll
+            for (int h2 = 0; h2 < 8; h2++) {
+                const i64 v = carry + ((h2 & 1) ? Q[h2 >> 1] : P[h2 >> 1]);
                 const u32 word = (u32)v;
                 carry = v >> 32;
+                const int wi = db * 8 + h2;
                 if (wi < NW) {
                     if (mode == SDPV_CRT_WORDS) {
-                        if (qtok) W[(i64)wi * Bq + qt] = (i32)word;
+                        if (qok) W[(i64)wi * Bq + q] = (i32)word;
                         if (word != 0u) { last0 = wi; w0 = word; }
                         if (word != 0xffffffffu) { last1 = wi; w1 = ~word; }
                         lastw = word;
                     } else {
-                        Wl[(size_t)t * (NW + 1) + wi] = word;
+                        wrow[wi] = word;
                     }
                 }
             }
         }
-        __syncthreads();
     }
-    if (!qtok) return;
-    const i64 i = qt / w, jj = qt % w;
+    if (half != 0 || !qok) return;
+
+    // ---- phase 3 (lane e, entry q; reads only its own LDS row)
+    const i64 i = q / w, jj = q % w;
     if (mode == SDPV_CRT_WORDS) {
         const bool neg = (lastw >> 31) != 0;
         const int last = neg ? last1 : last0;
         const u32 top = neg ? w1 : w0;
-        bitlen[qt] = last >= 0 ? 32 * (i64)last + 32 - clz32(top) : 0;
+        bitlen[q] = last >= 0 ? 32 * (i64)last + 32 - clz32(top) : 0;
         return;
     }
     const i64 o = i * ldo + jo + jj;
     const i64 cidx = (i / Mb) * ldo + jo + jj;
-    LdsWords src{Wl + (size_t)t * (NW + 1)};
+    LdsWords src{wrow};
     if (mode == SDPV_CRT_NORMALIZE) {
         MagView<LdsWords> mag(src, NW);
         const int bl = bitlen_mag(mag, NW);
@@ -553,11 +557,11 @@ __global__ void __launch_bounds__(256) k_rns_crt(
             oute[o] = EZV;
             return;
         }
-        const int s = bl - (32 * L - 2);
+        const int sh = bl - (32 * L - 2);
         NegStream ng(mag.neg);
-        shift_mag(mag, NW, s, L, [&](int k, u32 v) { outm[(i64)k * plane + o] = (i32)ng(v); });
+        shift_mag(mag, NW, sh, L, [&](int k, u32 v) { outm[(i64)k * plane + o] = (i32)ng(v); });
         const i64 ein = (erow ? erow[i] : 0) + (ecol ? ecol[cidx] : 0);
-        oute[o] = ein + s;
+        oute[o] = ein + sh;
     } else {
         const int bad = shr_round(src, NW, scol[cidx], L, [&](int k, u32 v) { outm[(i64)k * plane + o] = (i32)v; });
         if (bad && flag) atomicOr(flag, 1);
@@ -655,36 +659,26 @@ int sdpv_rns_crt(const int16_t* acc, int64_t n, int64_t ld_g, int64_t rows, int6
     if (mode == SDPV_CRT_NORMALIZE && !oute) return SDPV_EARG;
     if (mode == SDPV_CRT_ROUND && !scol) return SDPV_EARG;
     if (npad > 1024) return SDPV_ESIZE;  // digit sums 128*Zh + Zl stay within int32
-    const int QW = (npad <= 352) ? 64 : 32;
-    const int DB = 4 / (QW / 32);
-    const int NW = (int)(E / 4);
+    const i64 NW = E / 4;
     const i64 Epad = 32 * ((E + 31) / 32);
-    size_t lds = (size_t)2 * QW * (npad + 16) + (size_t)(256 / QW) * QW * 8 + (size_t)QW * 4 + (size_t)Epad * 4 +
-                 (size_t)npad * 16 + (size_t)DB * 32 * (QW + 1) * 4;
-    if (mode != SDPV_CRT_WORDS) lds += (size_t)QW * (NW + 1) * 4;
+    const size_t cbytes = (size_t)Epad * 4 + (size_t)npad * 16;
+    const size_t wbytes = (size_t)64 * (npad + 16) + (mode != SDPV_CRT_WORDS ? (size_t)128 * (NW + 1) : 0);
+    int nwave = 4;
+    while (nwave > 1 && cbytes + nwave * wbytes > 160 * 1024) nwave >>= 1;
+    const size_t lds = cbytes + nwave * wbytes;
     if (lds > 160 * 1024) return SDPV_ESIZE;
-    const i64 grid = (Bq + QW - 1) / QW;
+    const i64 per = 32 * (i64)nwave;
+    const i64 grid = (Bq + per - 1) / per;
     if (grid > 0x7fffffffLL) return SDPV_ESIZE;
     hipStream_t st = as_stream(stream);
-    if (QW == 64) {
-        static bool attr64 = false;
-        if (!attr64) {
-            (void)hipFuncSetAttribute((const void*)k_rns_crt<64>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-            attr64 = true;
-        }
-        hipLaunchKernelGGL(k_rns_crt<64>, dim3((unsigned)grid), dim3(256), lds, st, acc, (int)n, (i64)ld_g, (i64)rows,
-                           (i64)N, (i64)j0, (i64)w, mods, minv, inv, Ct, Md, (int)E, (int)npad, (int)mode, W, bitlen,
-                           outm, oute, (int)L, (i64)plane, (i64)ldo, (i64)jo, (i64)Mb, erow, ecol, scol, flag);
-    } else {
-        static bool attr32 = false;
-        if (!attr32) {
-            (void)hipFuncSetAttribute((const void*)k_rns_crt<32>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-            attr32 = true;
-        }
-        hipLaunchKernelGGL(k_rns_crt<32>, dim3((unsigned)grid), dim3(256), lds, st, acc, (int)n, (i64)ld_g, (i64)rows,
-                           (i64)N, (i64)j0, (i64)w, mods, minv, inv, Ct, Md, (int)E, (int)npad, (int)mode, W, bitlen,
-                           outm, oute, (int)L, (i64)plane, (i64)ldo, (i64)jo, (i64)Mb, erow, ecol, scol, flag);
+    static bool attr = false;
+    if (!attr) {
+        (void)hipFuncSetAttribute((const void*)k_rns_crt, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+        attr = true;
     }
+    hipLaunchKernelGGL(k_rns_crt, dim3((unsigned)grid), dim3(64 * nwave), lds, st, acc, (int)n, (i64)ld_g, (i64)rows,
+                       (i64)N, (i64)j0, (i64)w, mods, minv, inv, Ct, Md, (int)E, (int)npad, (int)mode, W, bitlen, outm,
+                       oute, (int)L, (i64)plane, (i64)ldo, (i64)jo, (i64)Mb, erow, ecol, scol, flag);
     SDPV_TRY_LAUNCH();
     return SDPV_OK;
 }

@@ -23,6 +23,8 @@ void emu_syncthreads() { g_block->arrive_and_wait(); }
 
 static Wave& my_wave() { return g_waves[threadIdx.x / 64]; }
 
+void emu_wave_barrier() { my_wave().bar->arrive_and_wait(); }
+
 long long emu_shfl_xor64(long long v, int mask) {
     Wave& w = my_wave();
     int lane = threadIdx.x & 63;

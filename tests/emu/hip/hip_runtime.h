@@ -52,6 +52,18 @@ inline long long atomicMax(long long* p, long long v) {
 long long emu_shfl_xor64(long long v, int mask);
 inline long long __shfl_xor(long long v, int mask, int width = 64) { (void)width; return emu_shfl_xor64(v, mask); }
 inline long __shfl_xor(long v, int mask, int width = 64) { (void)width; return (long)emu_shfl_xor64(v, mask); }
+inline double __shfl_xor(double v, int mask, int width = 64) {
+    (void)width;
+    long long b;
+    std::memcpy(&b, &v, 8);
+    b = emu_shfl_xor64(b, mask);
+    double r;
+    std::memcpy(&r, &b, 8);
+    return r;
+}
+void emu_wave_barrier();
+#define __builtin_amdgcn_wave_barrier() emu_wave_barrier()
+#define __builtin_amdgcn_fence(order, scope) __atomic_thread_fence(__ATOMIC_SEQ_CST)
 
 typedef int emu_v4i __attribute__((ext_vector_type(4)));
 typedef int emu_v16i __attribute__((ext_vector_type(16)));

@@ -134,6 +134,32 @@ def chol_cases():
 CASES["chol"] = chol_cases
 
 
+def crt_cases():
+    """RNS.crt_block at L=1, 24 and 130 (n > 512 primes: fewer waves per workgroup)"""
+    from sdpvibe_amd import mpmat as MM
+    from tests.intutil import from_planes as fp
+    ok = True
+    for L in (1, 24, 130):
+        R = MM.ctx(L)
+        rng = random.Random(41 + L)
+        rows, N, start, width = 5, 13, 2, 11
+        limit = R.M // 4
+        vals = [[rng.randrange(-limit + 1, limit) for _ in range(N)] for _ in range(rows)]
+        vals[0][2:7] = [0, 1, -1, limit - 1, -limit + 1]
+        acc = np.array([[[v % p for v in row] for row in vals] for p in R.mods], np.int16)
+        W, bl, Bq = R.crt_block(torch.from_numpy(acc), N, start, width, rows)
+        exp = [v for row in vals for v in row[start:start + width]]
+        sel = acc[:, :, start:start + width].reshape(R.n, -1)
+        rW, rbl = O.crt(sel, np.array(R.mods), R.E // 4)
+        good = np.array_equal(W.numpy(), rW) and np.array_equal(bl.numpy(), rbl) and fp(W.numpy()) == exp
+        print(f"crt_block L={L} n={R.n}: {'ok' if good else 'MISMATCH'}", flush=True)
+        ok &= good
+    return ok
+
+
+CASES["crt"] = crt_cases
+
+
 if __name__ == "__main__":  # noqa: E305
     ok = True
     for c in sys.argv[1:] or list(CASES):
