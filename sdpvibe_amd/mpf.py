@@ -17,8 +17,8 @@ import torch
 
 from ._lib import call, ptr, stream
 from .limbops import bitlen_words, shift_round
-from .mpmat import ACC_BUDGET, ctx, residue_gemm
-from .rns import dev
+from .mpmat import ACC_BUDGET, RES_BUDGET, ctx, residue_gemm
+from .rns import KMAX_LAUNCH, dev
 
 EZ = -(1 << 40)                                      # exponent of zero
 LMAX_WIDE = 1 << 20                                  # normalize() takes any input width
@@ -473,3 +473,91 @@ class Chol:
     def solve(self, B):
         """A^-1 B"""
         return self.solve_upper(self.solve_lower(B))
+
+
+# ------------------------------------------------------------------ Gram accumulation (Q = sum_c P_c^T P_c)
+class GramAcc:
+    """Q = sum over chunks of P_c^T P_c, accumulated exactly as int16 residues of ctx(L + 1) and
+    reconstructed once (mpf.py:536-630).
+
+    Each column j is rounded to a running reference exponent E_j (the largest column top seen),
+    leaving HEAD bits of headroom in the extra limb; a chunk whose column top exceeds its
+    reference by more than HEAD forces a flush: the partial sum is reconstructed and added in MPF.
+    Only tiles on or below the diagonal are accumulated (the product's syrk mode); callers mirror
+    the lower triangle."""
+    HEAD = 28
+
+    def __init__(self, N, L, E_ref=None):
+        self.N, self.L = N, L
+        self.R = ctx(L + 1)
+        self.E = E_ref.clone() if E_ref is not None else torch.full((N,), EZ, dtype=torch.int64, device=dev)
+        self.Tmax = torch.full((N,), EZ, dtype=torch.int64, device=dev)
+        self.acc = None
+        self.part = None
+        self.flushes = 0
+
+    def add(self, P):
+        """P: MPF (rows, N). The chunk is rounded to the reference exponents and split into residues
+        by one fused kernel reading P's columns in place; its residue Gram matrix is added modulo
+        every prime (lower tiles)."""
+        L, N, R = self.L, self.N, self.R
+        rows = P.shape[0]
+        tp = _top(P).amax(0) if rows else torch.full((N,), EZ, dtype=torch.int64, device=dev)
+        live = tp > EZ // 2
+        self.Tmax = torch.maximum(self.Tmax, tp)
+        known = self.E > EZ // 2
+        if bool((live & known & (tp > self.E + self.HEAD)).any()):
+            self._flush()
+            self.E = torch.where(live & (tp > self.E), tp, self.E)
+            known = self.E > EZ // 2
+        self.E = torch.where(live & ~known, tp, self.E)
+        if self.acc is None:
+            self.acc = torch.zeros((R.n, N, N), dtype=torch.int16, device=dev)
+        if rows == 0:
+            return
+        Pm, Pe = P.m.contiguous(), P.e.contiguous()
+        Erow = (self.E - NBITS(L)).contiguous()
+        per_k = 2 * R.n * N
+        kc = max(64, min(KMAX_LAUNCH, RES_BUDGET // max(per_k, 1) // 64 * 64))
+        for k0 in range(0, rows, kc):
+            kk = min(kc, rows - k0)
+            kp = R.kp(kk)
+            Rh = torch.empty((R.n, N, kp), dtype=torch.int8, device=dev)
+            Rl = torch.empty_like(Rh)
+            # rows of the split = columns j of P (stride 1), inner index = rows of P (stride N)
+            R.split(Pm, Pe, Erow, L, rows * N, 1, N, 0, 1, N, k0, kk, kp, Rh, Rl)
+            R.product(Rh, Rl, Rh, Rl, self.acc, 1, N, N, N * N, accumulate=True, syrk=True)
+            del Rh, Rl
+
+    def _crt(self):
+        L, N, R = self.L, self.N, self.R
+        out_m = torch.empty((L, N, N), dtype=torch.int32, device=dev)
+        out_e = torch.empty((N, N), dtype=torch.int64, device=dev)
+        rE = torch.where(self.E > EZ // 2, self.E - NBITS(L), torch.zeros_like(self.E)).contiguous()
+        R.crt(self.acc, N, N, 0, N, 1, outm=out_m, oute=out_e, L=L, plane=N * N, ldo=N, jo=0, Mb=N, erow=rE,
+              ecol=rE)
+        return MPF(out_m, out_e)
+
+    def _flush(self):
+        if self.acc is None:
+            return
+        q = self._crt()
+        self.part = q if self.part is None else add(self.part, q)
+        self.acc = None
+        self.flushes += 1
+
+    def next_reference(self, margin=4):
+        """reference exponents for the next accumulation (columns never seen: none)"""
+        return torch.where(self.Tmax > EZ // 2, self.Tmax + margin, self.Tmax)
+
+    def precision_loss(self):
+        """max over columns of (reference - actual column maximum), in bits"""
+        ok = (self.Tmax > EZ // 2) & (self.E > EZ // 2)
+        return int((self.E - self.Tmax)[ok].max()) if bool(ok.any()) else 0
+
+    def result(self):
+        if self.acc is None:
+            return self.part
+        q = self._crt()
+        self.acc = None
+        return q if self.part is None else add(self.part, q)

@@ -160,6 +160,19 @@ def crt_cases():
 CASES["crt"] = crt_cases
 
 
+def gram_cases():
+    import tests.test_gpu_gram as T
+    T.cu = lambda a: torch.from_numpy(np.ascontiguousarray(a))
+    T.gram_parity(3, 40, (5, 70), [0, 0], seed=1)
+    assert T.gram_parity(2, 33, (20, 9, 11), [0, 60, -5], seed=2) == 1
+    T.gram_parity(2, 20, (10, 10), [0, 3], seed=3, e_ref=True)
+    print("gram: ok", flush=True)
+    return True
+
+
+CASES["gram"] = gram_cases
+
+
 if __name__ == "__main__":  # noqa: E305
     ok = True
     for c in sys.argv[1:] or list(CASES):

@@ -141,3 +141,20 @@ def mpf_host(vals_exps, L):
         ms.append(r)
         es.append(EZ if v == 0 else ev + sh)
     return to_planes(ms, L), np.array(es, np.int64)
+
+
+def rand_mpf_block(rng, shape, L, spread=8, shift=0, zero_frac=0.0):
+    """normalised random MPF host arrays (planes (L, *shape), exps shape): |mantissa| has NB bits,
+    exponent -NB + u + shift with u uniform in [-spread, spread]; a fraction of exact zeros"""
+    NB = 32 * L - 2
+    n = int(np.prod(shape))
+    ms, es = [], []
+    for _ in range(n):
+        if zero_frac and rng.random() < zero_frac:
+            ms.append(0)
+            es.append(EZ)
+            continue
+        v = rng.randrange(1 << (NB - 1), 1 << NB)
+        ms.append(-v if rng.random() < 0.5 else v)
+        es.append(-NB + rng.randint(-spread, spread) + shift)
+    return to_planes(ms, L).reshape((L,) + tuple(shape)), np.array(es, np.int64).reshape(shape)
