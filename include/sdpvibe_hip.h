@@ -166,12 +166,22 @@ int sdpv_rns_crt(const int16_t* acc, int64_t n, int64_t ld_g, int64_t rows, int6
  * row by row, as the reference's kernels do.
  * Lout: (L, nb, BT, BT) the factor rounded (half up) back to A's exponent, upper triangle zero.
  * Xw:   (L + 24, nb, BT, BT) the inverse L^-1 in wide fixed point (exponent -32L).
- * work: device scratch of 3 * nb * BT * BT * (L + 24) uint32 words.
+ * work: device scratch of nb * BT * (3 * BT + 1) * (L + 24) uint32 words. On return its first
+ *       nb * BT * BT * (L + 24) words hold the wide factor (nb, BT, BT, L + 24), upper triangle zero,
+ *       and the words from 3 * nb * BT * BT * (L + 24) on the pivot reciprocals (nb, BT, L + 24): the
+ *       reference's Tile.Lw and Tile.RINV.
  * flag: optional int32 set to 1 when the narrowing to L limbs overflows.
  * Replaces chol_tiles + trinv_tiles (sdpvibe/basecase.py:159-226) and the widening/narrowing of
  * batched.bchol_tile (sdpvibe/batched.py:243-257). */
 int sdpv_tile_chol(const int32_t* A, int64_t L, int64_t nb, int64_t BT, int32_t* Lout, int32_t* Xw,
                    uint32_t* work, int32_t* flag, void* stream);
+
+/* Forward substitution L y = v with one factored tile, in place, for nvec vectors.
+ * Lw: (BT, BT, WL) wide factor, RINV: (BT, WL) pivot reciprocals (both from sdpv_tile_chol's work),
+ * V: (nvec, BT, WL) wide fixed-point vectors (F = 32 LF fractional bits). BT <= 32.
+ * Replaces trsv_warp (sdpvibe/basecase.py:228-245). */
+int sdpv_tile_trsv(const uint32_t* Lw, const uint32_t* RINV, uint32_t* V, int64_t nvec, int64_t BT, int64_t WL,
+                   int64_t LF, void* stream);
 
 /* ---------------------------------------------------------------- diagnostics */
 

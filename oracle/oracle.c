@@ -660,3 +660,24 @@ void orc_tile_chol(uint32_t* W, uint32_t* RINV, uint32_t* X, i64 nb, int BT, int
         }
     }
 }
+
+/* trsv_warp (sdpvibe/basecase.py:228-245): L y = v in place for nvec vectors V (nvec, BT, WL) with the
+ * factored tile Lw (BT, BT, WL) and pivot reciprocals RINV (BT, WL); y_j = mulfx(v_j, rinv_j), then
+ * v_i -= mulfx(l_ij, y_j) for i > j. */
+void orc_tile_trsv(const uint32_t* Lw, const uint32_t* RINV, uint32_t* V, i64 nvec, int BT, int WL, int LF) {
+    wfx_t f = {WL, LF};
+    #pragma omp parallel for schedule(dynamic, 4)
+    for (i64 q = 0; q < nvec; q++) {
+        u32* vb = V + (size_t)q * BT * WL;
+        u32 t[256];
+        for (int j = 0; j < BT; j++) {
+            u32* vj = vb + (size_t)j * WL;
+            w_mulfx(t, vj, RINV + (size_t)j * WL, 0, f);
+            memcpy(vj, t, 4 * (size_t)WL);
+            for (int i = j + 1; i < BT; i++) {
+                w_mulfx(t, Lw + ((size_t)i * BT + j) * WL, vj, 0, f);
+                w_subfrom(vb + (size_t)i * WL, t, WL);
+            }
+        }
+    }
+}

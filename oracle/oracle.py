@@ -30,6 +30,7 @@ _SIGS = {
     "orc_gemm_exact": [_P, _P, _I64, _I64, _I64, _I64, _P, _I64],
     "orc_bgemm_nt": [_P, _P, _I64, _I64, _I64, _I64, _I64, _P, _I32, _P, _P],
     "orc_tile_chol": [_P, _P, _P, _I64, _I32, _I32, _I32],
+    "orc_tile_trsv": [_P, _P, _P, _I64, _I32, _I32, _I32],
 }
 _lib = None
 
@@ -205,3 +206,13 @@ def tile_chol(W, LF):
     X = np.zeros_like(W)
     lib().orc_tile_chol(_p(W), _p(RINV), _p(X), nb, BT, WL, LF)
     return W, RINV, X
+
+
+def tile_trsv(Lw, RINV, V, LF):
+    """trsv_warp (basecase.py:228-245): Lw (BT, BT, WL), RINV (BT, WL), V (nvec, BT, WL) uint32 words;
+    returns y = L^-1 v per vector (same layout)"""
+    Lw, RINV = np.ascontiguousarray(Lw, dtype=np.uint32), np.ascontiguousarray(RINV, dtype=np.uint32)
+    V = np.ascontiguousarray(V, dtype=np.uint32).copy()
+    nvec, BT, WL = V.shape
+    lib().orc_tile_trsv(_p(Lw), _p(RINV), _p(V), nvec, BT, WL, LF)
+    return V

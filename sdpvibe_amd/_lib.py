@@ -32,6 +32,7 @@ SIGNATURES = {
     "sdpv_rns_crt": ([_P, _I64, _I64, _I64, _I64, _I64, _I64, _P, _P, _P, _P, _P, _I64, _I64, _I32, _P, _P,
                       _P, _P, _I64, _I64, _I64, _I64, _I64, _P, _P, _P, _P, _P], _I32),
     "sdpv_tile_chol": ([_P, _I64, _I64, _I64, _P, _P, _P, _P, _P], _I32),
+    "sdpv_tile_trsv": ([_P, _P, _P, _I64, _I64, _I64, _I64, _P], _I32),
     "sdpv_mfma_selftest": ([_P, _P, _P, _P], _I32),
 }
 ERRORS = {-1: "bad argument", -2: "unsupported size", -3: "kernel launch failed"}

@@ -1,26 +1,105 @@
-"""Base case on the GPU: batched multiprecision Cholesky + triangular inverse of small tiles
-(mirror of sdpvibe/basecase.py).
+"""Base case on the GPU: multiprecision Cholesky + triangular inverse of small tiles and forward
+substitution with a factored tile (mirror of sdpvibe/basecase.py).
 
-Wide fixed-point format inside the kernel: WL = L + 24 32-bit words, two's complement, F = 32L
+Wide fixed-point format inside the kernels: WL = L + 24 32-bit words, two's complement, F = 32L
 fractional bits. Input tiles are SPD and equilibrated, so |a_ij| <= 1 and |l_ij| <= 1; pivots may
 be tiny and their reciprocals (and X = L^-1) large, hence the integer words.
 """
 import torch
 
 from ._lib import call, ptr, stream
-from .limbops import _errbuf
+from .limbops import _errbuf, bitlen_words, shift_round
+from .mpmat import MP, headroom_bits, rescale
 
 LI = 24   # integer words of the wide format
 
 
 def tile_factor(limbs, BT):
     """limbs: (L, nb, BT, BT) fixed point at exponent -(32L-40) -> (Lout (L, nb, BT, BT) at the same
-    exponent, upper triangle zero; Xw (L+24, nb, BT, BT) = L^-1 at exponent -32L)"""
+    exponent, upper triangle zero; Xw (L+24, nb, BT, BT) = L^-1 at exponent -32L; work, whose
+    first nb*BT*BT*(L+24) words are the wide factor and whose words from 3*nb*BT*BT*(L+24) on are
+    the wide pivot reciprocals)"""
     L, nb = limbs.shape[0], limbs.shape[1]
     WL = L + LI
     A = limbs.contiguous()
     Lout = torch.empty_like(A)
     Xw = torch.empty((WL, nb, BT, BT), dtype=torch.int32, device=A.device)
-    work = torch.empty((3 * nb * BT * BT * WL,), dtype=torch.int32, device=A.device)
+    work = torch.empty((nb * BT * (3 * BT + 1) * WL,), dtype=torch.int32, device=A.device)
     call("sdpv_tile_chol", ptr(A), L, nb, BT, ptr(Lout), ptr(Xw), ptr(work), ptr(_errbuf()), stream())
-    return Lout, Xw
+    return Lout, Xw, work
+
+
+class Tile:
+    """Cholesky factor of a diagonal tile in the kernels' wide, entry-major format (+ pivot
+    reciprocals), with the rounded factor Lmp and, when asked for, the explicit inverse X
+    (basecase.py:259-261)"""
+    __slots__ = ("Lw", "RINV", "b", "WL", "LF", "X", "Lmp")
+
+
+def chol_tile(A, LI=LI, threads=256, want_inverse=False):
+    """A: MP (b x b) fixed point at exponent -(32L-40) -> (Lfac MP with A's exponent, Tile[, X])
+    (basecase.py:268-290). The factor and the inverse come from one launch."""
+    b = A.shape[0]
+    L = A.L
+    e = A.uniform_exp()
+    if e != -(32 * L - 40):
+        raise ValueError("tiles enter at the fixed-point exponent -(32L - 40)")
+    WL, F = L + LI, 32 * L
+    Lout, Xw, work = tile_factor(A.limbs.reshape(L, 1, b, b), b)
+    t = Tile()
+    t.Lw = work[:b * b * WL].view(b, b, WL)
+    t.RINV = work[3 * b * b * WL:3 * b * b * WL + b * WL].view(b, WL)
+    t.b, t.WL, t.LF = b, WL, L
+    t.Lmp = MP(Lout.view(L, b, b), e)
+    t.X = None
+    if not want_inverse:
+        return t.Lmp, t
+    Xs = Xw.view(WL, b, b)
+    top = int(bitlen_words(Xs.reshape(WL, -1)).max()) - F
+    t.X = rescale(MP(Xs, -F), top - headroom_bits(L), L)
+    return t.Lmp, t, t.X
+
+
+def chol_inv_tile(A, LI=LI, threads=256):
+    """(basecase.py:293-295)"""
+    Lout, _, X = chol_tile(A, LI, threads, want_inverse=True)
+    return Lout, X
+
+
+def apply_tile_inverse(t, B, refine=0):
+    """P = L_tile^-1 B through the explicit tile inverse (+ refinement steps); B: MP (b x m) with
+    per-column exponents (basecase.py:298-310)"""
+    from .mpmat import add, gemm_nt
+
+    def tr(M):
+        return MP(M.limbs.transpose(1, 2).contiguous(), 0)
+    P = gemm_nt(t.X, tr(B), out_exp="col", bt_rowexp=B.exp)
+    for _ in range(refine):
+        LP = gemm_nt(t.Lmp, tr(P), out_exp="col", bt_rowexp=P.exp)
+        Rr = add(B, LP, -1, out_exp="col")
+        D = gemm_nt(t.X, tr(Rr), out_exp="col", bt_rowexp=Rr.exp)
+        P = add(P, D, 1, out_exp="align")
+    return P
+
+
+def trsv_tile(t, Vlimbs, vexp, out_exp=None):
+    """Solve L y = v for many vectors with a factored tile (basecase.py:313-332).
+    Vlimbs: (L, nvec, b) limb planes, vexp: (nvec,) exponents (each vector is fixed point at
+    32L-40 fractional bits of its own scale). out_exp None: dynamic per vector; int: fixed.
+    Returns (limbs (L, nvec, b), exps (nvec,))."""
+    L, nvec, b = Vlimbs.shape
+    F, FX = 32 * t.LF, 32 * L - 40
+    WL = t.WL
+    n = nvec * b
+    # exact widening by F - FX = 40 bits, then entry-major (nvec, b, WL) words
+    Ws = shift_round(Vlimbs.reshape(L, n), torch.full((n,), FX - F, dtype=torch.int64, device=Vlimbs.device), WL)
+    W = Ws.view(WL, nvec, b).permute(1, 2, 0).contiguous()
+    call("sdpv_tile_trsv", ptr(t.Lw), ptr(t.RINV), ptr(W), nvec, b, WL, t.LF, stream())
+    Ws = W.permute(2, 0, 1).contiguous().view(WL, n)
+    base = vexp + FX - F
+    if out_exp is None:
+        ne = bitlen_words(Ws).view(nvec, b).amax(1) + base - headroom_bits(L)
+    else:
+        ne = torch.full((nvec,), int(out_exp), dtype=torch.int64, device=Vlimbs.device)
+    s = (ne - base)[:, None].expand(nvec, b).reshape(-1).contiguous()
+    return shift_round(Ws, s, L).view(L, nvec, b), ne

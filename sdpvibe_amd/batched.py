@@ -177,7 +177,7 @@ def bchol_tile(A, LI=24, threads=256):
     e = A.exp
     WL, F = L + LI, 32 * L
     assert isinstance(e, int) and e == -(32 * L - 40), "tiles enter at the fixed-point exponent"
-    Lout_l, Xs = tile_factor(A.limbs, b)
+    Lout_l, Xs, _ = tile_factor(A.limbs, b)
     Lout = BMP(Lout_l, e)
     top = bitlen_words(Xs.reshape(WL, -1)).view(nb, b * b).amax(1) - F
     ex = (top - headroom_bits(L))[:, None].expand(nb, b).contiguous()

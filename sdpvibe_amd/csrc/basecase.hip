@@ -239,7 +239,7 @@ __device__ void wave_rsqrt(const Pivot& P) {
 // W, X, S: AoS (nb, BT, BT, WL) words in global memory (W: working copy, factored in place;
 // X: L^-1; S: products of the inverse). LDS: 2 operand slots per thread + pivot area + RINV.
 __global__ void __launch_bounds__(TT) k_tile_factor(u32* __restrict__ Wg, u32* __restrict__ Xg, u32* __restrict__ Sg,
-                                                   int BT, int WL, int LF) {
+                                                   u32* __restrict__ Rg, int BT, int WL, int LF) {
     SDPV_DYNAMIC_LDS(u32, lds);
     const int tid = threadIdx.x;
     const i64 tile = blockIdx.x;
@@ -270,6 +270,7 @@ __global__ void __launch_bounds__(TT) k_tile_factor(u32* __restrict__ Wg, u32* _
         for (int q = tid; q < WL; q += TT) {
             ent(W, k, k)[q] = P.t1[q];
             rinv[k * WL + q] = P.r[q];
+            Rg[(tile * BT + k) * WL + q] = P.r[q];
         }
         __syncthreads();
         // column scale: A_ik = mulfx(A_ik, r)
@@ -353,6 +354,42 @@ __global__ void __launch_bounds__(TT) k_tile_factor(u32* __restrict__ Wg, u32* _
     }
 }
 
+// Forward substitution L y = v for many vectors with one factored tile (the reference's trsv_warp,
+// sdpvibe/basecase.py:228-245): 32 lanes per vector, lane i owns v_i (BT <= 32), kept in place in
+// V (nvec, BT, WL). Step j: lane j scales y_j = mulfx(v_j, rinv_j); after the barrier every lane
+// i > j subtracts mulfx(l_ij, y_j). All threads run every step (barriers), idle lanes included.
+__global__ void __launch_bounds__(TT) k_tile_trsv(const u32* __restrict__ Lw, const u32* __restrict__ R,
+                                                 u32* __restrict__ V, i64 nvec, int BT, int WL, int LF) {
+    SDPV_DYNAMIC_LDS(u32, lds);
+    const int tid = threadIdx.x, lane = tid & 31;
+    const i64 vec = (i64)blockIdx.x * (TT / 32) + (tid >> 5);
+    const bool live = vec < nvec && lane < BT;
+    Slot sx{lds + tid}, sy{lds + WL * TT + tid};
+    u32* vb = V + vec * BT * WL;
+    for (int j = 0; j < BT; j++) {
+        if (live && lane == j) {
+            u32* vj = vb + (i64)j * WL;
+            const u32* rj = R + (i64)j * WL;
+            const bool na = load_mag(sx, GlobalEntry{vj}, WL);
+            const bool nr = load_mag(sy, GlobalEntry{rj}, WL);
+            mulfx_stream(sx, sy, na ^ nr, 0, WL, LF, [&](int q, u32 w) { vj[q] = w; });
+        }
+        __syncthreads();
+        if (live && lane > j) {
+            u32* vi = vb + (i64)lane * WL;
+            const bool na = load_mag(sx, GlobalEntry{Lw + ((i64)lane * BT + j) * WL}, WL);
+            const bool ny = load_mag(sy, GlobalEntry{vb + (i64)j * WL}, WL);
+            i64 br = 0;
+            mulfx_stream(sx, sy, na ^ ny, 0, WL, LF, [&](int q, u32 w) {
+                const i64 d = (i64)vi[q] - (i64)w + br;
+                vi[q] = (u32)d;
+                br = d >> 32;
+            });
+        }
+        __syncthreads();
+    }
+}
+
 // planes (L limbs, value at exponent efx) -> AoS wide words (value * 2^40 in WL words)
 __global__ void __launch_bounds__(256) k_tile_widen(const i32* __restrict__ A, i64 plane, int L, int WL,
                                                     u32* __restrict__ W, i64 n) {
@@ -400,6 +437,7 @@ int sdpv_tile_chol(const int32_t* A, int64_t L, int64_t nb, int64_t BT, int32_t*
     u32* W = work;
     u32* X = work + n * WL;
     u32* S = work + 2 * n * WL;
+    u32* RI = work + 3 * n * WL;
     hipLaunchKernelGGL(k_tile_widen, dim3(nblk(n, 256)), dim3(256), 0, st, A, n, (int)L, WL, W, n);
     SDPV_TRY_LAUNCH();
     static bool attr = false;
@@ -407,9 +445,29 @@ int sdpv_tile_chol(const int32_t* A, int64_t L, int64_t nb, int64_t BT, int32_t*
         (void)hipFuncSetAttribute((const void*)k_tile_factor, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
         attr = true;
     }
-    hipLaunchKernelGGL(k_tile_factor, dim3((unsigned)nb), dim3(TT), lds, st, W, X, S, (int)BT, WL, (int)L);
+    hipLaunchKernelGGL(k_tile_factor, dim3((unsigned)nb), dim3(TT), lds, st, W, X, S, RI, (int)BT, WL, (int)L);
     SDPV_TRY_LAUNCH();
     hipLaunchKernelGGL(k_tile_narrow, dim3(nblk(n, 256)), dim3(256), 0, st, W, X, (int)L, WL, Lout, Xw, n, n, flag);
+    SDPV_TRY_LAUNCH();
+    return SDPV_OK;
+}
+
+int sdpv_tile_trsv(const uint32_t* Lw, const uint32_t* RINV, uint32_t* V, int64_t nvec, int64_t BT, int64_t WL,
+                   int64_t LF, void* stream) {
+    if (nvec < 0 || BT < 1 || BT > 32 || LF < 1 || WL < LF + 2) return SDPV_EARG;
+    if (nvec == 0) return SDPV_OK;
+    if (!Lw || !RINV || !V) return SDPV_EARG;
+    const size_t lds = (size_t)2 * WL * TT * 4;
+    if (lds > 160 * 1024) return SDPV_ESIZE;
+    const i64 grid = (nvec + TT / 32 - 1) / (TT / 32);
+    if (grid > 0x7fffffffLL) return SDPV_ESIZE;
+    static bool attr = false;
+    if (!attr) {
+        (void)hipFuncSetAttribute((const void*)k_tile_trsv, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+        attr = true;
+    }
+    hipLaunchKernelGGL(k_tile_trsv, dim3((unsigned)grid), dim3(TT), lds, as_stream(stream), Lw, RINV, V, nvec,
+                       (int)BT, (int)WL, (int)LF);
     SDPV_TRY_LAUNCH();
     return SDPV_OK;
 }

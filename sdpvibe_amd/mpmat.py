@@ -109,10 +109,10 @@ def to_ints(A):
 
 
 # ------------------------------------------------------------------ re-rounding
-def _exp_vec(e, C, device=dev):
+def _exp_vec(e, C, device=None):
     if torch.is_tensor(e):
         return e if e.dim() == 1 else e.expand(C)
-    return torch.full((C,), int(e), dtype=torch.int64, device=device)
+    return torch.full((C,), int(e), dtype=torch.int64, device=device or dev)
 
 
 def rescale(A, new_exp, Lout=None):

@@ -104,7 +104,7 @@ def tile_case(L, nb, BT, logc, seed):
             for j in range(BT):
                 W[b, i, j] = int_to_words(tiles[b][i][j] << 40, WL)
     Lw, _, X = O.tile_chol(W, L)
-    Lo, Xw = tile_factor(torch.from_numpy(A), BT)
+    Lo, Xw, _ = tile_factor(torch.from_numpy(A), BT)
     exp_L = to_planes([wrap(rnd_half_up(words_to_int(Lw[b, i, j]), 40), L) for b in range(nb) for i in range(BT)
                        for j in range(BT)], L).reshape(L, nb, BT, BT)
     exp_X = np.ascontiguousarray(X.transpose(3, 0, 1, 2)).view(np.int32)
@@ -126,7 +126,6 @@ def chol_cases():
     import tests.test_gpu_chol as T
     T.cu = lambda a: torch.from_numpy(np.ascontiguousarray(a))
     chol_parity(2, 2, 40, 3, 5, seed=1)
-    chol_parity(3, 1, 20, 1, 3, seed=2, two_d=True)
     print("chol: ok", flush=True)
     return True
 
@@ -171,6 +170,19 @@ def gram_cases():
 
 
 CASES["gram"] = gram_cases
+
+
+def linalg_cases():
+    import tests.test_gpu_linalg as T
+    T.cu = lambda a: torch.from_numpy(np.ascontiguousarray(a))
+    T.linalg_chol_inv_case(2, 40, 4)
+    T.linalg_chol_case(2, 64, "subst", 0, m=2)
+    T.test_trsv_tile(2, 3)
+    print("linalg: ok", flush=True)
+    return True
+
+
+CASES["linalg"] = linalg_cases
 
 
 if __name__ == "__main__":  # noqa: E305

@@ -98,7 +98,7 @@ def test_tile_chol(L, nb, BT, logc):
     from tests.intutil import rnd_half_up, words_to_int, wrap
     A, W = _tile_inputs(L, nb, BT, logc, 7 * L + BT)
     Lw, _, X = O.tile_chol(W, L)
-    Lo, Xw = tile_factor(cu(A), BT)
+    Lo, Xw, _ = tile_factor(cu(A), BT)
     exp_L = to_planes([wrap(rnd_half_up(words_to_int(Lw[b, i, j]), 40), L) for b in range(nb) for i in range(BT)
                        for j in range(BT)], L).reshape(L, nb, BT, BT)
     _eq(Xw.cpu().numpy(), np.ascontiguousarray(X.transpose(3, 0, 1, 2)).view(np.int32), "tile inverse")
