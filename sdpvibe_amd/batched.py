@@ -84,7 +84,7 @@ def _residue_desc(R, X, nb, rows, K):
             base, rn = X, rows
         else:
             base, rn = X[:, :, j0:j0 + jw].contiguous(), jw
-        Rh = torch.empty((R.n, nb * rn, kp), dtype=torch.int8, device=X.device)
+        Rh = torch.empty((R.n, kp // 16, nb * rn, 16), dtype=torch.int8, device=X.device)
         Rl = torch.empty_like(Rh)
         R.split(base, None, None, L, nb * rn * K, nb, rn, rn * K, K, 1, k0, kc, kp, Rh, Rl)
         return Rh, Rl

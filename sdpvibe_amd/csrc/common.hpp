@@ -21,6 +21,16 @@ typedef int v16i __attribute__((ext_vector_type(16)));
 #ifndef SDPV_DYNAMIC_LDS
 #define SDPV_DYNAMIC_LDS(type, name) extern __shared__ __align__(16) type name[]
 #endif
+// 16 bytes per lane from global memory straight into LDS at ldsbase + 16 * lane (gfx950 direct
+// load; ldsbase uniform across the wave), and a wait until at most n vector-memory operations of
+// this wave are outstanding (s_waitcnt immediate: vmcnt split over bits 3:0 and 15:14).
+#ifndef SDPV_LDS_DMA16
+#define SDPV_LDS_DMA16(gptr, ldsbase)                                                            \
+    __builtin_amdgcn_global_load_lds((const void*)(gptr),                                        \
+                                     (__attribute__((address_space(3))) void*)(ldsbase), 16, 0, 0)
+#define SDPV_WAIT_VMCNT(n) \
+    __builtin_amdgcn_s_waitcnt(((n) & 15) | (7 << 4) | (15 << 8) | ((((n) >> 4) & 3) << 14))
+#endif
 #define SDPV_TRY_LAUNCH()                                   \
     do {                                                    \
         hipError_t err__ = hipGetLastError();               \

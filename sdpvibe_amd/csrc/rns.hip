@@ -178,7 +178,7 @@ __global__ void __launch_bounds__(256) k_rns_split(
             const i64 R = tr * TR + rr, kb = tk * TK + cc * 16;
             if (R < nrows && kb < kp) {
                 const v4i val = *(const v4i*)(stg + (size_t)(pl * 32 + pr) * EPW + rr * TK + cc * 16);
-                int8_t* dst = (pl ? Rl : Rh) + ((i64)(pb * 32 + pr) * nrows + R) * kp + kb;
+                int8_t* dst = (pl ? Rl : Rh) + (((i64)(pb * 32 + pr) * (kp >> 4) + (kb >> 4)) * nrows + R) * 16;
                 *(v4i*)dst = val;
             }
         }
@@ -187,22 +187,27 @@ __global__ void __launch_bounds__(256) k_rns_split(
 }
 
 // ================================================================== modular products
-// WM x WN waves, each BLM x BLN blocks of 32x32; BK = 64 bytes per stage, double-buffered LDS
-// holding the h, l and s = h + l planes of both operands (row stride 80 B: conflict-free b128).
-template <int WM, int WN, int BLM, int BLN>
-__global__ void __launch_bounds__(WM* WN * 64) k_rns_gemm(
+// Residues are stored chunk-major: byte k of row r of prime g at ((g*(kp/16) + k/16)*rows + r)*16
+// + k%16, so 64 consecutive rows of one 16-byte k-chunk are 1 KB of contiguous memory.
+// WM x WN waves, each BLM x BLN blocks of 32x32; BK = 64 bytes (4 chunks) per stage. Stages are
+// copied global -> LDS by direct loads (no registers), NST stages in flight, one barrier per stage.
+// LDS stage layout: [A h | A l | B h | B l], each plane [chunk][tile row][16 B]: fragment reads of
+// 32 consecutive rows are contiguous and conflict-free. The s = h + l operand is formed in
+// registers (bytewise add) from the h and l fragments.
+template <int WM, int WN, int BLM, int BLN, int NST>
+__global__ void __launch_bounds__(WM* WN * 64, 2) k_rns_gemm(
     const int8_t* __restrict__ Ah, const int8_t* __restrict__ Al, const int8_t* __restrict__ Bh,
     const int8_t* __restrict__ Bl, int G, int nb, i64 M, i64 N, i64 kp, int16_t* __restrict__ acc, i64 ld_g,
     const i32* __restrict__ mods, const double* __restrict__ minv, const i32* __restrict__ kcoef, int g0, int mode,
     i64 tiles_m, i64 tiles_n) {
-    constexpr int NT = WM * WN * 64;
+    constexpr int NWAVE = WM * WN;
     constexpr int BM = WM * BLM * 32, BN = WN * BLN * 32;
-    constexpr int RS = 80;
-    constexpr int ASZ = BM * RS, BSZ = BN * RS;
+    constexpr int PA = 64 * BM, PB = 64 * BN;       // bytes of one plane of one stage
+    constexpr int SBYTES = 2 * PA + 2 * PB;
+    constexpr int QA = BM / 16, QB = BN / 16;       // 1 KB direct loads per plane per stage
+    constexpr int LPW = (2 * QA + 2 * QB) / NWAVE;  // direct loads per wave per stage
+    static_assert((2 * QA + 2 * QB) % NWAVE == 0, "load split");
     SDPV_DYNAMIC_LDS(int8_t, smem8);
-    // buffer layout: [stage][plane h,l,s][A tile | B tile]
-    auto sA = [&](int st, int p) { return smem8 + (size_t)(st * 3 + p) * (ASZ + BSZ); };
-    auto sB = [&](int st, int p) { return smem8 + (size_t)(st * 3 + p) * (ASZ + BSZ) + ASZ; };
 
     const i64 total = tiles_m * tiles_n * (i64)G * nb;
     const i64 lg = xcd_remap(blockIdx.x, total);
@@ -215,72 +220,33 @@ __global__ void __launch_bounds__(WM* WN * 64) k_rns_gemm(
     const bool syrk = (mode & SDPV_GEMM_SYRK_LOWER) != 0;
     if (syrk && tn > tm) return;
 
-    const int t = threadIdx.x, wave = t / 64, lane = t & 63, l32 = lane & 31, hsel = lane >> 5;
+    const int t = threadIdx.x, lane = t & 63, l32 = lane & 31, hsel = lane >> 5;
+    const int wave = __builtin_amdgcn_readfirstlane(t / 64);   // uniform: scalar load addressing
     const int wm = wave / WN, wn = wave % WN;
+    const i64 nck = kp / 16;
     const i64 rowsA = (i64)nb * M, rowsB = (i64)nb * N;
-    const int8_t* pAh = Ah + ((i64)gi * rowsA + b * M) * kp;
-    const int8_t* pAl = Al + ((i64)gi * rowsA + b * M) * kp;
-    const int8_t* pBh = Bh + ((i64)gi * rowsB + b * N) * kp;
-    const int8_t* pBl = Bl + ((i64)gi * rowsB + b * N) * kp;
     const i64 r0 = tm * BM, c0 = tn * BN;
+    const i64 offA = ((i64)gi * nck * rowsA + b * M + r0) * 16;  // tile origin, chunk 0
+    const i64 offB = ((i64)gi * nck * rowsB + b * N + c0) * 16;
 
-    constexpr int PA = BM * 4, PB = BN * 4;  // (row, 16-byte chunk) pairs per stage
-    constexpr int NA = (PA + NT - 1) / NT, NBr = (PB + NT - 1) / NT;
-    v4i ra_h[1][NA], ra_l[1][NA], rb_h[1][NBr], rb_l[1][NBr];
-
-    auto gload = [&](int set, i64 kk) {
+    // direct loads of stage s into LDS buffer s % NST
+    auto issue = [&](int s) {
+        int8_t* sbase = smem8 + (s % NST) * SBYTES;
+        const i64 kc0 = (i64)s * 4;
 #pragma unroll
-        for (int u = 0; u < NA; u++) {
-            const int idx = t + u * NT;
-            const int row = idx >> 2, ch = idx & 3;
-            const i64 gr = r0 + row;
-            if (idx < PA && gr < M) {
-                ra_h[set][u] = *(const v4i*)(pAh + gr * kp + kk + ch * 16);
-                ra_l[set][u] = *(const v4i*)(pAl + gr * kp + kk + ch * 16);
+        for (int u = 0; u < LPW; u++) {
+            const int q = wave * LPW + u;                // uniform per wave
+            if (q < 2 * QA) {
+                const int p = q / QA, qq = q % QA;
+                const int e = qq * 64 + lane, c = e / BM, r = e % BM;
+                const int8_t* src = (p ? Al : Ah) + offA + ((kc0 + c) * rowsA + r) * 16;
+                if (r0 + r < M) SDPV_LDS_DMA16(src, sbase + p * PA + qq * 1024);
             } else {
-                ra_h[set][u] = zero4();
-                ra_l[set][u] = zero4();
-            }
-        }
-#pragma unroll
-        for (int u = 0; u < NBr; u++) {
-            const int idx = t + u * NT;
-            const int row = idx >> 2, ch = idx & 3;
-            const i64 gc = c0 + row;
-            if (idx < PB && gc < N) {
-                rb_h[set][u] = *(const v4i*)(pBh + gc * kp + kk + ch * 16);
-                rb_l[set][u] = *(const v4i*)(pBl + gc * kp + kk + ch * 16);
-            } else {
-                rb_h[set][u] = zero4();
-                rb_l[set][u] = zero4();
-            }
-        }
-    };
-    auto lstore = [&](int set, int st) {
-#pragma unroll
-        for (int u = 0; u < NA; u++) {
-            const int idx = t + u * NT;
-            if (idx < PA) {
-                const int row = idx >> 2, ch = idx & 3;
-                v4i s;
-#pragma unroll
-                for (int q = 0; q < 4; q++) s[q] = (int)add_bytes((u32)ra_h[set][u][q], (u32)ra_l[set][u][q]);
-                *(v4i*)(sA(st, 0) + row * RS + ch * 16) = ra_h[set][u];
-                *(v4i*)(sA(st, 1) + row * RS + ch * 16) = ra_l[set][u];
-                *(v4i*)(sA(st, 2) + row * RS + ch * 16) = s;
-            }
-        }
-#pragma unroll
-        for (int u = 0; u < NBr; u++) {
-            const int idx = t + u * NT;
-            if (idx < PB) {
-                const int row = idx >> 2, ch = idx & 3;
-                v4i s;
-#pragma unroll
-                for (int q = 0; q < 4; q++) s[q] = (int)add_bytes((u32)rb_h[set][u][q], (u32)rb_l[set][u][q]);
-                *(v4i*)(sB(st, 0) + row * RS + ch * 16) = rb_h[set][u];
-                *(v4i*)(sB(st, 1) + row * RS + ch * 16) = rb_l[set][u];
-                *(v4i*)(sB(st, 2) + row * RS + ch * 16) = s;
+                const int q2 = q - 2 * QA;
+                const int p = q2 / QB, qq = q2 % QB;
+                const int e = qq * 64 + lane, c = e / BN, r = e % BN;
+                const int8_t* src = (p ? Bl : Bh) + offB + ((kc0 + c) * rowsB + r) * 16;
+                if (c0 + r < N) SDPV_LDS_DMA16(src, sbase + 2 * PA + p * PB + qq * 1024);
             }
         }
     };
@@ -295,45 +261,55 @@ __global__ void __launch_bounds__(WM* WN * 64) k_rns_gemm(
             cLL[i][j] = zero16();
         }
 
-    auto compute = [&](int cur) {
+    auto compute = [&](int s) {
+        const int8_t* sA = smem8 + (s % NST) * SBYTES;
+        const int8_t* sB = sA + 2 * PA;
 #pragma unroll
         for (int ks = 0; ks < 2; ks++) {
-            const int koff = ks * 32 + 16 * hsel;
-            v4i fa[BLM][3], fb[BLN][3];
+            const int c = 2 * ks + hsel;
+            v4i ah[BLM], al[BLM], as[BLM], bh[BLN], bl[BLN], bs[BLN];
 #pragma unroll
             for (int i = 0; i < BLM; i++) {
-                const int row = (wm * BLM + i) * 32 + l32;
-#pragma unroll
-                for (int p = 0; p < 3; p++) fa[i][p] = *(const v4i*)(sA(cur, p) + row * RS + koff);
+                const int off = (c * BM + (wm * BLM + i) * 32 + l32) * 16;
+                ah[i] = *(const v4i*)(sA + off);
+                al[i] = *(const v4i*)(sA + PA + off);
             }
 #pragma unroll
             for (int j = 0; j < BLN; j++) {
-                const int col = (wn * BLN + j) * 32 + l32;
-#pragma unroll
-                for (int p = 0; p < 3; p++) fb[j][p] = *(const v4i*)(sB(cur, p) + col * RS + koff);
+                const int off = (c * BN + (wn * BLN + j) * 32 + l32) * 16;
+                bh[j] = *(const v4i*)(sB + off);
+                bl[j] = *(const v4i*)(sB + PB + off);
             }
 #pragma unroll
             for (int i = 0; i < BLM; i++)
 #pragma unroll
+                for (int q = 0; q < 4; q++) as[i][q] = (int)add_bytes((u32)ah[i][q], (u32)al[i][q]);
+#pragma unroll
+            for (int j = 0; j < BLN; j++)
+#pragma unroll
+                for (int q = 0; q < 4; q++) bs[j][q] = (int)add_bytes((u32)bh[j][q], (u32)bl[j][q]);
+#pragma unroll
+            for (int i = 0; i < BLM; i++)
+#pragma unroll
                 for (int j = 0; j < BLN; j++) {
-                    cHH[i][j] = mfma(fa[i][0], fb[j][0], cHH[i][j]);
-                    cLL[i][j] = mfma(fa[i][1], fb[j][1], cLL[i][j]);
-                    cSS[i][j] = mfma(fa[i][2], fb[j][2], cSS[i][j]);
+                    cHH[i][j] = mfma(ah[i], bh[j], cHH[i][j]);
+                    cLL[i][j] = mfma(al[i], bl[j], cLL[i][j]);
+                    cSS[i][j] = mfma(as[i], bs[j], cSS[i][j]);
                 }
         }
     };
 
-    // stage s: global loads issued during the compute of stage s-1, then stored to LDS buffer s % 2
-    const i64 nst = kp / 64;
-    gload(0, 0);
-    lstore(0, 0);
-    __syncthreads();
-    for (i64 st = 0; st < nst; st++) {
-        const int cur = (int)(st & 1);
-        if (st + 1 < nst) gload(0, (st + 1) * 64);
-        compute(cur);
-        if (st + 1 < nst) lstore(0, cur ^ 1);
+    const int nst = (int)(kp / 64);
+#pragma unroll
+    for (int s = 0; s < NST - 1; s++)
+        if (s < nst) issue(s);
+    for (int s = 0; s < nst; s++) {
+        // this wave's loads of stage s are done once at most (stages issued after s) x LPW remain
+        if (s + NST - 2 < nst) SDPV_WAIT_VMCNT((NST - 2) * LPW);
+        else SDPV_WAIT_VMCNT(0);
         __syncthreads();
+        if (s + NST - 1 < nst) issue(s + NST - 1);
+        compute(s);
     }
 
     // Epilogue: v = k0*HH + k1*SS + k2*LL (+ old) is exact in double (|v| < 2^47), reduced once.
@@ -364,7 +340,7 @@ __global__ void __launch_bounds__(WM* WN * 64) k_rns_gemm(
         }
 }
 
-template <int WM, int WN, int BLM, int BLN>
+template <int WM, int WN, int BLM, int BLN, int NST>
 int launch_gemm(const int8_t* Ah, const int8_t* Al, const int8_t* Bh, const int8_t* Bl, int G, int nb, i64 M, i64 N,
                 i64 kp, int16_t* acc, i64 ld_g, const i32* mods, const double* minv, const i32* kcoef, int g0,
                 int mode, hipStream_t st) {
@@ -373,15 +349,15 @@ int launch_gemm(const int8_t* Ah, const int8_t* Al, const int8_t* Bh, const int8
     const i64 total = tiles_m * tiles_n * (i64)G * nb;
     if (total == 0) return SDPV_OK;
     if (total > 0x7fffffffLL) return SDPV_ESIZE;
-    const size_t lds = (size_t)2 * 3 * (BM + BN) * 80;
+    const size_t lds = (size_t)NST * 128 * (BM + BN);
     static bool attr_set = false;
     if (!attr_set) {
-        (void)hipFuncSetAttribute((const void*)k_rns_gemm<WM, WN, BLM, BLN>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                            (int)lds);
+        (void)hipFuncSetAttribute((const void*)k_rns_gemm<WM, WN, BLM, BLN, NST>,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         attr_set = true;
     }
-    hipLaunchKernelGGL((k_rns_gemm<WM, WN, BLM, BLN>), dim3((unsigned)total), dim3(WM * WN * 64), lds, st, Ah, Al,
-                       Bh, Bl, G, nb, M, N, kp, acc, ld_g, mods, minv, kcoef, g0, mode, tiles_m, tiles_n);
+    hipLaunchKernelGGL((k_rns_gemm<WM, WN, BLM, BLN, NST>), dim3((unsigned)total), dim3(WM * WN * 64), lds, st, Ah,
+                       Al, Bh, Bl, G, nb, M, N, kp, acc, ld_g, mods, minv, kcoef, g0, mode, tiles_m, tiles_n);
     SDPV_TRY_LAUNCH();
     return SDPV_OK;
 }
@@ -634,12 +610,12 @@ int sdpv_rns_gemm(const int8_t* Ah, const int8_t* Al, const int8_t* Bh, const in
     if (kp == 0) return SDPV_EARG;
     const i64 mn = M < N ? M : N;
     if (mn >= 128)
-        return launch_gemm<2, 2, 2, 2>(Ah, Al, Bh, Bl, (int)G, (int)nb, M, N, kp, acc, ld_g, mods, minv, kcoef,
+        return launch_gemm<2, 2, 2, 2, 2>(Ah, Al, Bh, Bl, (int)G, (int)nb, M, N, kp, acc, ld_g, mods, minv, kcoef,
                                        (int)g0, mode, st);
     if (mn >= 64)
-        return launch_gemm<2, 2, 1, 1>(Ah, Al, Bh, Bl, (int)G, (int)nb, M, N, kp, acc, ld_g, mods, minv, kcoef,
+        return launch_gemm<2, 2, 1, 1, 2>(Ah, Al, Bh, Bl, (int)G, (int)nb, M, N, kp, acc, ld_g, mods, minv, kcoef,
                                        (int)g0, mode, st);
-    return launch_gemm<1, 1, 1, 1>(Ah, Al, Bh, Bl, (int)G, (int)nb, M, N, kp, acc, ld_g, mods, minv, kcoef, (int)g0,
+    return launch_gemm<1, 1, 1, 1, 2>(Ah, Al, Bh, Bl, (int)G, (int)nb, M, N, kp, acc, ld_g, mods, minv, kcoef, (int)g0,
                                    mode, st);
 }
 

@@ -354,13 +354,13 @@ def gemm(A, B, transA=False, transB=False):
         return MPF(out_m, out_e)
 
     def fa(k0, kc, kp, j0, jw):
-        Rh = torch.empty((R.n, nb * M, kp), dtype=torch.int8, device=dev)
+        Rh = torch.empty((R.n, kp // 16, nb * M, 16), dtype=torch.int8, device=dev)
         Rl = torch.empty_like(Rh)
         R.split(Am, Ae, EA, L, Ae.numel(), nb, M, aldb, aldr, aldk, k0, kc, kp, Rh, Rl)
         return Rh, Rl
 
     def fb(k0, kc, kp, j0, jw):
-        Rh = torch.empty((R.n, nb * jw, kp), dtype=torch.int8, device=dev)
+        Rh = torch.empty((R.n, kp // 16, nb * jw, 16), dtype=torch.int8, device=dev)
         Rl = torch.empty_like(Rh)
         if (j0, jw) == (0, N):
             R.split(Bm, Be, EB, L, Be.numel(), nb, N, bldb, bldr, bldk, k0, kc, kp, Rh, Rl)
@@ -522,7 +522,7 @@ class GramAcc:
         for k0 in range(0, rows, kc):
             kk = min(kc, rows - k0)
             kp = R.kp(kk)
-            Rh = torch.empty((R.n, N, kp), dtype=torch.int8, device=dev)
+            Rh = torch.empty((R.n, kp // 16, N, 16), dtype=torch.int8, device=dev)
             Rl = torch.empty_like(Rh)
             # rows of the split = columns j of P (stride 1), inner index = rows of P (stride N)
             R.split(Pm, Pe, Erow, L, rows * N, 1, N, 0, 1, N, k0, kk, kp, Rh, Rl)

@@ -219,7 +219,7 @@ def _limb_desc(R, X, rows, K, nb=1):
         Xs = X.view(L, nb, rows, K)
         base = Xs[:, :, r0:r0 + rn] if (r0, rn) != (0, rows) else Xs
         base = base.contiguous()
-        Rh = torch.empty((R.n, nb * rn, kp), dtype=torch.int8, device=dev)
+        Rh = torch.empty((R.n, kp // 16, nb * rn, 16), dtype=torch.int8, device=dev)
         Rl = torch.empty_like(Rh)
         R.split(base, None, None, L, nb * rn * K, nb, rn, rn * K, K, 1, k0, kc, kp, Rh, Rl)
         return Rh, Rl

@@ -61,6 +61,18 @@ def gemm_config(M, N, K):
     return 32, 32
 
 
+def row_major(R):
+    """chunk-major residues (G, kp/16, rows, 16) -> (G, rows, kp)"""
+    G, nc, rows, _ = R.shape
+    return R.permute(0, 2, 1, 3).reshape(G, rows, nc * 16).contiguous()
+
+
+def chunk_major(R):
+    """(G, rows, kp) residues -> the kernels' chunk-major layout (G, kp/16, rows, 16)"""
+    G, rows, kp = R.shape
+    return R.reshape(G, rows, kp // 16, 16).permute(0, 2, 1, 3).contiguous()
+
+
 class RNS:
     """Moduli and tables for exact products of L-limb integers with inner dimension <= kmax
     (rns.py:163-191)."""
@@ -121,10 +133,10 @@ class RNS:
         G = self.n - g0 if G is None else G
         kp = self.kp(kc)
         X = X.contiguous()
-        Rh = torch.empty((G, rows, kp), dtype=torch.int8, device=X.device)
+        Rh = torch.empty((G, kp // 16, rows, 16), dtype=torch.int8, device=X.device)
         Rl = torch.empty_like(Rh)
         self.split(X, None, None, L, rows * K, 1, rows, 0, K, 1, k0, kc, kp, Rh, Rl, g0, G)
-        return Rh, Rl
+        return row_major(Rh), row_major(Rl)
 
     def split(self, X, e, Erow, Lin, plane, nb, rows, ldb, ldr, ldk, k0, kc, kp, Rh, Rl, g0=0, G=None,
               flag=None):
@@ -138,18 +150,20 @@ class RNS:
 
     # ---- step 2: accumulate residues of A @ Bt^T
     def product(self, Ah, Al, Bh, Bl, acc, nb, M, N, ld_g, g0=0, accumulate=False, syrk=False):
-        G, _, kp = Ah.shape
+        """residues in the kernels' chunk-major layout (G, kp/16, rows, 16)"""
+        G, kp = Ah.shape[0], Ah.shape[1] * 16
         mode = (1 if accumulate else 0) | (2 if syrk else 0)
         call("sdpv_rns_gemm", ptr(Ah), ptr(Al), ptr(Bh), ptr(Bl), G, nb, M, N, kp, ptr(acc), ld_g,
              ptr(self.m32), ptr(self.minv), ptr(self.kcoef), g0, mode, stream())
 
     def accumulate(self, Ahi, Alo, Bhi, Blo, acc, g0=0):
-        """acc[g0:g0+G] += residues of A @ B^T for the G moduli whose residues are given
-        (rns.py:217-224)"""
+        """acc[g0:g0+G] += residues of A @ B^T for the G moduli whose residues are given in the
+        (G, rows, kp) layout of residues() (rns.py:217-224)"""
         G, M, KC = Ahi.shape
         N = Bhi.shape[1]
         assert acc.is_contiguous() and acc.shape[1:] == (M, N)
-        self.product(Ahi, Alo, Bhi, Blo, acc, 1, M, N, M * N, g0, accumulate=True)
+        Ah, Al, Bh, Bl = (chunk_major(x) for x in (Ahi, Alo, Bhi, Blo))
+        self.product(Ah, Al, Bh, Bl, acc, 1, M, N, M * N, g0, accumulate=True)
 
     # ---- step 3: CRT of a block of entries
     def crt(self, acc, rows, N, j0, w, mode, ld_g=None, W=None, bitlen=None, outm=None, oute=None, L=None,

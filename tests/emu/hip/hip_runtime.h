@@ -70,6 +70,13 @@ typedef int emu_v16i __attribute__((ext_vector_type(16)));
 emu_v16i emu_mfma_i32_32x32x32_i8(emu_v4i a, emu_v4i b, emu_v16i c, int, int, int);
 #define __builtin_amdgcn_mfma_i32_32x32x32_i8 emu_mfma_i32_32x32x32_i8
 
+// Direct global -> LDS load of 16 bytes per lane to ldsbase + 16 * lane; vmcnt waits are no-ops
+// (the emulated load completes at once).
+#define SDPV_LDS_DMA16(gptr, ldsbase) \
+    std::memcpy((unsigned char*)(ldsbase) + 16 * (threadIdx.x & 63), (const void*)(gptr), 16)
+#define SDPV_WAIT_VMCNT(n) ((void)(n))
+#define __builtin_amdgcn_readfirstlane(x) (x)
+
 // Dynamic shared memory of the block being run (blocks run one at a time).
 extern unsigned char* emu_dynamic_lds;
 #define SDPV_DYNAMIC_LDS(type, name) type* name = reinterpret_cast<type*>(emu_dynamic_lds)
