@@ -183,6 +183,27 @@ int sdpv_tile_chol(const int32_t* A, int64_t L, int64_t nb, int64_t BT, int32_t*
 int sdpv_tile_trsv(const uint32_t* Lw, const uint32_t* RINV, uint32_t* V, int64_t nvec, int64_t BT, int64_t WL,
                    int64_t LF, void* stream);
 
+/* ---------------------------------------------------------------- FP64 step-length pieces */
+
+/* out[i] = MPF.to_float of entry i (m: (L, n) planes, e: (n,)) after multiplying it by
+ * 2^-shift[i / per] (shift may be null): the top two limbs as a double times
+ * 2^clamp(e - shift + 32(L - 2), -1100, 1100); zeros give 0.
+ * Replaces MPF.shift(...).to_float() (sdpvibe/mpf.py:194-197, :210-215). */
+int sdpv_mpf_to_double(const int32_t* m, const int64_t* e, int64_t L, int64_t n, const int64_t* shift,
+                       int64_t per, double* out, void* stream);
+
+/* Exact conversion of n doubles to normalised MPF (L >= 2): mantissa * 2^(32L - 55), zero -> EZ.
+ * Replaces solver._mpf_from_float (sdpvibe/solver.py:1019-1025). */
+int sdpv_mpf_from_double(const double* v, int64_t n, int64_t L, int32_t* m, int64_t* e, void* stream);
+
+/* Eigenvalues (ascending, w (nb, n)) and optionally eigenvectors (columns of V (nb, n, n)) of the
+ * symmetrised (A + A^T)/2 of nb float64 n x n matrices (n <= 255), and their Frobenius norms
+ * (fro (nb,), may be null). work: 2 * nb * m * m doubles, m = n rounded up to even.
+ * Parallel cyclic Jacobi. Replaces numpy.linalg.eigvalsh / eigh in min_eig and refine_min_eig
+ * (sdpvibe/solver.py:1039, :1077). */
+int sdpv_symeig(const double* A, int64_t nb, int64_t n, double* work, double* w, double* V, double* fro,
+                void* stream);
+
 /* ---------------------------------------------------------------- diagnostics */
 
 /* D = A * B through one wave of v_mfma_i32_32x32x32_i8 using the same fragment mapping as the

@@ -32,7 +32,7 @@ def build():
 
 def install():
     import torch
-    from sdpvibe_amd import _lib, basecase, batched, limbops, mpf, mpmat, rns
+    from sdpvibe_amd import _lib, basecase, batched, limbops, mpf, mpmat, rns, solver
 
     h = ctypes.CDLL(EMU)
     for name, (args, res) in _lib.SIGNATURES.items():
@@ -47,7 +47,7 @@ def install():
         return ctypes.c_void_p(t.data_ptr())
 
     torch.cuda.current_device = lambda: 0
-    for mod in (limbops, mpf, mpmat, rns, batched, basecase):
+    for mod in (limbops, mpf, mpmat, rns, batched, basecase, solver):
         if hasattr(mod, "ptr"):
             mod.ptr = emu_ptr
         if hasattr(mod, "stream"):

@@ -185,6 +185,27 @@ def linalg_cases():
 CASES["linalg"] = linalg_cases
 
 
+def eig_cases():
+    import tests.test_gpu_steplen as T
+    from sdpvibe_amd import solver as S
+    orig = torch.Tensor.cuda
+    torch.Tensor.cuda = lambda self, *a, **k: self
+    try:
+        for n in (1, 2, 7, 40):
+            T.test_symeig_matches_numpy(n)
+        T.test_mpf_from_float_exact()
+    finally:
+        torch.Tensor.cuda = orig
+    A = F.from_mp([mp_val for mp_val in ["1.5", "-2.25e300", "3e-320", "0", "7", "-1e-5"]], 24, (1, 2, 3))
+    d = S.to_double(A, torch.tensor([0], dtype=torch.int64))
+    assert np.array_equal(d.numpy(), A.to_float())
+    print("eig: ok", flush=True)
+    return True
+
+
+CASES["eig"] = eig_cases
+
+
 if __name__ == "__main__":  # noqa: E305
     ok = True
     for c in sys.argv[1:] or list(CASES):
