@@ -39,6 +39,13 @@ typedef int v16i __attribute__((ext_vector_type(16)));
 
 static inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
 
+// Orders LDS writes of a wave before later reads by other lanes of the same wave.
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 __device__ __forceinline__ int clz32(u32 x) { return __clz((int)x); }
 
 // Word i of the infinite two's-complement extension of an ns-word value (sdpvibe/limbops.py:22-28).

@@ -43,146 +43,132 @@ __device__ __forceinline__ v16i mfma(v4i a, v4i b, v16i c) {
 }
 
 // ================================================================== residue split
-// One workgroup: EPW entries (TR rows x TK consecutive k).
-// Phase 1: the entries' Lin limbs are copied into LDS with independent (unrolled) loads, then each
-//   thread shift-rounds its entry into Lout limbs and stores the limb bytes (as u - 128) in LDS.
-// Phase 2: per group of 32 primes, the byte dot products T_hi . x and T_lo . x run on MFMA (primes
-//   = rows, entries = columns); the reduction modulo m uses per-block prime constants in LDS; the
-//   int8 planes are staged in LDS and written as 16-byte chunks.
+// One wavefront splits 64 entries on its own (no workgroup barriers): a tile of 4 rows x 16
+// consecutive k (k contiguous in memory, ldk == 1) or 16 rows x 4 consecutive k (otherwise).
+// Phase 1: lane s streams its entry's limbs through the rounding shift into Lout words, stored as
+//   bytes u - 128 in the wave's LDS rows (slot s); the entry's sign goes into a wave ballot.
+// Phase 2, per block of 32 primes: the byte dot products x . T_hi and x . T_lo run on MFMA with the
+//   64 slots as rows (two 32-row blocks) and the primes as columns, so each lane ends up with four
+//   runs of 4 slots that are 4 consecutive k of one row; r = 128 T_hi.x + T_lo.x + C1 - neg C2 is
+//   reduced to the centred residue (double reciprocal, exact for |r| < 2^30), split into
+//   128 hi + lo and written as packed 4-byte runs straight into the chunk-major residue planes.
 struct LdsStrided {
     const u32* p;
     int stride;
     __device__ __forceinline__ u32 operator()(int k) const { return p[k * stride]; }
 };
 
+__device__ __forceinline__ u32 pack4(int a, int b, int c, int d) {  // low bytes of a, b, c, d
+    const u32 ab = __builtin_amdgcn_perm((u32)b, (u32)a, 0x0c0c0400u);
+    const u32 cd = __builtin_amdgcn_perm((u32)d, (u32)c, 0x0c0c0400u);
+    return __builtin_amdgcn_perm(cd, ab, 0x05040100u);
+}
+
 __global__ void __launch_bounds__(256) k_rns_split(
     const i32* __restrict__ X, const i64* __restrict__ e, const i64* __restrict__ Erow, int L, int Lout, i64 plane,
     i64 nb, i64 rows, i64 ldb, i64 ldr, i64 ldk, i64 k0, i64 kc, i64 kp, const int8_t* __restrict__ Thi,
     const int8_t* __restrict__ Tlo, int NBp, const i32* __restrict__ C1, const i32* __restrict__ C2,
     const i32* __restrict__ mods, const double* __restrict__ minv, int g0, int G, int8_t* __restrict__ Rh,
-    int8_t* __restrict__ Rl, i32* __restrict__ flag, int EPW, int TK, i64 tiles_k) {
+    int8_t* __restrict__ Rl, i32* __restrict__ flag, i64 ntiles_k, i64 ntiles) {
     SDPV_DYNAMIC_LDS(unsigned char, smem);
-    const int XS = NBp + 16;
-    unsigned char* xs = smem;                           // EPW x XS limb bytes
-    unsigned char* negs = xs + (size_t)EPW * XS;        // EPW sign flags
-    double* cmi = (double*)(negs + EPW);                // 32 prime constants of the current group
-    i32* cm = (i32*)(cmi + 32);
-    i32* cc1 = cm + 32;
-    i32* cc2 = cc1 + 32;
-    unsigned char* uni = (unsigned char*)(cc2 + 32);    // phase 1: raw limbs [L][EPW]; phase 2: staging
-    u32* raw = (u32*)uni;
-    int8_t* stg = (int8_t*)uni;                         // 2 planes x 32 primes x EPW
-    const int t = threadIdx.x;
-    const int TR = EPW / TK;
-    const i64 tr = blockIdx.x / tiles_k, tk = blockIdx.x % tiles_k;
+    const int XS = NBp + 16;                            // bytes per slot row (multiple of 16)
+    const int t = threadIdx.x, lane = t & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(t >> 6);
+    const i64 tile = (i64)blockIdx.x * 4 + wave;
+    if (tile >= ntiles) return;                         // whole wave
+    unsigned char* xs = smem + (size_t)wave * 64 * XS;
+    const bool kmajor = ldk == 1;                       // 4 rows x 16 k, else 16 rows x 4 k
+    const i64 tr = tile / ntiles_k, tk = tile % ntiles_k;
+    const i64 row0 = kmajor ? tr * 4 : tr * 16;
+    const i64 kk0 = kmajor ? tk * 16 : tk * 4;
     const i64 nrows = nb * rows;
+    auto slot_row = [&](int s) { return row0 + (kmajor ? (s >> 4) : (s >> 2)); };
+    auto slot_k = [&](int s) { return kk0 + (kmajor ? (s & 15) : (s & 3)); };
 
-    if (t < EPW) {
-        const i64 R = tr * TR + t / TK, k = tk * TK + t % TK;
-        u32* row = (u32*)(xs + (size_t)t * XS);
-        int neg = 0;
+    // ---- phase 1: slot s = lane
+    int neg = 0;
+    {
+        const i64 R = slot_row(lane), k = slot_k(lane);
+        u32* row = (u32*)(xs + (size_t)lane * XS);
         if (R < nrows && k < kc) {
             const i64 b = R / rows, r = R % rows;
             const i64 off = b * ldb + r * ldr + (k0 + k) * ldk;
-            const i32* src = X + off;
-            int w = 0;
-            for (; w + 8 <= L; w += 8) {
-                u32 v[8];
-#pragma unroll
-                for (int u = 0; u < 8; u++) v[u] = (u32)src[(i64)(w + u) * plane];
-#pragma unroll
-                for (int u = 0; u < 8; u++) raw[(w + u) * EPW + t] = v[u];
-            }
-            for (; w < L; w++) raw[w * EPW + t] = (u32)src[(i64)w * plane];
             i64 s = 0;
             if (e) {
                 const i64 ex = e[off];
                 s = (ex == EZV) ? (i64)(1 << 30) : Erow[R] - ex;
             }
-            const int bad = shr_round(LdsStrided{raw + t, EPW}, L, s, Lout, [&](int wi, u32 v) {
+            const int bad = shr_round(PlaneSrc{X + off, plane}, L, s, Lout, [&](int wi, u32 v) {
                 row[wi] = v ^ 0x80808080u;
                 if (wi == Lout - 1) neg = (int)(v >> 31);
             });
             if (bad && flag) atomicOr(flag, 1);
         } else {
-            for (int w = 0; w < Lout; w++) row[w] = 0u;
+            for (int w = 0; w < Lout; w++) row[w] = 0x80808080u;   // zero entry
         }
-        for (int w = Lout; w < NBp / 4; w++) row[w] = 0u;
-        negs[t] = (unsigned char)neg;
+        for (int w = Lout; w < NBp / 4; w++) row[w] = 0x80808080u;
     }
+    const u64 negmask = __ballot(neg);
+    wave_sync();
 
-    const int wave = t >> 6, lane = t & 63, l32 = lane & 31, hsel = lane >> 5;
-    const int nEB = EPW / 32, nPB = (G + 31) / 32, KS = NBp / 32;
+    // ---- phase 2
+    const int l32 = lane & 31, hsel = lane >> 5;
+    const int KS = NBp / 32, nPB = (G + 31) / 32;
+    const i64 nck = kp >> 4;
     for (int pb = 0; pb < nPB; pb++) {
-        if (t < 32) {
-            const int gi = pb * 32 + t;
-            const int g = g0 + (gi < G ? gi : 0);
-            cm[t] = mods[g];
-            cmi[t] = minv[g];
-            cc1[t] = C1[g];
-            cc2[t] = C2[g];
-        }
-        __syncthreads();
-        for (int eb = wave; eb < nEB; eb += 4) {
+        const int pr = pb * 32 + l32;
+        const bool pok = pr < G;
+        const int g = g0 + (pok ? pr : 0);
+        const int8_t* th = Thi + (i64)g * NBp + 16 * hsel;
+        const int8_t* tl = Tlo + (i64)g * NBp + 16 * hsel;
+        const int m = mods[g];
+        const double mi = minv[g];
+        const int c1 = C1[g], c2 = C2[g];
+#pragma unroll
+        for (int eb = 0; eb < 2; eb++) {
             v16i ah = zero16(), al = zero16();
-            const int prow = pb * 32 + l32;
-            const bool pok = prow < G;
-            const int8_t* th = Thi + (i64)(g0 + (pok ? prow : 0)) * NBp;
-            const int8_t* tl = Tlo + (i64)(g0 + (pok ? prow : 0)) * NBp;
-            const unsigned char* xrow = xs + (size_t)(eb * 32 + l32) * XS;
-            for (int kb = 0; kb < KS; kb += 4) {
-                v4i fah[4], fal[4];
+            const unsigned char* xrow = xs + (size_t)(eb * 32 + l32) * XS + 16 * hsel;
+            for (int ks = 0; ks < KS; ks += 4) {
+                v4i fh[4], fl[4];
 #pragma unroll
                 for (int u = 0; u < 4; u++) {
-                    const int boff = (kb + u) * 32 + 16 * hsel;
-                    const bool ok = pok && (kb + u < KS);
-                    fah[u] = ok ? *(const v4i*)(th + boff) : zero4();
-                    fal[u] = ok ? *(const v4i*)(tl + boff) : zero4();
+                    const bool ok = ks + u < KS;
+                    fh[u] = ok ? *(const v4i*)(th + (ks + u) * 32) : zero4();
+                    fl[u] = ok ? *(const v4i*)(tl + (ks + u) * 32) : zero4();
                 }
 #pragma unroll
                 for (int u = 0; u < 4; u++) {
-                    if (kb + u < KS) {
-                        const v4i fb = *(const v4i*)(xrow + (kb + u) * 32 + 16 * hsel);
-                        ah = mfma(fah[u], fb, ah);
-                        al = mfma(fal[u], fb, al);
+                    if (ks + u < KS) {
+                        const v4i fx = *(const v4i*)(xrow + (ks + u) * 32);
+                        ah = mfma(fx, fh[u], ah);
+                        al = mfma(fx, fl[u], al);
                     }
                 }
             }
-            const int c = eb * 32 + l32;
-            const int cneg = negs[c];
-            const i64 R = tr * TR + c / TK, k = tk * TK + c % TK;
-            const bool cvalid = (R < nrows) && (k < kc);
+            // lane: prime pr, slots eb*32 + 8 jq + 4 hsel + u (u = 0..3: 4 consecutive k of one row)
 #pragma unroll
-            for (int j = 0; j < 16; j++) {
-                const int pr = 8 * (j >> 2) + 4 * hsel + (j & 3);
-                int hi = 0, lo = 0;
-                if (pb * 32 + pr < G && cvalid) {
-                    const int m = cm[pr];
-                    const int x = ah[j] * 128 + al[j] + cc1[pr] - (cneg ? cc2[pr] : 0);
-                    split_centred(mod_m(x, m, cmi[pr]), m, hi, lo);
+            for (int jq = 0; jq < 4; jq++) {
+                const int s0 = eb * 32 + 8 * jq + 4 * hsel;
+                int hi[4], lo[4];
+#pragma unroll
+                for (int u = 0; u < 4; u++) {
+                    const int j = 4 * jq + u;
+                    const int nb_ = (int)((negmask >> (s0 + u)) & 1ull);
+                    const int x = ah[j] * 128 + al[j] + c1 - (nb_ ? c2 : 0);
+                    const double q = rint((double)x * mi);
+                    const int c = x - (int)q * m;                   // centred residue
+                    const int tt = c + 64;
+                    hi[u] = tt >> 7;
+                    lo[u] = (tt & 127) - 64;
                 }
-                stg[(size_t)pr * EPW + c] = (int8_t)hi;
-                stg[(size_t)(32 + pr) * EPW + c] = (int8_t)lo;
+                const i64 R = slot_row(s0), k = slot_k(s0);
+                if (pok && R < nrows && k < kp) {
+                    const i64 o = (((i64)pr * nck + (k >> 4)) * nrows + R) * 16 + (k & 15);
+                    *(u32*)(Rh + o) = pack4(hi[0], hi[1], hi[2], hi[3]);
+                    *(u32*)(Rl + o) = pack4(lo[0], lo[1], lo[2], lo[3]);
+                }
             }
         }
-        __syncthreads();
-        const int npr = (G - pb * 32) < 32 ? (G - pb * 32) : 32;
-        const int cpr = TK / 16;
-        const int total = 2 * npr * TR * cpr;
-        for (int idx = t; idx < total; idx += 256) {
-            const int cc = idx % cpr;
-            int rest = idx / cpr;
-            const int rr = rest % TR;
-            rest /= TR;
-            const int pr = rest % npr, pl = rest / npr;
-            const i64 R = tr * TR + rr, kb = tk * TK + cc * 16;
-            if (R < nrows && kb < kp) {
-                const v4i val = *(const v4i*)(stg + (size_t)(pl * 32 + pr) * EPW + rr * TK + cc * 16);
-                int8_t* dst = (pl ? Rl : Rh) + (((i64)(pb * 32 + pr) * (kp >> 4) + (kb >> 4)) * nrows + R) * 16;
-                *(v4i*)dst = val;
-            }
-        }
-        __syncthreads();
     }
 }
 
@@ -373,11 +359,6 @@ int launch_gemm(const int8_t* Ah, const int8_t* Al, const int8_t* Bh, const int8
 //   words 1, 3, 5, 7; each lane folds its digits into 4 word partials sum (Z_d - k Md_d) 256^u,
 //   the odd partials move across with one exchange and lane e carries the 8 words in order.
 // Phase 3: words -> W + bitlen (mode 0), or normalised MPF limbs (1), or a fixed-exponent rounding (2).
-__device__ __forceinline__ void wave_sync() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
 
 __global__ void __launch_bounds__(256) k_rns_crt(
     const int16_t* __restrict__ acc, int n, i64 ld_g, i64 rows, i64 N, i64 j0, i64 w, const i32* __restrict__ mods,
@@ -576,14 +557,13 @@ int sdpv_rns_split(const int32_t* X, const int64_t* e, const int64_t* Erow, int6
     if (nrows == 0 || kp == 0 || G == 0) return SDPV_OK;
     if (!X || !Thi || !Tlo || !C1 || !C2 || !mods || !minv || !Rh || !Rl) return SDPV_EARG;
     if (e && !Erow) return SDPV_EARG;
-    const int EPW = (4 * Lout <= 128) ? 256 : ((4 * Lout <= 256) ? 128 : 64);
-    const int TK = (ldk == 1) ? 64 : 16;
-    const int TR = EPW / TK;
-    const i64 tiles_k = kp / TK, tiles_r = (nrows + TR - 1) / TR;
-    const i64 grid = tiles_r * tiles_k;
+    const bool kmajor = ldk == 1;
+    const i64 ntiles_k = kmajor ? kp / 16 : kp / 4;
+    const i64 ntiles_r = kmajor ? (nrows + 3) / 4 : (nrows + 15) / 16;
+    const i64 ntiles = ntiles_r * ntiles_k;
+    const i64 grid = (ntiles + 3) / 4;
     if (grid > 0x7fffffffLL) return SDPV_ESIZE;
-    const size_t uni = (size_t)EPW * (L * 4 > 64 ? L * 4 : 64);
-    const size_t lds = (size_t)EPW * (NBp + 16) + EPW + 32 * 8 + 3 * 32 * 4 + uni;
+    const size_t lds = (size_t)4 * 64 * (NBp + 16);
     if (lds > 160 * 1024) return SDPV_ESIZE;
     static bool attr_set = false;
     if (!attr_set) {
@@ -592,8 +572,8 @@ int sdpv_rns_split(const int32_t* X, const int64_t* e, const int64_t* Erow, int6
     }
     hipLaunchKernelGGL(k_rns_split, dim3((unsigned)grid), dim3(256), lds, as_stream(stream), X, e, Erow, (int)L,
                        (int)Lout, (i64)plane, (i64)nb, (i64)rows, (i64)ldb, (i64)ldr, (i64)ldk, (i64)k0, (i64)kc,
-                       (i64)kp, Thi, Tlo, (int)NBp, C1, C2, mods, minv, (int)g0, (int)G, Rh, Rl, flag, EPW, TK,
-                       tiles_k);
+                       (i64)kp, Thi, Tlo, (int)NBp, C1, C2, mods, minv, (int)g0, (int)G, Rh, Rl, flag, ntiles_k,
+                       ntiles);
     SDPV_TRY_LAUNCH();
     return SDPV_OK;
 }

@@ -25,6 +25,18 @@ static Wave& my_wave() { return g_waves[threadIdx.x / 64]; }
 
 void emu_wave_barrier() { my_wave().bar->arrive_and_wait(); }
 
+unsigned long long emu_ballot(int pred) {
+    Wave& w = my_wave();
+    int lane = threadIdx.x & 63;
+    w.x[lane] = pred ? 1 : 0;
+    w.bar->arrive_and_wait();
+    unsigned long long m = 0;
+    for (int l = 0; l < 64; l++)
+        if (w.x[l]) m |= 1ull << l;
+    w.bar->arrive_and_wait();
+    return m;
+}
+
 long long emu_shfl_xor64(long long v, int mask) {
     Wave& w = my_wave();
     int lane = threadIdx.x & 63;

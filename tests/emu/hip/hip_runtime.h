@@ -62,6 +62,21 @@ inline double __shfl_xor(double v, int mask, int width = 64) {
     return r;
 }
 void emu_wave_barrier();
+unsigned long long emu_ballot(int pred);
+#define __ballot(p) emu_ballot((int)(p))
+inline unsigned __builtin_amdgcn_perm(unsigned s0, unsigned s1, unsigned sel) {
+    unsigned long long v = ((unsigned long long)s0 << 32) | s1;   // bytes 0-3 from s1, 4-7 from s0
+    unsigned r = 0;
+    for (int b = 0; b < 4; b++) {
+        unsigned q = (sel >> (8 * b)) & 0xffu, byte;
+        if (q < 8) byte = (unsigned)(v >> (8 * q)) & 0xffu;
+        else if (q == 12) byte = 0;
+        else if (q == 13) byte = 0xff;
+        else byte = 0;  // sign-replicating selectors are not used by the kernels
+        r |= byte << (8 * b);
+    }
+    return r;
+}
 #define __builtin_amdgcn_wave_barrier() emu_wave_barrier()
 #define __builtin_amdgcn_fence(order, scope) __atomic_thread_fence(__ATOMIC_SEQ_CST)
 
