@@ -245,17 +245,20 @@ __device__ __forceinline__ i64 entry_top(const i32* X, const i64* e, int L, i64 
     return ex + bitlen_tc(PlaneSrc{X + off, plane}, L);
 }
 
-__global__ void __launch_bounds__(256) k_fill_i64(i64* __restrict__ p, i64 n, i64 v) {
-    const i64 i = blockIdx.x * (i64)256 + threadIdx.x;
-    if (i < n) p[i] = v;
+// top of an entry when its exponent is known and its top limb is neither 0 nor -1 (normalised
+// MPF); otherwise the full scan
+__device__ __forceinline__ i64 entry_top_fast(const i32* X, i64 plane, int L, i64 off, i64 ex, u32 topw) {
+    if (ex == EZV) return EZV;
+    if (topw != 0u && topw != 0xffffffffu) {
+        const u32 x = topw ^ ((topw >> 31) ? 0xffffffffu : 0u);
+        return ex + 32 * (i64)(L - 1) + 32 - clz32(x);
+    }
+    return ex + bitlen_tc(PlaneSrc{X + off, plane}, L);
 }
 
-__device__ __forceinline__ void row_max_commit(i64* Erow, i64 R, i64 mx, int L) {
-    if (mx != EZV) atomicMax((long long*)(Erow + R), (long long)(mx - (32 * (i64)L - 2)));
-}
+__device__ __forceinline__ i64 row_exp_of(i64 mx, int L) { return mx == EZV ? EZV : mx - (32 * (i64)L - 2); }
 
-// rows contiguous in k (ldk == 1): one wave per (row, slice of RK_WAVE entries)
-constexpr int RK_WAVE = 512;
+// rows contiguous in k (ldk == 1): one wave per row, 8 entries per lane in flight
 __global__ void __launch_bounds__(256) k_row_exp_wave(const i32* __restrict__ X, const i64* __restrict__ e, int L,
                                                       i64 plane, i64 nrows, i64 rows, i64 K, i64 ldb, i64 ldr,
                                                       i64* __restrict__ Erow) {
@@ -264,44 +267,64 @@ __global__ void __launch_bounds__(256) k_row_exp_wave(const i32* __restrict__ X,
     if (R >= nrows) return;
     const i64 b = R / rows, r = R % rows;
     const i64 base = b * ldb + r * ldr;
-    const i64 k0 = blockIdx.y * (i64)RK_WAVE, k1 = (k0 + RK_WAVE < K) ? k0 + RK_WAVE : K;
+    const i32* top = X + (i64)(L - 1) * plane;
     i64 mx = EZV;
-#pragma unroll 4
-    for (i64 k = k0 + lane; k < k1; k += 64) {
-        const i64 t = entry_top(X, e, L, plane, base + k);
-        mx = t > mx ? t : mx;
+    for (i64 k0 = 0; k0 < K; k0 += 8 * 64) {
+        i64 ex[8];
+        u32 tw[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+            const i64 k = k0 + u * 64 + lane;
+            ex[u] = k < K ? e[base + k] : EZV;
+            tw[u] = k < K ? (u32)top[base + k] : 0u;
+        }
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+            const i64 t = entry_top_fast(X, plane, L, base + k0 + u * 64 + lane, ex[u], tw[u]);
+            mx = t > mx ? t : mx;
+        }
     }
     for (int o = 32; o >= 1; o >>= 1) {
         const i64 v = __shfl_xor(mx, o, 64);
         mx = v > mx ? v : mx;
     }
-    if (lane == 0) row_max_commit(Erow, R, mx, L);
+    if (lane == 0) Erow[R] = row_exp_of(mx, L);
 }
 
-// other layouts: 64 rows (lanes) x 4 k-groups per block, one slice of RK_COLS entries per block
-constexpr int RK_COLS = 32;
-__global__ void __launch_bounds__(256) k_row_exp_cols(const i32* __restrict__ X, const i64* __restrict__ e, int L,
-                                                      i64 plane, i64 nrows, i64 rows, i64 K, i64 ldb, i64 ldr,
-                                                      i64 ldk, i64* __restrict__ Erow) {
-    __shared__ i64 part[4][64];
+// other layouts: 64 rows (lanes) per block of 16 waves, wave w takes k = w, w + 16, ...;
+// the 16 partial maxima per row meet in LDS
+__global__ void __launch_bounds__(1024) k_row_exp_cols(const i32* __restrict__ X, const i64* __restrict__ e, int L,
+                                                       i64 plane, i64 nrows, i64 rows, i64 K, i64 ldb, i64 ldr,
+                                                       i64 ldk, i64* __restrict__ Erow) {
+    __shared__ i64 part[16][64];
     const int lane = threadIdx.x & 63, kg = threadIdx.x >> 6;
     const i64 R = blockIdx.x * (i64)64 + lane;
-    const i64 k0 = blockIdx.y * (i64)RK_COLS, k1 = (k0 + RK_COLS < K) ? k0 + RK_COLS : K;
     i64 mx = EZV;
     if (R < nrows) {
         const i64 b = R / rows, r = R % rows;
         const i64 base = b * ldb + r * ldr;
-#pragma unroll 4
-        for (i64 k = k0 + kg; k < k1; k += 4) {
-            const i64 t = entry_top(X, e, L, plane, base + k * ldk);
-            mx = t > mx ? t : mx;
+        const i32* top = X + (i64)(L - 1) * plane;
+        for (i64 k0 = kg; k0 < K; k0 += 8 * 16) {
+            i64 ex[8];
+            u32 tw[8];
+#pragma unroll
+            for (int u = 0; u < 8; u++) {
+                const i64 k = k0 + u * 16;
+                ex[u] = k < K ? e[base + k * ldk] : EZV;
+                tw[u] = k < K ? (u32)top[base + k * ldk] : 0u;
+            }
+#pragma unroll
+            for (int u = 0; u < 8; u++) {
+                const i64 t = entry_top_fast(X, plane, L, base + (k0 + u * 16) * ldk, ex[u], tw[u]);
+                mx = t > mx ? t : mx;
+            }
         }
     }
     part[kg][lane] = mx;
     __syncthreads();
     if (kg == 0 && R < nrows) {
-        for (int g = 1; g < 4; g++) mx = part[g][lane] > mx ? part[g][lane] : mx;
-        row_max_commit(Erow, R, mx, L);
+        for (int q = 1; q < 16; q++) mx = part[q][lane] > mx ? part[q][lane] : mx;
+        Erow[R] = row_exp_of(mx, L);
     }
 }
 
@@ -414,19 +437,12 @@ int sdpv_row_exp(const int32_t* X, const int64_t* e, int64_t L, int64_t plane, i
     if (nrows == 0) return SDPV_OK;
     if (!X || !e || !Erow) return SDPV_EARG;
     hipStream_t st = as_stream(stream);
-    hipLaunchKernelGGL(k_fill_i64, dim3(nblocks(nrows, 256)), dim3(256), 0, st, Erow, nrows, (i64)EZV);
-    SDPV_TRY_LAUNCH();
-    if (K == 0) return SDPV_OK;
     if (ldk == 1) {
-        const i64 ns = (K + RK_WAVE - 1) / RK_WAVE;
-        if (ns > 65535) return SDPV_ESIZE;
-        hipLaunchKernelGGL(k_row_exp_wave, dim3(nblocks(nrows, 4), (unsigned)ns), dim3(256), 0, st, X, e, (int)L,
-                           (i64)plane, nrows, (i64)rows, (i64)K, (i64)ldb, (i64)ldr, Erow);
+        hipLaunchKernelGGL(k_row_exp_wave, dim3(nblocks(nrows, 4)), dim3(256), 0, st, X, e, (int)L, (i64)plane, nrows,
+                           (i64)rows, (i64)K, (i64)ldb, (i64)ldr, Erow);
     } else {
-        const i64 ns = (K + RK_COLS - 1) / RK_COLS;
-        if (ns > 65535) return SDPV_ESIZE;
-        hipLaunchKernelGGL(k_row_exp_cols, dim3(nblocks(nrows, 64), (unsigned)ns), dim3(256), 0, st, X, e, (int)L,
-                           (i64)plane, nrows, (i64)rows, (i64)K, (i64)ldb, (i64)ldr, (i64)ldk, Erow);
+        hipLaunchKernelGGL(k_row_exp_cols, dim3(nblocks(nrows, 64)), dim3(1024), 0, st, X, e, (int)L, (i64)plane,
+                           nrows, (i64)rows, (i64)K, (i64)ldb, (i64)ldr, (i64)ldk, Erow);
     }
     SDPV_TRY_LAUNCH();
     return SDPV_OK;

@@ -409,25 +409,23 @@ __global__ void __launch_bounds__(256) k_rns_crt(
             const int g = gb + u;
             a[u] = (g < n && qok) ? (int)acc[(i64)g * ld_g + off] : 0;
         }
-        u32 ph[4] = {0u, 0u, 0u, 0u}, pl[4] = {0u, 0u, 0u, 0u};
+        int hi[16], lo[16];
 #pragma unroll
         for (int u = 0; u < 16; u++) {
             const int g = gb + u;
-            const int m = cm[g];
             const double mi = cmi[g];
-            int y = mod_m(a[u] * cinv[g], m, mi);
-            if (y > (m >> 1)) y -= m;
+            const int x = a[u] * cinv[g];                    // 0 <= x < m^2 < 2^28
+            const int y = x - (int)rint((double)x * mi) * cm[g];   // centred residue, exact
             s += (double)y * mi;
-            const int lo = ((y + 64) & 127) - 64;
-            const int hi = (y - lo) >> 7;
-            ph[u >> 2] |= (u32)(hi & 255) << (8 * (u & 3));
-            pl[u >> 2] |= (u32)(lo & 255) << (8 * (u & 3));
+            const int tt = y + 64;
+            hi[u] = tt >> 7;
+            lo[u] = (tt & 127) - 64;
         }
         v4i vh, vl;
 #pragma unroll
         for (int c = 0; c < 4; c++) {
-            vh[c] = (int)ph[c];
-            vl[c] = (int)pl[c];
+            vh[c] = (int)pack4(hi[4 * c], hi[4 * c + 1], hi[4 * c + 2], hi[4 * c + 3]);
+            vl[c] = (int)pack4(lo[4 * c], lo[4 * c + 1], lo[4 * c + 2], lo[4 * c + 3]);
         }
         *(v4i*)(Yh + (size_t)e * YS + gb) = vh;
         *(v4i*)(Yl + (size_t)e * YS + gb) = vl;
