@@ -144,6 +144,20 @@ class RNS:
         b*ldb + r*ldr + (k0+k)*ldk of every limb plane of X (Lin limbs) and of e, shifted by
         Erow[b*rows + r] - e (none when e is None) into self.L limbs."""
         G = self.n - g0 if G is None else G
+        # host-side extent checks: the kernel trusts these sizes
+        nrows = nb * rows
+        if not (0 <= g0 and g0 + G <= self.n and kp % self.KPAD == 0 and 0 <= kc <= kp):
+            raise ValueError("split: prime range or padded width out of range")
+        for Rx in (Rh, Rl):
+            if tuple(Rx.shape) != (G, kp // 16, nrows, 16) or not Rx.is_contiguous() or Rx.dtype != torch.int8:
+                raise ValueError(f"split: residue planes must be contiguous int8 {(G, kp // 16, nrows, 16)}, "
+                                 f"got {tuple(Rx.shape)}")
+        if nrows and kc:
+            last = (nb - 1) * ldb + (rows - 1) * ldr + (k0 + kc - 1) * ldk
+            if min(ldb, ldr, ldk, k0) < 0 or last >= plane or X.numel() < Lin * plane:
+                raise ValueError("split: entry offsets exceed the limb planes")
+            if e is not None and (e.numel() < plane or Erow is None or Erow.numel() < nrows):
+                raise ValueError("split: exponent arrays too small")
         call("sdpv_rns_split", ptr(X), ptr(e), ptr(Erow), Lin, self.L, plane, nb, rows, ldb, ldr, ldk, k0, kc,
              kp, ptr(self.Thi), ptr(self.Tlo), self.NBp, ptr(self.C1), ptr(self.C2), ptr(self.m32),
              ptr(self.minv), g0, G, ptr(Rh), ptr(Rl), ptr(flag), stream())
@@ -151,7 +165,15 @@ class RNS:
     # ---- step 2: accumulate residues of A @ Bt^T
     def product(self, Ah, Al, Bh, Bl, acc, nb, M, N, ld_g, g0=0, accumulate=False, syrk=False):
         """residues in the kernels' chunk-major layout (G, kp/16, rows, 16)"""
+        if Ah.dim() != 4:
+            raise ValueError("product: residues must be in the chunk-major layout (G, kp/16, rows, 16)")
         G, kp = Ah.shape[0], Ah.shape[1] * 16
+        for X, rows in ((Ah, nb * M), (Al, nb * M), (Bh, nb * N), (Bl, nb * N)):
+            if tuple(X.shape) != (G, kp // 16, rows, 16) or not X.is_contiguous():
+                raise ValueError(f"product: residues must be contiguous {(G, kp // 16, rows, 16)}, "
+                                 f"got {tuple(X.shape)}")
+        if g0 < 0 or g0 + G > self.n or ld_g < nb * M * N or acc.numel() < (g0 + G - 1) * ld_g + nb * M * N:
+            raise ValueError("product: accumulator too small for the primes and output block")
         mode = (1 if accumulate else 0) | (2 if syrk else 0)
         call("sdpv_rns_gemm", ptr(Ah), ptr(Al), ptr(Bh), ptr(Bl), G, nb, M, N, kp, ptr(acc), ld_g,
              ptr(self.m32), ptr(self.minv), ptr(self.kcoef), g0, mode, stream())
@@ -169,6 +191,14 @@ class RNS:
     def crt(self, acc, rows, N, j0, w, mode, ld_g=None, W=None, bitlen=None, outm=None, oute=None, L=None,
             plane=0, ldo=0, jo=0, Mb=1, erow=None, ecol=None, scol=None, flag=None):
         ld_g = acc.stride(0) if ld_g is None else ld_g
+        if acc.numel() < (self.n - 1) * ld_g + (rows - 1) * N + j0 + w or j0 + w > N:
+            raise ValueError("crt: block exceeds the residue accumulator")
+        Bq = rows * w
+        if mode == 0 and (W is None or W.numel() < (self.E // 4) * Bq or bitlen is None or bitlen.numel() < Bq):
+            raise ValueError("crt: word outputs too small")
+        if mode != 0 and Bq and (outm is None or outm.numel() < ((L or self.L) - 1) * plane + (rows - 1) * ldo
+                                 + jo + w):
+            raise ValueError("crt: limb output too small")
         call("sdpv_rns_crt", ptr(acc), self.n, ld_g, rows, N, j0, w, ptr(self.m32), ptr(self.minv),
              ptr(self.inv32), ptr(self.Ct), ptr(self.Md), self.E, self.npad, mode, ptr(W), ptr(bitlen),
              ptr(outm), ptr(oute), L or self.L, plane, ldo, jo, Mb, ptr(erow), ptr(ecol), ptr(scol),
