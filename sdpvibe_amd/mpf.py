@@ -80,11 +80,23 @@ class MPF:
         return out.reshape(self.shape)
 
     def to_float(self):
-        """approximate float64 values (with exponent clamping); host numpy (mpf.py:210-215)"""
+        """approximate float64 values: the top two limbs times 2^(e + 32(L-2)), the exponent
+        clamped to [-1100, 1100]; host numpy (mpf.py:210-215). The power of two is built exactly
+        from its bit pattern (a generic pow may round for subnormal results)."""
         top = self.m[-1].double() * 2.0 ** 32 + (self.m[-2].double() % 2.0 ** 32) if self.L > 1 else self.m[-1].double()
-        sc = (self.e + 32 * (self.L - 2)).clamp(-1100, 1100).double()
-        v = torch.where(self.e == EZ, torch.zeros_like(top), top * torch.pow(2.0, sc))
+        sc = (self.e + 32 * (self.L - 2)).clamp(-1100, 1100)
+        v = torch.where(self.e == EZ, torch.zeros_like(top), top * _pow2(sc))
         return v.cpu().numpy()
+
+
+def _pow2(k):
+    """exact float64 2^k for an int64 tensor k: normal range from the exponent field, subnormals
+    from the mantissa field, 0 below 2^-1074 and inf above 2^1023"""
+    normal = ((k.clamp(-1022, 1023) + 1023) << 52).view(torch.float64)
+    sub = (torch.ones_like(k) << (k.clamp(-1074, -1023) + 1074)).view(torch.float64)
+    v = torch.where(k >= -1022, normal, sub)
+    v = torch.where(k < -1074, torch.zeros_like(v), v)
+    return torch.where(k > 1023, torch.full_like(v, float("inf")), v)
 
 
 def zeros(shape, L):
