@@ -91,6 +91,19 @@ def main():
     dt = (time.perf_counter() - t0) / args.steps
     del C
 
+    # ---- the same step replayed from a HIP graph (one capture, no host work per step)
+    from sdpvibe_amd.graph import Captured
+    cap = Captured(F.gemm, A, B)
+    for _ in range(args.warmup):
+        cap.replay()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        cap.replay()
+    torch.cuda.synchronize()
+    dt_graph = (time.perf_counter() - t0) / args.steps
+    del cap
+
     # ---- per-phase times with HIP events on the launch stream (torch's current stream)
     st = torch.cuda.current_stream()
     nb = 1
@@ -158,6 +171,8 @@ def main():
         "data": "synthetic (host PCG64 Gaussian mantissas, exponents in [-16, 16], seed 0/1)",
         "config": {"workload": f"mpf.gemm {M}x{K}x{N} at {32 * L}-bit (L={L}), plain-kernel configuration",
                    "L": L, "primes": nP, "crt_digits": E},
+        "ms_per_step_graph": dt_graph * 1e3,
+        "pct_peak_graph": round(100.0 * ref_ops / dt_graph / (PEAK_INT8_TOPS * 1e12), 3),
         "mp_mac_per_s": M * N * K / dt, "effective_int8_ops_per_step": ref_ops,
         "executed_int8_ops_per_step": ops_total,
         "pct_peak_executed_ops": round(100.0 * ops_total / dt / (PEAK_INT8_TOPS * 1e12), 3),
