@@ -105,7 +105,8 @@ int sdpv_row_exp(const int32_t* X, const int64_t* e, int64_t L, int64_t plane, i
  * (as sdpv_shift_round) by s = Erow[b*rows + r] - e[entry] (s = 2^30 when e == SDPV_EZ;
  * s = 0 for every entry when e == NULL) from L to Lout limbs, then reduced modulo the primes
  * g0 .. g0+G-1 to the centred residue r = 128*hi + lo (lo in [-64, 63]).
- * Rh, Rl: (G, nb*rows, kp) int8; columns kc <= k < kp hold residue 0.
+ * Rh, Rl: (G, kp/16, nb*rows, 16) int8, chunk-major: byte k of row r of prime g at
+ * ((g*(kp/16) + k/16)*nb*rows + r)*16 + k%16; columns kc <= k < kp hold residue 0.
  * Tables (built by the caller for Lout): Thi/Tlo (npad, NBp) int8 with NBp = 32*ceil(4 Lout/32),
  * C1, C2, mods (n,) int32, minv (n,) double = 1/m.
  * flag: optional int32 set to 1 when the shifted value overflows Lout limbs.
@@ -125,14 +126,18 @@ int sdpv_rns_split(const int32_t* X, const int64_t* e, const int64_t* Erow, int6
  * b = 128*Bh + Bl the centred residues,
  *   acc[(g0+gi)*ld_g + (b*M + i)*N + j] = (sum_k a[gi, b*M+i, k] * b[gi, b*N+j, k]) mod m_(g0+gi)
  * stored as int16 in [0, m).
- * Ah/Al: (G, nb*M, kp), Bh/Bl: (G, nb*N, kp) int8 with kp a multiple of 64, kp <= 65536.
+ * Ah/Al: (G, kp/16, nb*M, 16), Bh/Bl: (G, kp/16, nb*N, 16) int8, chunk-major as written by
+ * sdpv_rns_split, kp a multiple of 64, kp <= 65536.
  * kcoef: (n, 3) int32 = (16256, 128, -127) mod m: ra*rb = 16256*HH + 128*SS - 127*LL.
  * Replaces RNS.accumulate / _gemm_kernel (sdpvibe/rns.py:217-224, :107-137),
  * _bgemm_kernel (sdpvibe/batched.py:97-127) and _syrk_lower_kernel (sdpvibe/batched.py:131-162). */
+/* config: -1 picks the tile by size (and is required with SDPV_GEMM_SYRK_LOWER); 0..5 select
+ * 128x128 (2 stages), 128x128 (3 stages), 64x64, 32x32, 256x128 and 128x256 tiles. Every
+ * configuration computes the same residues (an autotuner picks by time). */
 int sdpv_rns_gemm(const int8_t* Ah, const int8_t* Al, const int8_t* Bh, const int8_t* Bl,
                   int64_t G, int64_t nb, int64_t M, int64_t N, int64_t kp, int16_t* acc,
                   int64_t ld_g, const int32_t* mods, const double* minv, const int32_t* kcoef,
-                  int64_t g0, int32_t mode, void* stream);
+                  int64_t g0, int32_t mode, int32_t config, void* stream);
 
 #define SDPV_CRT_WORDS 0     /* W (E/4, B) int32 two's complement + bitlen (B,) */
 #define SDPV_CRT_NORMALIZE 1 /* MPF output: L limbs with NB bits, exponent ein + shift */

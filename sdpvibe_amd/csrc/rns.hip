@@ -181,7 +181,7 @@ __global__ void __launch_bounds__(256) k_rns_split(
 // 32 consecutive rows are contiguous and conflict-free. The s = h + l operand is formed in
 // registers (bytewise add) from the h and l fragments.
 template <int WM, int WN, int BLM, int BLN, int NST>
-__global__ void __launch_bounds__(WM* WN * 64, 2) k_rns_gemm(
+__global__ void __launch_bounds__(WM* WN * 64, (WM * WN <= 4) ? 2 : 1) k_rns_gemm(
     const int8_t* __restrict__ Ah, const int8_t* __restrict__ Al, const int8_t* __restrict__ Bh,
     const int8_t* __restrict__ Bl, int G, int nb, i64 M, i64 N, i64 kp, int16_t* __restrict__ acc, i64 ld_g,
     const i32* __restrict__ mods, const double* __restrict__ minv, const i32* __restrict__ kcoef, int g0, int mode,
@@ -578,23 +578,28 @@ int sdpv_rns_split(const int32_t* X, const int64_t* e, const int64_t* Erow, int6
 
 int sdpv_rns_gemm(const int8_t* Ah, const int8_t* Al, const int8_t* Bh, const int8_t* Bl, int64_t G, int64_t nb,
                   int64_t M, int64_t N, int64_t kp, int16_t* acc, int64_t ld_g, const int32_t* mods,
-                  const double* minv, const int32_t* kcoef, int64_t g0, int32_t mode, void* stream) {
+                  const double* minv, const int32_t* kcoef, int64_t g0, int32_t mode, int32_t config, void* stream) {
     if (G < 0 || nb < 0 || M < 0 || N < 0 || kp < 0 || g0 < 0 || (mode & ~3)) return SDPV_EARG;
+    if (config < -1 || config > 5) return SDPV_EARG;
     if (kp % 64 != 0 || kp > 65536) return SDPV_ESIZE;
-    if ((mode & SDPV_GEMM_SYRK_LOWER) && M != N) return SDPV_EARG;
+    if ((mode & SDPV_GEMM_SYRK_LOWER) && (M != N || config != -1)) return SDPV_EARG;
     if (G == 0 || nb == 0 || M == 0 || N == 0) return SDPV_OK;
     if (!Ah || !Al || !Bh || !Bl || !acc || !mods || !minv || !kcoef) return SDPV_EARG;
     hipStream_t st = as_stream(stream);
     if (kp == 0) return SDPV_EARG;
-    const i64 mn = M < N ? M : N;
-    if (mn >= 128)
-        return launch_gemm<2, 2, 2, 2, 2>(Ah, Al, Bh, Bl, (int)G, (int)nb, M, N, kp, acc, ld_g, mods, minv, kcoef,
-                                       (int)g0, mode, st);
-    if (mn >= 64)
-        return launch_gemm<2, 2, 1, 1, 2>(Ah, Al, Bh, Bl, (int)G, (int)nb, M, N, kp, acc, ld_g, mods, minv, kcoef,
-                                       (int)g0, mode, st);
-    return launch_gemm<1, 1, 1, 1, 2>(Ah, Al, Bh, Bl, (int)G, (int)nb, M, N, kp, acc, ld_g, mods, minv, kcoef, (int)g0,
-                                   mode, st);
+    if (config == -1) {
+        const i64 mn = M < N ? M : N;  // syrk: square tiles of 128 / 64 / 32 by N, as the reference
+        config = mn >= 128 ? 0 : (mn >= 64 ? 2 : 3);
+    }
+    const int g = (int)G, b = (int)nb, q0 = (int)g0;
+    switch (config) {
+        case 0: return launch_gemm<2, 2, 2, 2, 2>(Ah, Al, Bh, Bl, g, b, M, N, kp, acc, ld_g, mods, minv, kcoef, q0, mode, st);
+        case 1: return launch_gemm<2, 2, 2, 2, 3>(Ah, Al, Bh, Bl, g, b, M, N, kp, acc, ld_g, mods, minv, kcoef, q0, mode, st);
+        case 2: return launch_gemm<2, 2, 1, 1, 2>(Ah, Al, Bh, Bl, g, b, M, N, kp, acc, ld_g, mods, minv, kcoef, q0, mode, st);
+        case 3: return launch_gemm<1, 1, 1, 1, 2>(Ah, Al, Bh, Bl, g, b, M, N, kp, acc, ld_g, mods, minv, kcoef, q0, mode, st);
+        case 4: return launch_gemm<4, 2, 2, 2, 2>(Ah, Al, Bh, Bl, g, b, M, N, kp, acc, ld_g, mods, minv, kcoef, q0, mode, st);
+        default: return launch_gemm<2, 4, 2, 2, 2>(Ah, Al, Bh, Bl, g, b, M, N, kp, acc, ld_g, mods, minv, kcoef, q0, mode, st);
+    }
 }
 
 int sdpv_rns_crt(const int16_t* acc, int64_t n, int64_t ld_g, int64_t rows, int64_t N, int64_t j0, int64_t w,

@@ -16,6 +16,7 @@ prime costs three INT8 products; the reconstructed integers, and hence every rou
 identical.
 """
 import math
+import os
 
 import numpy as np
 import torch
@@ -59,6 +60,56 @@ def gemm_config(M, N, K):
     if mn >= 64:
         return 64, 64
     return 32, 32
+
+
+# ------------------------------------------------------------------ product tile autotuner
+GEMM_CONFIGS = (0, 1, 2, 3, 4, 5)      # see sdpv_rns_gemm: 128^2 x2 stages, 128^2 x3, 64^2, 32^2, 256x128, 128x256
+_tuned = {}
+TUNE = os.environ.get("SDPV_TUNE", "1") != "0"
+
+
+def _bucket(x):
+    """shape class: next power of two (tile choice depends on the size class only)"""
+    return 1 << max(0, int(x - 1).bit_length())
+
+
+def tuned_config(R, nb, M, N, kp):
+    """fastest product tile configuration for this shape class, measured once per process on
+    scratch residues (every configuration computes identical residues); -1 (size rule) when
+    tuning is off or the product is small"""
+    if not TUNE or nb * M * N * kp < (1 << 24):
+        return -1
+    key = (R.L, _bucket(nb), _bucket(M), _bucket(N), _bucket(kp))
+    if key not in _tuned:
+        _tuned[key] = _measure(R, nb, M, N, kp)
+    return _tuned[key]
+
+
+def _measure(R, nb, M, N, kp, reps=3):
+    G = min(R.n, 32)
+    dev_ = R.m32.device
+    gen = torch.Generator(device=dev_)
+    gen.manual_seed(0)
+    mk = lambda rows: torch.randint(-64, 64, (G, kp // 16, nb * rows, 16), dtype=torch.int8, device=dev_,
+                                    generator=gen)
+    Ah, Al, Bh, Bl = mk(M), mk(M), mk(N), mk(N)
+    acc = torch.empty((G, nb * M, N), dtype=torch.int16, device=dev_)
+    best, best_t = -1, None
+    for cfg in GEMM_CONFIGS:
+        st = torch.cuda.current_stream()
+        call("sdpv_rns_gemm", ptr(Ah), ptr(Al), ptr(Bh), ptr(Bl), G, nb, M, N, kp, ptr(acc), nb * M * N,
+             ptr(R.m32), ptr(R.minv), ptr(R.kcoef), 0, 0, cfg, stream())          # warm (attributes)
+        ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        ev0.record(st)
+        for _ in range(reps):
+            call("sdpv_rns_gemm", ptr(Ah), ptr(Al), ptr(Bh), ptr(Bl), G, nb, M, N, kp, ptr(acc), nb * M * N,
+                 ptr(R.m32), ptr(R.minv), ptr(R.kcoef), 0, 0, cfg, stream())
+        ev1.record(st)
+        ev1.synchronize()
+        t = ev0.elapsed_time(ev1)
+        if best_t is None or t < best_t:
+            best, best_t = cfg, t
+    return best
 
 
 def row_major(R):
@@ -163,7 +214,7 @@ class RNS:
              ptr(self.minv), g0, G, ptr(Rh), ptr(Rl), ptr(flag), stream())
 
     # ---- step 2: accumulate residues of A @ Bt^T
-    def product(self, Ah, Al, Bh, Bl, acc, nb, M, N, ld_g, g0=0, accumulate=False, syrk=False):
+    def product(self, Ah, Al, Bh, Bl, acc, nb, M, N, ld_g, g0=0, accumulate=False, syrk=False, config=None):
         """residues in the kernels' chunk-major layout (G, kp/16, rows, 16)"""
         if Ah.dim() != 4:
             raise ValueError("product: residues must be in the chunk-major layout (G, kp/16, rows, 16)")
@@ -175,8 +226,10 @@ class RNS:
         if g0 < 0 or g0 + G > self.n or ld_g < nb * M * N or acc.numel() < (g0 + G - 1) * ld_g + nb * M * N:
             raise ValueError("product: accumulator too small for the primes and output block")
         mode = (1 if accumulate else 0) | (2 if syrk else 0)
+        if config is None:
+            config = -1 if syrk else tuned_config(self, nb, M, N, kp)
         call("sdpv_rns_gemm", ptr(Ah), ptr(Al), ptr(Bh), ptr(Bl), G, nb, M, N, kp, ptr(acc), ld_g,
-             ptr(self.m32), ptr(self.minv), ptr(self.kcoef), g0, mode, stream())
+             ptr(self.m32), ptr(self.minv), ptr(self.kcoef), g0, mode, config, stream())
 
     def accumulate(self, Ahi, Alo, Bhi, Blo, acc, g0=0):
         """acc[g0:g0+G] += residues of A @ B^T for the G moduli whose residues are given in the

@@ -206,6 +206,16 @@ def eig_cases():
 CASES["eig"] = eig_cases
 
 
+def product_cases():
+    from tests.test_gpu_kernels import product_configs_case
+    product_configs_case(lambda a: torch.from_numpy(np.ascontiguousarray(a)), M=70, N=40, K=65, nb=2, L=2)
+    print("product configs: ok", flush=True)
+    return True
+
+
+CASES["product"] = product_cases
+
+
 if __name__ == "__main__":  # noqa: E305
     ok = True
     for c in sys.argv[1:] or list(CASES):

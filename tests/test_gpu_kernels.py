@@ -208,6 +208,44 @@ def test_product_accumulate():
         _eq(host(acc).astype(np.int64), ref, f"product {M}x{N}x{K}")
 
 
+def product_configs_case(dev_put, M=300, N=200, K=130, nb=2, L=10):
+    """every tile configuration of the product (the autotuner's choices) gives the same residues,
+    equal to numpy int64; the syrk mode too on its lower tiles"""
+    from sdpvibe_amd.rns import GEMM_CONFIGS, chunk_major
+    R = mpmat.ctx(L)
+    rng = np.random.default_rng(11)
+    G = R.n
+    m = np.array(R.mods, np.int64)[:, None, None, None]
+    ra = rng.integers(-8128, 8129, (G, nb, M, K))
+    rb = rng.integers(-8128, 8129, (G, nb, N, K))
+    ra = np.where(np.abs(ra) <= (m - 1) // 2, ra, 0)
+    rb = np.where(np.abs(rb) <= (m - 1) // 2, rb, 0)
+    kp = R.kp(K)
+    pad = lambda r: np.pad(r, ((0, 0), (0, 0), (0, 0), (0, kp - K))).reshape(G, -1, kp)
+    sp = lambda r: (((r + 64) & 127) - 64, (r - (((r + 64) & 127) - 64)) >> 7)
+    la, ha = sp(pad(ra))
+    lb, hb = sp(pad(rb))
+    A = [chunk_major(dev_put(x.astype(np.int8))) for x in (ha, la)]
+    B = [chunk_major(dev_put(x.astype(np.int8))) for x in (hb, lb)]
+    ref = np.einsum("gbik,gbjk->gbij", ra, rb) % m
+    for cfg in GEMM_CONFIGS:
+        acc = dev_put(np.zeros((G, nb * M, N), np.int16))
+        R.product(A[0], A[1], B[0], B[1], acc, nb, M, N, nb * M * N, config=cfg)
+        _eq(acc.cpu().numpy().astype(np.int64).reshape(G, nb, M, N), ref, f"config {cfg}")
+    # syrk: A = B, lower tiles (tile size by N as the reference: 128 here)
+    acc = dev_put(np.zeros((G, M, M), np.int16))
+    A1 = [chunk_major(dev_put(x.astype(np.int8)[:, :M])) for x in (ha, la)]
+    R.product(A1[0], A1[1], A1[0], A1[1], acc, 1, M, M, M * M, accumulate=True, syrk=True)
+    full = np.einsum("gik,gjk->gij", ra[:, 0], ra[:, 0]) % m[:, 0]
+    got = acc.cpu().numpy().astype(np.int64)
+    low = np.tril(np.ones((M, M), bool))
+    _eq(np.where(low, got, 0), np.where(low, full, 0), "syrk lower")
+
+
+def test_product_configs():
+    product_configs_case(cu)
+
+
 def _gemm_case(L, nb, M, K, N, transA, transB, seed):
     rng = random.Random(seed)
     shpA = (nb, K, M) if transA else (nb, M, K)
