@@ -241,3 +241,14 @@ def step_from_eig(lam, gamma):
 def step_length(chols, dMs, gamma):
     return step_from_eig(min_eig(chols, dMs), gamma)
 
+
+
+def _rand_mpf(shape, L, gen, spread=4):
+    """random normalised MPF entries of magnitude ~2^[-spread, spread] (solver.py:401-406): random
+    limbs from the generator, normalised on the GPU"""
+    n = 1
+    for s in shape:
+        n *= s
+    m = torch.randint(-2 ** 31, 2 ** 31, (L, n), dtype=torch.int32, device=gen.device, generator=gen)
+    e = torch.randint(-spread, spread + 1, (n,), dtype=torch.int64, device=gen.device, generator=gen) - 32 * L
+    return F.normalize(m, e, L).reshape(*shape)
