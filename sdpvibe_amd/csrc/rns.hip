@@ -72,11 +72,13 @@ __global__ void __launch_bounds__(256) k_rns_split(
     int8_t* __restrict__ Rl, i32* __restrict__ flag, i64 ntiles_k, i64 ntiles) {
     SDPV_DYNAMIC_LDS(unsigned char, smem);
     const int XS = NBp + 16;                            // bytes per slot row (multiple of 16)
+    const int RS = L + 1;                               // words per raw-limb row (odd stride when L even)
     const int t = threadIdx.x, lane = t & 63;
     const int wave = __builtin_amdgcn_readfirstlane(t >> 6);
-    const i64 tile = (i64)blockIdx.x * 4 + wave;
+    const i64 tile = (i64)blockIdx.x * (blockDim.x >> 6) + wave;
     if (tile >= ntiles) return;                         // whole wave
-    unsigned char* xs = smem + (size_t)wave * 64 * XS;
+    unsigned char* xs = smem + (size_t)wave * 64 * (XS + 4 * RS);
+    u32* raw = (u32*)(xs + (size_t)64 * XS) + (size_t)lane * RS;
     const bool kmajor = ldk == 1;                       // 4 rows x 16 k, else 16 rows x 4 k
     const i64 tr = tile / ntiles_k, tk = tile % ntiles_k;
     const i64 row0 = kmajor ? tr * 4 : tr * 16;
@@ -90,15 +92,27 @@ __global__ void __launch_bounds__(256) k_rns_split(
     {
         const i64 R = slot_row(lane), k = slot_k(lane);
         u32* row = (u32*)(xs + (size_t)lane * XS);
-        if (R < nrows && k < kc) {
-            const i64 b = R / rows, r = R % rows;
-            const i64 off = b * ldb + r * ldr + (k0 + k) * ldk;
-            i64 s = 0;
-            if (e) {
-                const i64 ex = e[off];
-                s = (ex == EZV) ? (i64)(1 << 30) : Erow[R] - ex;
+        const bool valid = R < nrows && k < kc;
+        const i64 Rc = valid ? R : 0;
+        const i64 off = valid ? (Rc / rows) * ldb + (Rc % rows) * ldr + (k0 + k) * ldk : 0;
+        // the entry's limbs and exponent with independent loads (8 in flight), into LDS
+        for (int w0 = 0; w0 < L; w0 += 8) {
+            u32 v[8];
+#pragma unroll
+            for (int u = 0; u < 8; u++) {
+                const int w = w0 + u < L ? w0 + u : L - 1;
+                v[u] = (u32)X[off + (i64)w * plane];
             }
-            const int bad = shr_round(PlaneSrc{X + off, plane}, L, s, Lout, [&](int wi, u32 v) {
+#pragma unroll
+            for (int u = 0; u < 8; u++)
+                if (w0 + u < L) raw[w0 + u] = v[u];
+        }
+        const i64 ex = e ? e[off] : 0;
+        const i64 er = e ? Erow[Rc] : 0;
+        if (valid) {
+            i64 s = 0;
+            if (e) s = (ex == EZV) ? (i64)(1 << 30) : er - ex;
+            const int bad = shr_round(LdsStrided{raw, 1}, L, s, Lout, [&](int wi, u32 v) {
                 row[wi] = v ^ 0x80808080u;
                 if (wi == Lout - 1) neg = (int)(v >> 31);
             });
@@ -405,10 +419,13 @@ __global__ void __launch_bounds__(256) k_rns_crt(
         const int gb = half * hp + gc;
         int a[16];
 #pragma unroll
-        for (int u = 0; u < 16; u++) {
+        for (int u = 0; u < 16; u++) {                  // independent loads (clamped prime, valid address)
             const int g = gb + u;
-            a[u] = (g < n && qok) ? (int)acc[(i64)g * ld_g + off] : 0;
+            const int gg = g < n ? g : n - 1;
+            a[u] = (int)acc[(i64)gg * ld_g + off];
         }
+#pragma unroll
+        for (int u = 0; u < 16; u++) a[u] = (gb + u < n && qok) ? a[u] : 0;
         int hi[16], lo[16];
 #pragma unroll
         for (int u = 0; u < 16; u++) {
@@ -559,16 +576,19 @@ int sdpv_rns_split(const int32_t* X, const int64_t* e, const int64_t* Erow, int6
     const i64 ntiles_k = kmajor ? kp / 16 : kp / 4;
     const i64 ntiles_r = kmajor ? (nrows + 3) / 4 : (nrows + 15) / 16;
     const i64 ntiles = ntiles_r * ntiles_k;
-    const i64 grid = (ntiles + 3) / 4;
-    if (grid > 0x7fffffffLL) return SDPV_ESIZE;
-    const size_t lds = (size_t)4 * 64 * (NBp + 16);
+    const size_t wbytes = (size_t)64 * (NBp + 16) + (size_t)64 * 4 * (L + 1);
+    int nwave = 4;
+    while (nwave > 1 && nwave * wbytes > 160 * 1024) nwave >>= 1;
+    const size_t lds = nwave * wbytes;
     if (lds > 160 * 1024) return SDPV_ESIZE;
+    const i64 grid = (ntiles + nwave - 1) / nwave;
+    if (grid > 0x7fffffffLL) return SDPV_ESIZE;
     static bool attr_set = false;
     if (!attr_set) {
         (void)hipFuncSetAttribute((const void*)k_rns_split, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
         attr_set = true;
     }
-    hipLaunchKernelGGL(k_rns_split, dim3((unsigned)grid), dim3(256), lds, as_stream(stream), X, e, Erow, (int)L,
+    hipLaunchKernelGGL(k_rns_split, dim3((unsigned)grid), dim3(64 * nwave), lds, as_stream(stream), X, e, Erow, (int)L,
                        (int)Lout, (i64)plane, (i64)nb, (i64)rows, (i64)ldb, (i64)ldr, (i64)ldk, (i64)k0, (i64)kc,
                        (i64)kp, Thi, Tlo, (int)NBp, C1, C2, mods, minv, (int)g0, (int)G, Rh, Rl, flag, ntiles_k,
                        ntiles);
