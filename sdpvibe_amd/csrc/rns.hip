@@ -43,15 +43,19 @@ __device__ __forceinline__ v16i mfma(v4i a, v4i b, v16i c) {
 }
 
 // ================================================================== residue split
-// One wavefront splits 64 entries on its own (no workgroup barriers): a tile of 4 rows x 16
-// consecutive k (k contiguous in memory, ldk == 1) or 16 rows x 4 consecutive k (otherwise).
-// Phase 1: lane s streams its entry's limbs through the rounding shift into Lout words, stored as
-//   bytes u - 128 in the wave's LDS rows (slot s); the entry's sign goes into a wave ballot.
-// Phase 2, per block of 32 primes: the byte dot products x . T_hi and x . T_lo run on MFMA with the
-//   64 slots as rows (two 32-row blocks) and the primes as columns, so each lane ends up with four
-//   runs of 4 slots that are 4 consecutive k of one row; r = 128 T_hi.x + T_lo.x + C1 - neg C2 is
-//   reduced to the centred residue (double reciprocal, exact for |r| < 2^30), split into
-//   128 hi + lo and written as packed 4-byte runs straight into the chunk-major residue planes.
+// One workgroup (4 waves) splits a tile of 16 rows x 16 consecutive k (one 16-byte chunk of every
+// row of the residue planes).
+// Phase 1: thread t takes one entry, with the lanes along the contiguous index (k when ldk == 1,
+//   rows otherwise) so the limb loads coalesce; the limbs are staged in LDS with independent
+//   loads, shift-rounded to the row exponent into Lout words and written as bytes u - 128 into
+//   the LDS slot of the wave that owns the entry's row in phase 2 (wave w: rows 4w..4w+3, slot
+//   (r % 4) * 16 + k); the sign goes into a byte array.
+// Phase 2, per wave and block of 32 primes: the byte dot products x . T_hi and x . T_lo run on MFMA
+//   with the wave's 64 slots as rows and the primes as columns, so each lane holds four runs of 4
+//   consecutive k of one row; r = 128 T_hi.x + T_lo.x + C1 - neg C2 is reduced to the centred
+//   residue (nearest quotient in double, exact for |r| < 2^30) and split into 128 hi + lo. The two
+//   lane halves swap 4-byte runs so that each lane holds whole 16-byte chunks (one row, one prime)
+//   and stores them with 16-byte writes: 64 contiguous bytes per prime and wave.
 struct LdsStrided {
     const u32* p;
     int stride;
@@ -69,63 +73,66 @@ __global__ void __launch_bounds__(256) k_rns_split(
     i64 nb, i64 rows, i64 ldb, i64 ldr, i64 ldk, i64 k0, i64 kc, i64 kp, const int8_t* __restrict__ Thi,
     const int8_t* __restrict__ Tlo, int NBp, const i32* __restrict__ C1, const i32* __restrict__ C2,
     const i32* __restrict__ mods, const double* __restrict__ minv, int g0, int G, int8_t* __restrict__ Rh,
-    int8_t* __restrict__ Rl, i32* __restrict__ flag, i64 ntiles_k, i64 ntiles) {
+    int8_t* __restrict__ Rl, i32* __restrict__ flag, i64 ntiles_k, int stage) {
     SDPV_DYNAMIC_LDS(unsigned char, smem);
     const int XS = NBp + 16;                            // bytes per slot row (multiple of 16)
-    const int RS = L + 1;                               // words per raw-limb row (odd stride when L even)
+    const int RS = L + 1;                               // words per staged-limb row
     const int t = threadIdx.x, lane = t & 63;
     const int wave = __builtin_amdgcn_readfirstlane(t >> 6);
-    const i64 tile = (i64)blockIdx.x * (blockDim.x >> 6) + wave;
-    if (tile >= ntiles) return;                         // whole wave
-    unsigned char* xs = smem + (size_t)wave * 64 * (XS + 4 * RS);
-    u32* raw = (u32*)(xs + (size_t)64 * XS) + (size_t)lane * RS;
-    const bool kmajor = ldk == 1;                       // 4 rows x 16 k, else 16 rows x 4 k
-    const i64 tr = tile / ntiles_k, tk = tile % ntiles_k;
-    const i64 row0 = kmajor ? tr * 4 : tr * 16;
-    const i64 kk0 = kmajor ? tk * 16 : tk * 4;
+    unsigned char* xs = smem;                           // [256 slots][XS]
+    unsigned char* negs = xs + (size_t)256 * XS;        // [256]
+    u32* raw = (u32*)(negs + 256) + (size_t)t * RS;     // [256 threads][RS] when staging
+    const i64 tr = blockIdx.x / ntiles_k, tk = blockIdx.x % ntiles_k;
+    const i64 row0 = tr * 16, kk0 = tk * 16;
     const i64 nrows = nb * rows;
-    auto slot_row = [&](int s) { return row0 + (kmajor ? (s >> 4) : (s >> 2)); };
-    auto slot_k = [&](int s) { return kk0 + (kmajor ? (s & 15) : (s & 3)); };
 
-    // ---- phase 1: slot s = lane
-    int neg = 0;
+    // ---- phase 1: thread t -> entry (r, k) of the tile
     {
-        const i64 R = slot_row(lane), k = slot_k(lane);
-        u32* row = (u32*)(xs + (size_t)lane * XS);
-        const bool valid = R < nrows && k < kc;
+        const bool kmajor = ldk == 1;
+        const int r = kmajor ? (t >> 4) : (t & 15), k = kmajor ? (t & 15) : (t >> 4);
+        const int S = r * 16 + k;                       // phase-2 slot: wave S / 64, row S % 64
+        u32* row = (u32*)(xs + (size_t)S * XS);
+        const i64 R = row0 + r, kg = kk0 + k;
+        const bool valid = R < nrows && kg < kc;
         const i64 Rc = valid ? R : 0;
-        const i64 off = valid ? (Rc / rows) * ldb + (Rc % rows) * ldr + (k0 + k) * ldk : 0;
-        // the entry's limbs and exponent with independent loads (8 in flight), into LDS
-        for (int w0 = 0; w0 < L; w0 += 8) {
-            u32 v[8];
+        const i64 off = valid ? (Rc / rows) * ldb + (Rc % rows) * ldr + (k0 + kg) * ldk : 0;
+        if (stage) {
+            for (int w0 = 0; w0 < L; w0 += 8) {          // independent loads, 8 in flight
+                u32 v[8];
 #pragma unroll
-            for (int u = 0; u < 8; u++) {
-                const int w = w0 + u < L ? w0 + u : L - 1;
-                v[u] = (u32)X[off + (i64)w * plane];
+                for (int u = 0; u < 8; u++) {
+                    const int w = w0 + u < L ? w0 + u : L - 1;
+                    v[u] = (u32)X[off + (i64)w * plane];
+                }
+#pragma unroll
+                for (int u = 0; u < 8; u++)
+                    if (w0 + u < L) raw[w0 + u] = v[u];
             }
-#pragma unroll
-            for (int u = 0; u < 8; u++)
-                if (w0 + u < L) raw[w0 + u] = v[u];
         }
         const i64 ex = e ? e[off] : 0;
         const i64 er = e ? Erow[Rc] : 0;
+        int neg = 0;
         if (valid) {
             i64 s = 0;
             if (e) s = (ex == EZV) ? (i64)(1 << 30) : er - ex;
-            const int bad = shr_round(LdsStrided{raw, 1}, L, s, Lout, [&](int wi, u32 v) {
+            auto put = [&](int wi, u32 v) {
                 row[wi] = v ^ 0x80808080u;
                 if (wi == Lout - 1) neg = (int)(v >> 31);
-            });
+            };
+            const int bad = stage ? shr_round(LdsStrided{raw, 1}, L, s, Lout, put)
+                                  : shr_round(PlaneSrc{X + off, plane}, L, s, Lout, put);
             if (bad && flag) atomicOr(flag, 1);
         } else {
             for (int w = 0; w < Lout; w++) row[w] = 0x80808080u;   // zero entry
         }
         for (int w = Lout; w < NBp / 4; w++) row[w] = 0x80808080u;
+        negs[S] = (unsigned char)neg;
     }
-    const u64 negmask = __ballot(neg);
-    wave_sync();
+    __syncthreads();
 
-    // ---- phase 2
+    // ---- phase 2: wave w owns slots 64w .. 64w + 63 (rows 4w .. 4w + 3 of the tile)
+    const u64 negmask = __ballot(negs[wave * 64 + lane] != 0);
+    const unsigned char* xw = xs + (size_t)wave * 64 * XS;
     const int l32 = lane & 31, hsel = lane >> 5;
     const int KS = NBp / 32, nPB = (G + 31) / 32;
     const i64 nck = kp >> 4;
@@ -138,10 +145,11 @@ __global__ void __launch_bounds__(256) k_rns_split(
         const int m = mods[g];
         const double mi = minv[g];
         const int c1 = C1[g], c2 = C2[g];
+        u32 dh[2][4], dl[2][4];                         // [eb][jq]: 4 bytes at k = 8 (jq & 1) + 4 hsel of row 2 eb + jq / 2
 #pragma unroll
         for (int eb = 0; eb < 2; eb++) {
             v16i ah = zero16(), al = zero16();
-            const unsigned char* xrow = xs + (size_t)(eb * 32 + l32) * XS + 16 * hsel;
+            const unsigned char* xrow = xw + (size_t)(eb * 32 + l32) * XS + 16 * hsel;
             for (int ks = 0; ks < KS; ks += 4) {
                 v4i fh[4], fl[4];
 #pragma unroll
@@ -159,7 +167,6 @@ __global__ void __launch_bounds__(256) k_rns_split(
                     }
                 }
             }
-            // lane: prime pr, slots eb*32 + 8 jq + 4 hsel + u (u = 0..3: 4 consecutive k of one row)
 #pragma unroll
             for (int jq = 0; jq < 4; jq++) {
                 const int s0 = eb * 32 + 8 * jq + 4 * hsel;
@@ -169,18 +176,35 @@ __global__ void __launch_bounds__(256) k_rns_split(
                     const int j = 4 * jq + u;
                     const int nb_ = (int)((negmask >> (s0 + u)) & 1ull);
                     const int x = ah[j] * 128 + al[j] + c1 - (nb_ ? c2 : 0);
-                    const double q = rint((double)x * mi);
-                    const int c = x - (int)q * m;                   // centred residue
+                    const int c = x - (int)rint((double)x * mi) * m;      // centred residue
                     const int tt = c + 64;
                     hi[u] = tt >> 7;
                     lo[u] = (tt & 127) - 64;
                 }
-                const i64 R = slot_row(s0), k = slot_k(s0);
-                if (pok && R < nrows && k < kp) {
-                    const i64 o = (((i64)pr * nck + (k >> 4)) * nrows + R) * 16 + (k & 15);
-                    *(u32*)(Rh + o) = pack4(hi[0], hi[1], hi[2], hi[3]);
-                    *(u32*)(Rl + o) = pack4(lo[0], lo[1], lo[2], lo[3]);
-                }
+                dh[eb][jq] = pack4(hi[0], hi[1], hi[2], hi[3]);
+                dl[eb][jq] = pack4(lo[0], lo[1], lo[2], lo[3]);
+            }
+        }
+        // swap runs across the halves: half 0 keeps rows 2 eb (jq 0, 1), half 1 rows 2 eb + 1 (jq 2, 3)
+#pragma unroll
+        for (int eb = 0; eb < 2; eb++) {
+            const int ja = hsel ? 0 : 2;                // runs this lane gives away
+            const u32 rh0 = (u32)__shfl_xor((int)dh[eb][ja], 32), rh1 = (u32)__shfl_xor((int)dh[eb][ja + 1], 32);
+            const u32 rl0 = (u32)__shfl_xor((int)dl[eb][ja], 32), rl1 = (u32)__shfl_xor((int)dl[eb][ja + 1], 32);
+            const int jo = hsel ? 2 : 0;                // runs this lane keeps
+            v4i vh, vl;
+            if (hsel == 0) {                            // k 0-3 own, 4-7 partner, 8-11 own, 12-15 partner
+                vh[0] = (int)dh[eb][jo]; vh[1] = (int)rh0; vh[2] = (int)dh[eb][jo + 1]; vh[3] = (int)rh1;
+                vl[0] = (int)dl[eb][jo]; vl[1] = (int)rl0; vl[2] = (int)dl[eb][jo + 1]; vl[3] = (int)rl1;
+            } else {                                    // k 0-3 partner, 4-7 own, 8-11 partner, 12-15 own
+                vh[0] = (int)rh0; vh[1] = (int)dh[eb][jo]; vh[2] = (int)rh1; vh[3] = (int)dh[eb][jo + 1];
+                vl[0] = (int)rl0; vl[1] = (int)dl[eb][jo]; vl[2] = (int)rl1; vl[3] = (int)dl[eb][jo + 1];
+            }
+            const i64 R = row0 + 4 * wave + 2 * eb + hsel;
+            if (pok && R < nrows) {
+                const i64 o = (((i64)pr * nck + tk) * nrows + R) * 16;
+                *(v4i*)(Rh + o) = vh;
+                *(v4i*)(Rl + o) = vl;
             }
         }
     }
@@ -572,26 +596,23 @@ int sdpv_rns_split(const int32_t* X, const int64_t* e, const int64_t* Erow, int6
     if (nrows == 0 || kp == 0 || G == 0) return SDPV_OK;
     if (!X || !Thi || !Tlo || !C1 || !C2 || !mods || !minv || !Rh || !Rl) return SDPV_EARG;
     if (e && !Erow) return SDPV_EARG;
-    const bool kmajor = ldk == 1;
-    const i64 ntiles_k = kmajor ? kp / 16 : kp / 4;
-    const i64 ntiles_r = kmajor ? (nrows + 3) / 4 : (nrows + 15) / 16;
-    const i64 ntiles = ntiles_r * ntiles_k;
-    const size_t wbytes = (size_t)64 * (NBp + 16) + (size_t)64 * 4 * (L + 1);
-    int nwave = 4;
-    while (nwave > 1 && nwave * wbytes > 160 * 1024) nwave >>= 1;
-    const size_t lds = nwave * wbytes;
-    if (lds > 160 * 1024) return SDPV_ESIZE;
-    const i64 grid = (ntiles + nwave - 1) / nwave;
+    const i64 ntiles_k = kp / 16, ntiles_r = (nrows + 15) / 16;
+    const i64 grid = ntiles_r * ntiles_k;
     if (grid > 0x7fffffffLL) return SDPV_ESIZE;
+    size_t lds = (size_t)256 * (NBp + 16) + 256;
+    const size_t stage_bytes = (size_t)256 * 4 * (L + 1);
+    const int stage = lds + stage_bytes <= 160 * 1024;   // stage the limbs when they fit
+    if (stage) lds += stage_bytes;
+    if (lds > 160 * 1024) return SDPV_ESIZE;
     static bool attr_set = false;
     if (!attr_set) {
         (void)hipFuncSetAttribute((const void*)k_rns_split, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
         attr_set = true;
     }
-    hipLaunchKernelGGL(k_rns_split, dim3((unsigned)grid), dim3(64 * nwave), lds, as_stream(stream), X, e, Erow, (int)L,
+    hipLaunchKernelGGL(k_rns_split, dim3((unsigned)grid), dim3(256), lds, as_stream(stream), X, e, Erow, (int)L,
                        (int)Lout, (i64)plane, (i64)nb, (i64)rows, (i64)ldb, (i64)ldr, (i64)ldk, (i64)k0, (i64)kc,
                        (i64)kp, Thi, Tlo, (int)NBp, C1, C2, mods, minv, (int)g0, (int)G, Rh, Rl, flag, ntiles_k,
-                       ntiles);
+                       stage);
     SDPV_TRY_LAUNCH();
     return SDPV_OK;
 }

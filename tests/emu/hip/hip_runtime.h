@@ -52,6 +52,7 @@ inline long long atomicMax(long long* p, long long v) {
 long long emu_shfl_xor64(long long v, int mask);
 inline long long __shfl_xor(long long v, int mask, int width = 64) { (void)width; return emu_shfl_xor64(v, mask); }
 inline long __shfl_xor(long v, int mask, int width = 64) { (void)width; return (long)emu_shfl_xor64(v, mask); }
+inline int __shfl_xor(int v, int mask, int width = 64) { (void)width; return (int)emu_shfl_xor64(v, mask); }
 inline double __shfl_xor(double v, int mask, int width = 64) {
     (void)width;
     long long b;

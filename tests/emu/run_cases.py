@@ -159,6 +159,28 @@ def crt_cases():
 CASES["crt"] = crt_cases
 
 
+def residue_cases():
+    """RNS.residues at L = 3 and L = 100 (no limb staging in LDS) against the oracle"""
+    from sdpvibe_amd import mpmat as MM
+    ok = True
+    for L, K in ((3, 17), (100, 20)):
+        R = MM.ctx(L)
+        rng = random.Random(L)
+        rows = 5
+        X = to_planes([rng.randrange(-(1 << (32 * L - 1)), 1 << (32 * L - 1)) for _ in range(rows * K)], L)
+        X = X.reshape(L, rows, K)
+        hi, lo = R.residues(torch.from_numpy(X), 3, K - 5)
+        got = hi.numpy().astype(np.int64) * 128 + lo.numpy().astype(np.int64)
+        ref = O.residues(X, 3, K - 5, np.array(R.mods))
+        good = np.array_equal(got[:, :, :K - 5], ref) and not got[:, :, K - 5:].any()
+        print(f"residues L={L}: {'ok' if good else 'MISMATCH'}", flush=True)
+        ok &= good
+    return ok
+
+
+CASES["residues"] = residue_cases
+
+
 def gram_cases():
     import tests.test_gpu_gram as T
     T.cu = lambda a: torch.from_numpy(np.ascontiguousarray(a))
