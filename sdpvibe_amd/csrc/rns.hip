@@ -112,20 +112,35 @@ __global__ void __launch_bounds__(256) k_rns_split(
         const i64 ex = e ? e[off] : 0;
         const i64 er = e ? Erow[Rc] : 0;
         int neg = 0;
+        // words leave in groups of four (16-byte LDS writes: conflict-free at this row stride)
+        u32 b0 = 0x80808080u, b1 = 0x80808080u, b2 = 0x80808080u, b3 = 0x80808080u;
+        auto flush4 = [&](int w4) {
+            v4i q4;
+            q4[0] = (int)b0; q4[1] = (int)b1; q4[2] = (int)b2; q4[3] = (int)b3;
+            *(v4i*)(row + 4 * w4) = q4;
+            b0 = b1 = b2 = b3 = 0x80808080u;
+        };
         if (valid) {
             i64 s = 0;
             if (e) s = (ex == EZV) ? (i64)(1 << 30) : er - ex;
             auto put = [&](int wi, u32 v) {
-                row[wi] = v ^ 0x80808080u;
+                const u32 x = v ^ 0x80808080u;
+                switch (wi & 3) {
+                    case 0: b0 = x; break;
+                    case 1: b1 = x; break;
+                    case 2: b2 = x; break;
+                    default: b3 = x; flush4(wi >> 2);
+                }
                 if (wi == Lout - 1) neg = (int)(v >> 31);
             };
             const int bad = stage ? shr_round(LdsStrided{raw, 1}, L, s, Lout, put)
                                   : shr_round(PlaneSrc{X + off, plane}, L, s, Lout, put);
             if (bad && flag) atomicOr(flag, 1);
+            if (Lout & 3) flush4(Lout >> 2);                  // last partial group, zero-padded
         } else {
-            for (int w = 0; w < Lout; w++) row[w] = 0x80808080u;   // zero entry
+            for (int w4 = 0; w4 < (Lout + 3) / 4; w4++) flush4(w4);   // zero entry
         }
-        for (int w = Lout; w < NBp / 4; w++) row[w] = 0x80808080u;
+        for (int w4 = (Lout + 3) / 4; w4 < NBp / 16; w4++) flush4(w4);
         negs[S] = (unsigned char)neg;
     }
     __syncthreads();
@@ -218,7 +233,7 @@ __global__ void __launch_bounds__(256) k_rns_split(
 // LDS stage layout: [A h | A l | B h | B l], each plane [chunk][tile row][16 B]: fragment reads of
 // 32 consecutive rows are contiguous and conflict-free. The s = h + l operand is formed in
 // registers (bytewise add) from the h and l fragments.
-template <int WM, int WN, int BLM, int BLN, int NST>
+template <int WM, int WN, int BLM, int BLN, int NST, int P4>
 __global__ void __launch_bounds__(WM* WN * 64, (WM * WN <= 4) ? 2 : 1) k_rns_gemm(
     const int8_t* __restrict__ Ah, const int8_t* __restrict__ Al, const int8_t* __restrict__ Bh,
     const int8_t* __restrict__ Bl, int G, int nb, i64 M, i64 N, i64 kp, int16_t* __restrict__ acc, i64 ld_g,
@@ -291,7 +306,7 @@ __global__ void __launch_bounds__(WM* WN * 64, (WM * WN <= 4) ? 2 : 1) k_rns_gem
 #pragma unroll
         for (int ks = 0; ks < 2; ks++) {
             const int c = 2 * ks + hsel;
-            v4i ah[BLM], al[BLM], as[BLM], bh[BLN], bl[BLN], bs[BLN];
+            v4i ah[BLM], al[BLM], as[BLM], bh[BLN], bl[BLN], bs[BLN];  // as/bs unused when P4
 #pragma unroll
             for (int i = 0; i < BLM; i++) {
                 const int off = (c * BM + (wm * BLM + i) * 32 + l32) * 16;
@@ -304,22 +319,35 @@ __global__ void __launch_bounds__(WM* WN * 64, (WM * WN <= 4) ? 2 : 1) k_rns_gem
                 bh[j] = *(const v4i*)(sB + off);
                 bl[j] = *(const v4i*)(sB + PB + off);
             }
+            if (P4) {
+                // four products, no bytewise adds: HH, LL and MID = HL + LH
 #pragma unroll
-            for (int i = 0; i < BLM; i++)
+                for (int i = 0; i < BLM; i++)
 #pragma unroll
-                for (int q = 0; q < 4; q++) as[i][q] = (int)add_bytes((u32)ah[i][q], (u32)al[i][q]);
+                    for (int j = 0; j < BLN; j++) {
+                        cHH[i][j] = mfma(ah[i], bh[j], cHH[i][j]);
+                        cLL[i][j] = mfma(al[i], bl[j], cLL[i][j]);
+                        cSS[i][j] = mfma(ah[i], bl[j], cSS[i][j]);
+                        cSS[i][j] = mfma(al[i], bh[j], cSS[i][j]);
+                    }
+            } else {
 #pragma unroll
-            for (int j = 0; j < BLN; j++)
+                for (int i = 0; i < BLM; i++)
 #pragma unroll
-                for (int q = 0; q < 4; q++) bs[j][q] = (int)add_bytes((u32)bh[j][q], (u32)bl[j][q]);
+                    for (int q = 0; q < 4; q++) as[i][q] = (int)add_bytes((u32)ah[i][q], (u32)al[i][q]);
 #pragma unroll
-            for (int i = 0; i < BLM; i++)
+                for (int j = 0; j < BLN; j++)
 #pragma unroll
-                for (int j = 0; j < BLN; j++) {
-                    cHH[i][j] = mfma(ah[i], bh[j], cHH[i][j]);
-                    cLL[i][j] = mfma(al[i], bl[j], cLL[i][j]);
-                    cSS[i][j] = mfma(as[i], bs[j], cSS[i][j]);
-                }
+                    for (int q = 0; q < 4; q++) bs[j][q] = (int)add_bytes((u32)bh[j][q], (u32)bl[j][q]);
+#pragma unroll
+                for (int i = 0; i < BLM; i++)
+#pragma unroll
+                    for (int j = 0; j < BLN; j++) {
+                        cHH[i][j] = mfma(ah[i], bh[j], cHH[i][j]);
+                        cLL[i][j] = mfma(al[i], bl[j], cLL[i][j]);
+                        cSS[i][j] = mfma(as[i], bs[j], cSS[i][j]);
+                    }
+            }
         }
     };
 
@@ -336,11 +364,14 @@ __global__ void __launch_bounds__(WM* WN * 64, (WM * WN <= 4) ? 2 : 1) k_rns_gem
         compute(s);
     }
 
-    // Epilogue: v = k0*HH + k1*SS + k2*LL (+ old) is exact in double (|v| < 2^47), reduced once.
+    // Epilogue: v = k0*HH + k1*SS + k2*LL (+ old) is exact in double (|v| < 2^48), reduced once.
     const int g = g0 + gi;
     const int m = mods[g];
     const double mi = minv[g], md = (double)m;
-    const double k0c = (double)kcoef[3 * g], k1c = (double)kcoef[3 * g + 1], k2c = (double)kcoef[3 * g + 2];
+    // three products: ra rb = 16256 HH + 128 SS - 127 LL; four: 16384 HH + 128 MID + LL (mod m)
+    const double k0c = P4 ? (double)kcoef[3 * g] + (double)kcoef[3 * g + 1] : (double)kcoef[3 * g];
+    const double k1c = (double)kcoef[3 * g + 1];
+    const double k2c = P4 ? 1.0 : (double)kcoef[3 * g + 2];
     const bool accum = (mode & SDPV_GEMM_ACCUMULATE) != 0;
     int16_t* out = acc + (i64)g * ld_g + b * M * N;
 #pragma unroll
@@ -364,7 +395,7 @@ __global__ void __launch_bounds__(WM* WN * 64, (WM * WN <= 4) ? 2 : 1) k_rns_gem
         }
 }
 
-template <int WM, int WN, int BLM, int BLN, int NST>
+template <int WM, int WN, int BLM, int BLN, int NST, int P4>
 int launch_gemm(const int8_t* Ah, const int8_t* Al, const int8_t* Bh, const int8_t* Bl, int G, int nb, i64 M, i64 N,
                 i64 kp, int16_t* acc, i64 ld_g, const i32* mods, const double* minv, const i32* kcoef, int g0,
                 int mode, hipStream_t st) {
@@ -376,11 +407,11 @@ int launch_gemm(const int8_t* Ah, const int8_t* Al, const int8_t* Bh, const int8
     const size_t lds = (size_t)NST * 128 * (BM + BN);
     static bool attr_set = false;
     if (!attr_set) {
-        (void)hipFuncSetAttribute((const void*)k_rns_gemm<WM, WN, BLM, BLN, NST>,
+        (void)hipFuncSetAttribute((const void*)k_rns_gemm<WM, WN, BLM, BLN, NST, P4>,
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         attr_set = true;
     }
-    hipLaunchKernelGGL((k_rns_gemm<WM, WN, BLM, BLN, NST>), dim3((unsigned)total), dim3(WM * WN * 64), lds, st, Ah,
+    hipLaunchKernelGGL((k_rns_gemm<WM, WN, BLM, BLN, NST, P4>), dim3((unsigned)total), dim3(WM * WN * 64), lds, st, Ah,
                        Al, Bh, Bl, G, nb, M, N, kp, acc, ld_g, mods, minv, kcoef, g0, mode, tiles_m, tiles_n);
     SDPV_TRY_LAUNCH();
     return SDPV_OK;
@@ -409,8 +440,8 @@ __global__ void __launch_bounds__(256) k_rns_crt(
     const int NW = E / 4;
     const int Epad = ((E + 31) / 32) * 32;
     const int nthreads = blockDim.x;
-    i32* Mds = (i32*)smem;                            // [Epad] digits of M
-    i32* cm = Mds + Epad;                             // [npad] moduli (1 beyond n)
+    i64* MdW = (i64*)smem;                            // [Epad/4] sum_u Md[4w + u] 256^u (word values of M's digits)
+    i32* cm = (i32*)(MdW + Epad / 4);                 // [npad] moduli (1 beyond n)
     i32* cinv = cm + npad;                            // [npad] (M/m)^-1 mod m
     double* cmi = (double*)(cinv + npad);             // [npad] 1/m
     const size_t wbytes = (size_t)64 * YS + (mode != SDPV_CRT_WORDS ? (size_t)128 * (NW + 1) : 0);
@@ -420,7 +451,8 @@ __global__ void __launch_bounds__(256) k_rns_crt(
     int8_t* Yl = Yh + 32 * YS;                        // [32][YS]
     u32* Wl = (u32*)(Yl + 32 * YS);                   // [32][NW + 1] (modes 1, 2)
 
-    for (int d = t; d < Epad; d += nthreads) Mds[d] = Md[d];
+    for (int w4 = t; w4 < Epad / 4; w4 += nthreads)
+        MdW[w4] = (i64)Md[4 * w4] + ((i64)Md[4 * w4 + 1] << 8) + ((i64)Md[4 * w4 + 2] << 16) + ((i64)Md[4 * w4 + 3] << 24);
     for (int g = t; g < npad; g += nthreads) {
         const bool ok = g < n;
         cm[g] = ok ? mods[g] : 1;
@@ -472,7 +504,7 @@ __global__ void __launch_bounds__(256) k_rns_crt(
         *(v4i*)(Yl + (size_t)e * YS + gb) = vl;
     }
     s += __shfl_xor(s, 32);
-    const i64 kq = (i64)floor(s + 0.5);
+    const int kq = (int)floor(s + 0.5);
     wave_sync();
 
     // ---- phase 2
@@ -498,14 +530,14 @@ __global__ void __launch_bounds__(256) k_rns_crt(
                 }
             }
         }
+        // word partial: sum_u Z_u 256^u - k * (word of M's digits), Z_u = 128 zh + zl (|Z_u| < 2^28)
         i64 P[4];
 #pragma unroll
         for (int wq = 0; wq < 4; wq++) {
-            const int d0 = db * 32 + 8 * wq + 4 * half;
-            i64 v = 0;
+            const int w4 = (db * 32 + 8 * wq + 4 * half) >> 2;
+            i64 v = (i64)kq * -MdW[w4];
 #pragma unroll
-            for (int u = 0; u < 4; u++)
-                v += ((i64)(128 * zh[4 * wq + u] + zl[4 * wq + u]) - kq * (i64)Mds[d0 + u]) * ((i64)1 << (8 * u));
+            for (int u = 0; u < 4; u++) v += (i64)(128 * zh[4 * wq + u] + zl[4 * wq + u]) << (8 * u);
             P[wq] = v;
         }
         i64 Q[4];
@@ -621,7 +653,7 @@ int sdpv_rns_gemm(const int8_t* Ah, const int8_t* Al, const int8_t* Bh, const in
                   int64_t M, int64_t N, int64_t kp, int16_t* acc, int64_t ld_g, const int32_t* mods,
                   const double* minv, const int32_t* kcoef, int64_t g0, int32_t mode, int32_t config, void* stream) {
     if (G < 0 || nb < 0 || M < 0 || N < 0 || kp < 0 || g0 < 0 || (mode & ~3)) return SDPV_EARG;
-    if (config < -1 || config > 5) return SDPV_EARG;
+    if (config < -1 || config > 7) return SDPV_EARG;
     if (kp % 64 != 0 || kp > 65536) return SDPV_ESIZE;
     if ((mode & SDPV_GEMM_SYRK_LOWER) && (M != N || config != -1)) return SDPV_EARG;
     if (G == 0 || nb == 0 || M == 0 || N == 0) return SDPV_OK;
@@ -634,12 +666,14 @@ int sdpv_rns_gemm(const int8_t* Ah, const int8_t* Al, const int8_t* Bh, const in
     }
     const int g = (int)G, b = (int)nb, q0 = (int)g0;
     switch (config) {
-        case 0: return launch_gemm<2, 2, 2, 2, 2>(Ah, Al, Bh, Bl, g, b, M, N, kp, acc, ld_g, mods, minv, kcoef, q0, mode, st);
-        case 1: return launch_gemm<2, 2, 2, 2, 3>(Ah, Al, Bh, Bl, g, b, M, N, kp, acc, ld_g, mods, minv, kcoef, q0, mode, st);
-        case 2: return launch_gemm<2, 2, 1, 1, 2>(Ah, Al, Bh, Bl, g, b, M, N, kp, acc, ld_g, mods, minv, kcoef, q0, mode, st);
-        case 3: return launch_gemm<1, 1, 1, 1, 2>(Ah, Al, Bh, Bl, g, b, M, N, kp, acc, ld_g, mods, minv, kcoef, q0, mode, st);
-        case 4: return launch_gemm<4, 2, 2, 2, 2>(Ah, Al, Bh, Bl, g, b, M, N, kp, acc, ld_g, mods, minv, kcoef, q0, mode, st);
-        default: return launch_gemm<2, 4, 2, 2, 2>(Ah, Al, Bh, Bl, g, b, M, N, kp, acc, ld_g, mods, minv, kcoef, q0, mode, st);
+        case 0: return launch_gemm<2, 2, 2, 2, 2, 0>(Ah, Al, Bh, Bl, g, b, M, N, kp, acc, ld_g, mods, minv, kcoef, q0, mode, st);
+        case 1: return launch_gemm<2, 2, 2, 2, 3, 0>(Ah, Al, Bh, Bl, g, b, M, N, kp, acc, ld_g, mods, minv, kcoef, q0, mode, st);
+        case 2: return launch_gemm<2, 2, 1, 1, 2, 0>(Ah, Al, Bh, Bl, g, b, M, N, kp, acc, ld_g, mods, minv, kcoef, q0, mode, st);
+        case 3: return launch_gemm<1, 1, 1, 1, 2, 0>(Ah, Al, Bh, Bl, g, b, M, N, kp, acc, ld_g, mods, minv, kcoef, q0, mode, st);
+        case 4: return launch_gemm<4, 2, 2, 2, 2, 0>(Ah, Al, Bh, Bl, g, b, M, N, kp, acc, ld_g, mods, minv, kcoef, q0, mode, st);
+        case 5: return launch_gemm<2, 4, 2, 2, 2, 0>(Ah, Al, Bh, Bl, g, b, M, N, kp, acc, ld_g, mods, minv, kcoef, q0, mode, st);
+        case 6: return launch_gemm<2, 2, 2, 2, 2, 1>(Ah, Al, Bh, Bl, g, b, M, N, kp, acc, ld_g, mods, minv, kcoef, q0, mode, st);
+        default: return launch_gemm<2, 4, 2, 2, 2, 1>(Ah, Al, Bh, Bl, g, b, M, N, kp, acc, ld_g, mods, minv, kcoef, q0, mode, st);
     }
 }
 

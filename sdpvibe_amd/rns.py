@@ -63,7 +63,7 @@ def gemm_config(M, N, K):
 
 
 # ------------------------------------------------------------------ product tile autotuner
-GEMM_CONFIGS = (0, 1, 2, 3, 4, 5)      # see sdpv_rns_gemm: 128^2 x2 stages, 128^2 x3, 64^2, 32^2, 256x128, 128x256
+GEMM_CONFIGS = (0, 1, 2, 3, 4, 5, 6, 7)   # see sdpv_rns_gemm: tile shapes, stages, three or four products
 _tuned = {}
 TUNE = os.environ.get("SDPV_TUNE", "1") != "0"
 
