@@ -151,37 +151,65 @@ __global__ void __launch_bounds__(256) k_rns_split(
     const int l32 = lane & 31, hsel = lane >> 5;
     const int KS = NBp / 32, nPB = (G + 31) / 32;
     const i64 nck = kp >> 4;
+    // prime-block operands: the first four 32-byte table chunks and the constants, loaded once per
+    // block of primes and shared by both 32-slot halves
+    struct PB {
+        v4i fh[4], fl[4];
+        int m, c1, c2;
+        double mi;
+    };
+    auto load_pb = [&](int pb, PB& o) {
+        const int pr = pb * 32 + l32;
+        const int g = g0 + (pr < G ? pr : 0);
+        const int8_t* th = Thi + (i64)g * NBp + 16 * hsel;
+        const int8_t* tl = Tlo + (i64)g * NBp + 16 * hsel;
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const bool ok = u < KS;
+            o.fh[u] = ok ? *(const v4i*)(th + u * 32) : zero4();
+            o.fl[u] = ok ? *(const v4i*)(tl + u * 32) : zero4();
+        }
+        o.m = mods[g];
+        o.mi = minv[g];
+        o.c1 = C1[g];
+        o.c2 = C2[g];
+    };
+    PB cur;
     for (int pb = 0; pb < nPB; pb++) {
+        load_pb(pb, cur);
         const int pr = pb * 32 + l32;
         const bool pok = pr < G;
         const int g = g0 + (pok ? pr : 0);
         const int8_t* th = Thi + (i64)g * NBp + 16 * hsel;
         const int8_t* tl = Tlo + (i64)g * NBp + 16 * hsel;
-        const int m = mods[g];
-        const double mi = minv[g];
-        const int c1 = C1[g], c2 = C2[g];
+        const int m = cur.m;
+        const double mi = cur.mi;
+        const int c1 = cur.c1, c2 = cur.c2;
+        // both 32-slot blocks against the same table chunks: four independent MFMA chains
+        v16i ah[2] = {zero16(), zero16()}, al[2] = {zero16(), zero16()};
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            if (u < KS) {
+#pragma unroll
+                for (int eb = 0; eb < 2; eb++) {
+                    const v4i fx = *(const v4i*)(xw + (size_t)(eb * 32 + l32) * XS + 16 * hsel + u * 32);
+                    ah[eb] = mfma(fx, cur.fh[u], ah[eb]);
+                    al[eb] = mfma(fx, cur.fl[u], al[eb]);
+                }
+            }
+        }
+        for (int ks = 4; ks < KS; ks++) {                // wide operands (L > 32)
+            const v4i fh = *(const v4i*)(th + ks * 32), fl = *(const v4i*)(tl + ks * 32);
+#pragma unroll
+            for (int eb = 0; eb < 2; eb++) {
+                const v4i fx = *(const v4i*)(xw + (size_t)(eb * 32 + l32) * XS + 16 * hsel + ks * 32);
+                ah[eb] = mfma(fx, fh, ah[eb]);
+                al[eb] = mfma(fx, fl, al[eb]);
+            }
+        }
         u32 dh[2][4], dl[2][4];                         // [eb][jq]: 4 bytes at k = 8 (jq & 1) + 4 hsel of row 2 eb + jq / 2
 #pragma unroll
         for (int eb = 0; eb < 2; eb++) {
-            v16i ah = zero16(), al = zero16();
-            const unsigned char* xrow = xw + (size_t)(eb * 32 + l32) * XS + 16 * hsel;
-            for (int ks = 0; ks < KS; ks += 4) {
-                v4i fh[4], fl[4];
-#pragma unroll
-                for (int u = 0; u < 4; u++) {
-                    const bool ok = ks + u < KS;
-                    fh[u] = ok ? *(const v4i*)(th + (ks + u) * 32) : zero4();
-                    fl[u] = ok ? *(const v4i*)(tl + (ks + u) * 32) : zero4();
-                }
-#pragma unroll
-                for (int u = 0; u < 4; u++) {
-                    if (ks + u < KS) {
-                        const v4i fx = *(const v4i*)(xrow + (ks + u) * 32);
-                        ah = mfma(fx, fh[u], ah);
-                        al = mfma(fx, fl[u], al);
-                    }
-                }
-            }
 #pragma unroll
             for (int jq = 0; jq < 4; jq++) {
                 const int s0 = eb * 32 + 8 * jq + 4 * hsel;
@@ -190,7 +218,7 @@ __global__ void __launch_bounds__(256) k_rns_split(
                 for (int u = 0; u < 4; u++) {
                     const int j = 4 * jq + u;
                     const int nb_ = (int)((negmask >> (s0 + u)) & 1ull);
-                    const int x = ah[j] * 128 + al[j] + c1 - (nb_ ? c2 : 0);
+                    const int x = ah[eb][j] * 128 + al[eb][j] + c1 - (nb_ ? c2 : 0);
                     const int c = x - (int)rint((double)x * mi) * m;      // centred residue
                     const int tt = c + 64;
                     hi[u] = tt >> 7;
@@ -203,18 +231,19 @@ __global__ void __launch_bounds__(256) k_rns_split(
         // swap runs across the halves: half 0 keeps rows 2 eb (jq 0, 1), half 1 rows 2 eb + 1 (jq 2, 3)
 #pragma unroll
         for (int eb = 0; eb < 2; eb++) {
-            const int ja = hsel ? 0 : 2;                // runs this lane gives away
-            const u32 rh0 = (u32)__shfl_xor((int)dh[eb][ja], 32), rh1 = (u32)__shfl_xor((int)dh[eb][ja + 1], 32);
-            const u32 rl0 = (u32)__shfl_xor((int)dl[eb][ja], 32), rl1 = (u32)__shfl_xor((int)dl[eb][ja + 1], 32);
-            const int jo = hsel ? 2 : 0;                // runs this lane keeps
+            // runs this lane gives away (half 0: jq 2, 3; half 1: jq 0, 1) and keeps (the others)
+            const u32 gh0 = hsel ? dh[eb][0] : dh[eb][2], gh1 = hsel ? dh[eb][1] : dh[eb][3];
+            const u32 gl0 = hsel ? dl[eb][0] : dl[eb][2], gl1 = hsel ? dl[eb][1] : dl[eb][3];
+            const u32 kh0 = hsel ? dh[eb][2] : dh[eb][0], kh1 = hsel ? dh[eb][3] : dh[eb][1];
+            const u32 kl0 = hsel ? dl[eb][2] : dl[eb][0], kl1 = hsel ? dl[eb][3] : dl[eb][1];
+            const u32 rh0 = (u32)__shfl_xor((int)gh0, 32), rh1 = (u32)__shfl_xor((int)gh1, 32);
+            const u32 rl0 = (u32)__shfl_xor((int)gl0, 32), rl1 = (u32)__shfl_xor((int)gl1, 32);
+            // half 0: k 0-3 own, 4-7 partner, 8-11 own, 12-15 partner; half 1 the other way round
             v4i vh, vl;
-            if (hsel == 0) {                            // k 0-3 own, 4-7 partner, 8-11 own, 12-15 partner
-                vh[0] = (int)dh[eb][jo]; vh[1] = (int)rh0; vh[2] = (int)dh[eb][jo + 1]; vh[3] = (int)rh1;
-                vl[0] = (int)dl[eb][jo]; vl[1] = (int)rl0; vl[2] = (int)dl[eb][jo + 1]; vl[3] = (int)rl1;
-            } else {                                    // k 0-3 partner, 4-7 own, 8-11 partner, 12-15 own
-                vh[0] = (int)rh0; vh[1] = (int)dh[eb][jo]; vh[2] = (int)rh1; vh[3] = (int)dh[eb][jo + 1];
-                vl[0] = (int)rl0; vl[1] = (int)dl[eb][jo]; vl[2] = (int)rl1; vl[3] = (int)dl[eb][jo + 1];
-            }
+            vh[0] = (int)(hsel ? rh0 : kh0); vh[1] = (int)(hsel ? kh0 : rh0);
+            vh[2] = (int)(hsel ? rh1 : kh1); vh[3] = (int)(hsel ? kh1 : rh1);
+            vl[0] = (int)(hsel ? rl0 : kl0); vl[1] = (int)(hsel ? kl0 : rl0);
+            vl[2] = (int)(hsel ? rl1 : kl1); vl[3] = (int)(hsel ? kl1 : rl1);
             const i64 R = row0 + 4 * wave + 2 * eb + hsel;
             if (pok && R < nrows) {
                 const i64 o = (((i64)pr * nck + tk) * nrows + R) * 16;
@@ -563,11 +592,10 @@ __global__ void __launch_bounds__(256) k_rns_crt(
             }
         }
     }
-    if (half != 0 || !qok) return;
-
-    // ---- phase 3 (lane e, entry q; reads only its own LDS row)
+    // ---- phase 3: entry q of lanes e and e + 32 (its words in lane e's LDS row)
     const i64 i = q / w, jj = q % w;
     if (mode == SDPV_CRT_WORDS) {
+        if (half != 0 || !qok) return;
         const bool neg = (lastw >> 31) != 0;
         const int last = neg ? last1 : last0;
         const u32 top = neg ? w1 : w0;
@@ -578,19 +606,79 @@ __global__ void __launch_bounds__(256) k_rns_crt(
     const i64 cidx = (i / Mb) * ldo + jo + jj;
     LdsWords src{wrow};
     if (mode == SDPV_CRT_NORMALIZE) {
+        // the two halves write output limbs [0, K2) and [K2, L); the rounding carry and the
+        // negation carry into limb K2 come from half 0 afterwards (usually zero) and are added
+        // to half 1's limbs in a short fix-up
+        wave_sync();                                    // half 1 reads half 0's LDS row
         MagView<LdsWords> mag(src, NW);
         const int bl = bitlen_mag(mag, NW);
+        const int K2 = L >> 1;
+        u32 cr = 0, cn = 0;                             // half 0: carries out of limb K2 - 1
+        int sh = 0;
+        if (qok && bl != 0) {
+            sh = bl - (32 * L - 2);
+            const int kb = half ? K2 : 0, ke = half ? L : K2;
+            u64 c = 0;                                  // rounding carry (half 0: the bit below)
+            const i64 q0s = (i64)sh >> 5;
+            const int r = (int)((i64)sh & 31);
+            if (sh > 0 && half == 0) {
+                const i64 sb = (i64)sh - 1, qb = sb >> 5;
+                c = (qb < NW) ? ((mag((int)qb) >> (int)(sb & 31)) & 1u) : 0u;
+            }
+            u64 nc = half ? 0u : 1u;                    // negation carry (~v + 1 starts at limb 0)
+            for (int k = kb; k < ke; k++) {
+                u32 d;
+                if (sh > 0) {
+                    const i64 i0 = q0s + k, i1 = q0s + k + 1;
+                    const u32 lo = i0 < NW ? mag((int)i0) : 0u;
+                    const u32 hi = i1 < NW ? mag((int)i1) : 0u;
+                    d = r ? ((lo >> r) | (hi << (32 - r))) : lo;
+                } else {
+                    const i64 t2 = -(i64)sh, qq = t2 >> 5;
+                    const int rr = (int)(t2 & 31);
+                    const i64 i0 = k - qq, i1 = k - qq - 1;
+                    const u32 lo = (i0 >= 0 && i0 < NW) ? mag((int)i0) : 0u;
+                    const u32 lo2 = (i1 >= 0 && i1 < NW) ? mag((int)i1) : 0u;
+                    d = rr ? ((lo << rr) | (lo2 >> (32 - rr))) : lo;
+                }
+                const u64 tt = (u64)d + c;
+                c = tt >> 32;
+                u32 v = (u32)tt;
+                if (mag.neg) {
+                    const u64 ng = (u64)(~v) + nc;
+                    nc = ng >> 32;
+                    v = (u32)ng;
+                }
+                outm[(i64)k * plane + o] = (i32)v;
+            }
+            cr = (u32)c;
+            cn = (u32)nc;
+        }
+        const u32 rcr = (u32)__shfl_xor((int)cr, 32), rcn = (u32)__shfl_xor((int)cn, 32);
+        if (!qok) return;
         if (bl == 0) {
-            for (int k = 0; k < L; k++) outm[(i64)k * plane + o] = 0;
-            oute[o] = EZV;
+            for (int k = half ? K2 : 0; k < (half ? L : K2); k++) outm[(i64)k * plane + o] = 0;
+            if (half == 0) oute[o] = EZV;
             return;
         }
-        const int sh = bl - (32 * L - 2);
-        NegStream ng(mag.neg);
-        shift_mag(mag, NW, sh, L, [&](int k, u32 v) { outm[(i64)k * plane + o] = (i32)ng(v); });
-        const i64 ein = (erow ? erow[i] : 0) + (ecol ? ecol[cidx] : 0);
-        oute[o] = ein + sh;
+        if (half == 0) {
+            const i64 ein = (erow ? erow[i] : 0) + (ecol ? ecol[cidx] : 0);
+            oute[o] = ein + sh;
+            return;
+        }
+        // half 1: value written = D_hi (or ~D_hi); the true limbs are D_hi + cr (or ~(D_hi + cr) + cn)
+        const int delta = mag.neg ? (int)rcn - (int)rcr : (int)rcr;
+        if (delta != 0) {
+            for (int k = K2; k < L; k++) {
+                i32* pk = outm + (i64)k * plane + o;
+                const u32 v = (u32)*pk;
+                const u32 nv = v + (u32)delta;
+                *pk = (i32)nv;
+                if (delta > 0 ? nv != 0u : v != 0u) break;   // carry / borrow absorbed
+            }
+        }
     } else {
+        if (half != 0 || !qok) return;
         const int bad = shr_round(src, NW, scol[cidx], L, [&](int k, u32 v) { outm[(i64)k * plane + o] = (i32)v; });
         if (bad && flag) atomicOr(flag, 1);
     }

@@ -238,6 +238,17 @@ def product_cases():
 CASES["product"] = product_cases
 
 
+def carry_cases():
+    from tests.test_gpu_kernels import carry_edge_case
+    for L in (1, 2, 3, 24, 25):
+        carry_edge_case(lambda a: torch.from_numpy(np.ascontiguousarray(a)), L)
+    print("carry edges: ok", flush=True)
+    return True
+
+
+CASES["carry"] = carry_cases
+
+
 if __name__ == "__main__":  # noqa: E305
     ok = True
     for c in sys.argv[1:] or list(CASES):

@@ -242,6 +242,28 @@ def product_configs_case(dev_put, M=300, N=200, K=130, nb=2, L=10):
     _eq(np.where(low, got, 0), np.where(low, full, 0), "syrk lower")
 
 
+def carry_edge_case(dev_put, L):
+    """exact products whose normalisation carries through every limb: 2(2^NB - 1) + 1 = 2^(NB+1) - 1
+    rounds up to 2^(NB+1) (the rounding carry), and negative powers of two (the negation carry)"""
+    NB = 32 * L - 2
+    big = (1 << NB) - 1
+    a = [[big, 1], [-big, -1], [-(1 << (NB - 1)), 0], [1 << (NB - 1), -(1 << (NB - 1))]]
+    b = [[2, 1 << (NB - 1)], [1, 1 << (NB - 1)]]
+    Am = to_planes([v for r in a for v in r], L).reshape(L, 1, 4, 2)
+    Bm = to_planes([v for r in b for v in r], L).reshape(L, 1, 2, 2)
+    Ae = np.array([[0, 0], [0, 0], [-5, EZ], [3, 3]], np.int64).reshape(1, 4, 2)
+    Be = np.array([[0, 0], [0, 0]], np.int64).reshape(1, 2, 2)
+    C = F.gemm(F.MPF(dev_put(Am), dev_put(Ae)), F.MPF(dev_put(Bm), dev_put(Be)))
+    rm, re_ = O.mpf_gemm(Am, Ae, Bm, Be)
+    _eq(C.m.cpu().numpy(), rm, f"carry edge m L={L}")
+    _eq(C.e.cpu().numpy(), re_, f"carry edge e L={L}")
+
+
+@pytest.mark.parametrize("L", [1, 2, 3, 24, 25])
+def test_gemm_carry_edges(L):
+    carry_edge_case(cu, L)
+
+
 def test_product_configs():
     product_configs_case(cu)
 
