@@ -126,11 +126,12 @@ def main():
         R.split(B.m, B.e, EB, L, B.e.numel(), 1, N, K * N, 1, N, 0, K, kp, RhB, RlB)
         ev[3].record(st)
         acc = torch.empty((R.n, M, N), dtype=torch.int16, device="cuda")
-        R.product(RhA, RlA, RhB, RlB, acc, 1, M, N, M * N)
+        R.product(RhA, RlA, RhB, RlB, acc, 1, M, N, M * N, scaled=True)
         ev[4].record(st)
         om = torch.empty((L, 1, M, N), dtype=torch.int32, device="cuda")
         oe = torch.empty((1, M, N), dtype=torch.int64, device="cuda")
-        R.crt(acc, M, N, 0, N, 1, outm=om, oute=oe, L=L, plane=M * N, ldo=N, jo=0, Mb=M, erow=EA, ecol=EB)
+        R.crt(acc, M, N, 0, N, 1, outm=om, oute=oe, L=L, plane=M * N, ldo=N, jo=0, Mb=M, erow=EA, ecol=EB,
+              scaled=True)
         ev[5].record(st)
         torch.cuda.synchronize()
         if rep >= 2:

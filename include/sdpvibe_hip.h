@@ -121,14 +121,17 @@ int sdpv_rns_split(const int32_t* X, const int64_t* e, const int64_t* Erow, int6
 
 #define SDPV_GEMM_ACCUMULATE 1 /* acc = (acc + product) mod m instead of acc = product mod m */
 #define SDPV_GEMM_SYRK_LOWER 2 /* B is A (M == N); only tiles on or below the diagonal */
+#define SDPV_GEMM_CENTRED 4    /* store residues centred in [-(m-1)/2, (m-1)/2] instead of [0, m) */
 
 /* Modular products of residues, per prime: for gi < G, b < nb, with a = 128*Ah + Al and
  * b = 128*Bh + Bl the centred residues,
- *   acc[(g0+gi)*ld_g + (b*M + i)*N + j] = (sum_k a[gi, b*M+i, k] * b[gi, b*N+j, k]) mod m_(g0+gi)
- * stored as int16 in [0, m).
+ *   acc[(g0+gi)*ld_g + (b*M + i)*N + j] = s_g * (sum_k a[gi, b*M+i, k] * b[gi, b*N+j, k]) mod m_g
+ * (g = g0+gi) stored as int16 in [0, m), or centred with SDPV_GEMM_CENTRED.
  * Ah/Al: (G, kp/16, nb*M, 16), Bh/Bl: (G, kp/16, nb*N, 16) int8, chunk-major as written by
  * sdpv_rns_split, kp a multiple of 64, kp <= 65536.
- * kcoef: (n, 3) int32 = (16256, 128, -127) mod m: ra*rb = 16256*HH + 128*SS - 127*LL.
+ * kcoef: (n, 3) int32 = (16256, 128, -127) * s_g mod m: ra*rb = 16256*HH + 128*SS - 127*LL.
+ * s_g = 1 gives the residues of the product; s_g = (M/m_g)^-1 mod m_g with SDPV_GEMM_CENTRED
+ * gives the CRT coefficients that sdpv_rns_crt takes with SDPV_CRT_SCALED.
  * Replaces RNS.accumulate / _gemm_kernel (sdpvibe/rns.py:217-224, :107-137),
  * _bgemm_kernel (sdpvibe/batched.py:97-127) and _syrk_lower_kernel (sdpvibe/batched.py:131-162). */
 /* config: -1 picks the tile by size (and is required with SDPV_GEMM_SYRK_LOWER); 0..5 select
@@ -144,12 +147,14 @@ int sdpv_rns_gemm(const int8_t* Ah, const int8_t* Al, const int8_t* Bh, const in
 #define SDPV_CRT_WORDS 0     /* W (E/4, B) int32 two's complement + bitlen (B,) */
 #define SDPV_CRT_NORMALIZE 1 /* MPF output: L limbs with NB bits, exponent ein + shift */
 #define SDPV_CRT_ROUND 2     /* fixed-exponent output: round_half_up(value / 2^s) to L limbs */
+#define SDPV_CRT_SCALED 4    /* flag: acc holds y = residue * (M/m)^-1 mod m, centred */
 
 /* CRT reconstruction of the exact integers held as residues acc[g*ld_g + i*N + j]
- * (g < n, 0 <= value residue < m_g) for the block of entries (i, j0 + jj), i < rows, jj < w;
- * entry index q = i*w + jj. Tables: mods, minv (1/m double), inv ((M/m)^-1 mod m), Ct (E, npad)
- * int8 base-256 signed digits of M/m, Md (E,) digits of M.
- * Ct is (Epad, npad) and Md (Epad,), Epad = 32*ceil(E/32), zero-padded.
+ * (g < n, 0 <= value residue < m_g; with SDPV_CRT_SCALED in mode the centred
+ * residue * (M/m_g)^-1 mod m_g) for the block of entries (i, j0 + jj), i < rows, jj < w;
+ * entry index q = i*w + jj. Tables: mods, minv (1/m double), inv ((M/m)^-1 mod m),
+ * Ct (Epad, npad) int8, Epad = 32*ceil(E/32), npad > n: column g < n holds the base-256 signed
+ * digits of M/m_g, column n the digits of M, the rest and rows >= E are zero.
  * mode SDPV_CRT_WORDS:     W[t*Bq + q] (t < E/4), bitlen[q]              (Bq = rows * w)
  * mode SDPV_CRT_NORMALIZE: outm[k*plane + o], oute[o], o = i*ldo + jo + jj, with
  *                          ein = erow[i] + ecol[c], c = (i / Mb)*ldo + jo + jj (NULL arrays = 0)
@@ -159,7 +164,7 @@ int sdpv_rns_gemm(const int8_t* Ah, const int8_t* Al, const int8_t* Bh, const in
  * (sdpvibe/mpf.py:440-450, sdpvibe/mpmat.py:205-236, sdpvibe/batched.py:200-231). */
 int sdpv_rns_crt(const int16_t* acc, int64_t n, int64_t ld_g, int64_t rows, int64_t N,
                  int64_t j0, int64_t w, const int32_t* mods, const double* minv,
-                 const int32_t* inv, const int8_t* Ct, const int32_t* Md, int64_t E, int64_t npad,
+                 const int32_t* inv, const int8_t* Ct, int64_t E, int64_t npad,
                  int32_t mode, int32_t* W, int64_t* bitlen, int32_t* outm, int64_t* oute,
                  int64_t L, int64_t plane, int64_t ldo, int64_t jo, int64_t Mb, const int64_t* erow,
                  const int64_t* ecol, const int64_t* scol, int32_t* flag, void* stream);

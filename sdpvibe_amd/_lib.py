@@ -29,7 +29,7 @@ SIGNATURES = {
                         _P, _P, _I64, _P, _P, _P, _P, _I64, _I64, _P, _P, _P, _P], _I32),
     "sdpv_rns_gemm": ([_P, _P, _P, _P, _I64, _I64, _I64, _I64, _I64, _P, _I64, _P, _P, _P, _I64, _I32, _I32, _P],
                       _I32),
-    "sdpv_rns_crt": ([_P, _I64, _I64, _I64, _I64, _I64, _I64, _P, _P, _P, _P, _P, _I64, _I64, _I32, _P, _P,
+    "sdpv_rns_crt": ([_P, _I64, _I64, _I64, _I64, _I64, _I64, _P, _P, _P, _P, _I64, _I64, _I32, _P, _P,
                       _P, _P, _I64, _I64, _I64, _I64, _I64, _P, _P, _P, _P, _P], _I32),
     "sdpv_tile_chol": ([_P, _I64, _I64, _I64, _P, _P, _P, _P, _P], _I32),
     "sdpv_tile_trsv": ([_P, _P, _P, _I64, _I64, _I64, _I64, _P], _I32),

@@ -149,9 +149,10 @@ def _bcrt_round_panel(R, acc, nb, M, N, j0, jw, e_in, out_exp, L, out, exps):
         else:
             s = (out_exp - e_in).reshape(-1).contiguous()
             exps[:, j0:j0 + jw] = out_exp
-        R.crt(acc, rows, jw, 0, jw, 2, outm=out, L=L, plane=nb * M * N, ldo=N, jo=j0, Mb=M, scol=s, flag=_errbuf())
+        R.crt(acc, rows, jw, 0, jw, 2, outm=out, L=L, plane=nb * M * N, ldo=N, jo=j0, Mb=M, scol=s, flag=_errbuf(),
+              scaled=True)
         return
-    Wd, bl, Bq = R.crt_block(acc, jw, 0, jw, rows)                 # entries (b*M + i)*jw + jj
+    Wd, bl, Bq = R.crt_block(acc, jw, 0, jw, rows, True)               # entries (b*M + i)*jw + jj
     ein = e_in if torch.is_tensor(e_in) else torch.full((nb, N), e_in, dtype=torch.int64, device=dev)
     sj = bl.view(nb, M, jw).amax(1) - headroom_bits(L)             # (nb, jw)
     ej = ein[:, j0:j0 + jw] + sj

@@ -46,6 +46,14 @@ __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// Exchanges lanes 32-63 of x with lanes 0-31 of y (v_permlane32_swap). Writing a register as
+// (lanes 0-31, lanes 32-63): (x_lo, x_hi), (y_lo, y_hi) become x = (x_lo, y_lo), y = (x_hi, y_hi).
+__device__ __forceinline__ void swap_halves(u32& x, u32& y) {
+    const auto r = __builtin_amdgcn_permlane32_swap(x, y, false, false);
+    x = r[0];
+    y = r[1];
+}
+
 __device__ __forceinline__ int clz32(u32 x) { return __clz((int)x); }
 
 // Word i of the infinite two's-complement extension of an ns-word value (sdpvibe/limbops.py:22-28).

@@ -397,11 +397,14 @@ __global__ void __launch_bounds__(WM* WN * 64, (WM * WN <= 4) ? 2 : 1) k_rns_gem
     const int g = g0 + gi;
     const int m = mods[g];
     const double mi = minv[g], md = (double)m;
-    // three products: ra rb = 16256 HH + 128 SS - 127 LL; four: 16384 HH + 128 MID + LL (mod m)
-    const double k0c = P4 ? (double)kcoef[3 * g] + (double)kcoef[3 * g + 1] : (double)kcoef[3 * g];
-    const double k1c = (double)kcoef[3 * g + 1];
-    const double k2c = P4 ? 1.0 : (double)kcoef[3 * g + 2];
+    // kcoef = (16256, 128, -127) * s mod m. Three products: ra rb = 16256 HH + 128 SS - 127 LL;
+    // four: 16384 HH + 128 MID + LL, with 16384 = 16256 + 128 and 1 = 128 - 127.
+    const double kc0 = (double)kcoef[3 * g], kc1 = (double)kcoef[3 * g + 1], kc2 = (double)kcoef[3 * g + 2];
+    const double k0c = P4 ? kc0 + kc1 : kc0;
+    const double k1c = kc1;
+    const double k2c = P4 ? kc1 + kc2 : kc2;
     const bool accum = (mode & SDPV_GEMM_ACCUMULATE) != 0;
+    const bool centred = (mode & SDPV_GEMM_CENTRED) != 0;
     int16_t* out = acc + (i64)g * ld_g + b * M * N;
 #pragma unroll
     for (int i = 0; i < BLM; i++)
@@ -414,10 +417,9 @@ __global__ void __launch_bounds__(WM* WN * 64, (WM * WN <= 4) ? 2 : 1) k_rns_gem
                 if (row < M && col < N) {
                     double v = fma(k0c, (double)cHH[i][j][q], fma(k1c, (double)cSS[i][j][q], k2c * (double)cLL[i][j][q]));
                     if (accum) v += (double)out[row * N + col];
-                    const double qt = floor(v * mi);
-                    int r = (int)fma(-qt, md, v);
-                    if (r < 0) r += m;
-                    if (r >= m) r -= m;
+                    // nearest quotient, exact: v*mi errs by < 2^-17, v/m is >= 1/(2m) > 2^-15 from a half-integer
+                    int r = (int)fma(-rint(v * mi), md, v);
+                    if (!centred && r < 0) r += m;
                     out[row * N + col] = (int16_t)r;
                 }
             }
@@ -447,81 +449,71 @@ int launch_gemm(const int8_t* Ah, const int8_t* Al, const int8_t* Bh, const int8
 }
 
 // ================================================================== CRT reconstruction
-// One wavefront reconstructs 32 entries on its own; the waves of a workgroup share only the prime
-// and digit constants in LDS, so there is no workgroup barrier after the prologue.
-// Phase 1 (2 lanes per entry, each over half of the primes): y_g = centred(acc_g * inv_g mod m_g),
-//   split to 128*hi + lo and written as packed bytes to the wave's LDS rows; k = round(sum y_g/m_g)
-//   (|sum - k| < 1/4 because |value| < M/4, so double suffices).
-// Phase 2, per block of 32 base-256 digits: Z = Ct . Y on MFMA with digits as rows and entries as
-//   columns, so lane e holds the digits of words 0, 2, 4, 6 of entry e and lane e + 32 those of
-//   words 1, 3, 5, 7; each lane folds its digits into 4 word partials sum (Z_d - k Md_d) 256^u,
-//   the odd partials move across with one exchange and lane e carries the 8 words in order.
-// Phase 3: words -> W + bitlen (mode 0), or normalised MPF limbs (1), or a fixed-exponent rounding (2).
+// One wavefront (one workgroup) reconstructs 64 entries; lane l owns entry q0 + l.
+// Phase 1 (lane = entry, every prime; the prime index is uniform, so its constants and plane
+//   address are scalar): y_g = centred(acc_g * inv_g mod m_g), or acc_g itself when the product
+//   stored it so (SCALED); split to 128*hi + lo and written as packed bytes to the lane's LDS rows;
+//   k = round(sum y_g/m_g) (|sum - k| < 1/4 because |value| < M/4), and -k into column n, whose
+//   Ct column holds the digits of M.
+// Phase 2, per block of 32 base-256 digits and per group of 32 entries: Z = Ct . Y on MFMA with
+//   digits as rows and entries as columns (Z_d = digit d of sum_g y_g M/m_g - k M). Lanes 0-31 then
+//   hold the digits of words 0, 2, 4, 6 and lanes 32-63 those of words 1, 3, 5, 7 of the group's
+//   entries; each lane folds its digits into word partials sum Z_d 256^u, one lane-half swap gives
+//   every lane the 8 partials of its own entry, and it carries them into words.
+// Phase 3 (lane = entry): words -> W + bitlen (mode 0), normalised MPF limbs (1), or a
+//   fixed-exponent rounding (2).
 
-__global__ void __launch_bounds__(256) k_rns_crt(
+template <bool SCALED>
+__global__ void __launch_bounds__(64) k_rns_crt(
     const int16_t* __restrict__ acc, int n, i64 ld_g, i64 rows, i64 N, i64 j0, i64 w, const i32* __restrict__ mods,
-    const double* __restrict__ minv, const i32* __restrict__ inv, const int8_t* __restrict__ Ct,
-    const i32* __restrict__ Md, int E, int npad, int mode, i32* __restrict__ W, i64* __restrict__ bitlen,
-    i32* __restrict__ outm, i64* __restrict__ oute, int L, i64 plane, i64 ldo, i64 jo, i64 Mb,
-    const i64* __restrict__ erow, const i64* __restrict__ ecol, const i64* __restrict__ scol, i32* __restrict__ flag) {
+    const double* __restrict__ minv, const i32* __restrict__ inv, const int8_t* __restrict__ Ct, int E, int npad,
+    int mode, i32* __restrict__ W, i64* __restrict__ bitlen, i32* __restrict__ outm, i64* __restrict__ oute, int L,
+    i64 plane, i64 ldo, i64 jo, i64 Mb, const i64* __restrict__ erow, const i64* __restrict__ ecol,
+    const i64* __restrict__ scol, i32* __restrict__ flag) {
     SDPV_DYNAMIC_LDS(unsigned char, smem);
     const int YS = npad + 16;                         // bytes per entry row (multiple of 16)
     const int NW = E / 4;
     const int Epad = ((E + 31) / 32) * 32;
-    const int nthreads = blockDim.x;
-    i64* MdW = (i64*)smem;                            // [Epad/4] sum_u Md[4w + u] 256^u (word values of M's digits)
-    i32* cm = (i32*)(MdW + Epad / 4);                 // [npad] moduli (1 beyond n)
-    i32* cinv = cm + npad;                            // [npad] (M/m)^-1 mod m
-    double* cmi = (double*)(cinv + npad);             // [npad] 1/m
-    const size_t wbytes = (size_t)64 * YS + (mode != SDPV_CRT_WORDS ? (size_t)128 * (NW + 1) : 0);
-    const int t = threadIdx.x, wave = t >> 6, lane = t & 63, e = lane & 31, half = lane >> 5;
-    unsigned char* wbase = (unsigned char*)(cmi + npad) + (size_t)wave * wbytes;
-    int8_t* Yh = (int8_t*)wbase;                      // [32][YS]
-    int8_t* Yl = Yh + 32 * YS;                        // [32][YS]
-    u32* Wl = (u32*)(Yl + 32 * YS);                   // [32][NW + 1] (modes 1, 2)
-
-    for (int w4 = t; w4 < Epad / 4; w4 += nthreads)
-        MdW[w4] = (i64)Md[4 * w4] + ((i64)Md[4 * w4 + 1] << 8) + ((i64)Md[4 * w4 + 2] << 16) + ((i64)Md[4 * w4 + 3] << 24);
-    for (int g = t; g < npad; g += nthreads) {
-        const bool ok = g < n;
-        cm[g] = ok ? mods[g] : 1;
-        cinv[g] = ok ? inv[g] : 0;
-        cmi[g] = ok ? minv[g] : 0.0;
-    }
-    __syncthreads();
+    const int lane = threadIdx.x, l32 = lane & 31, half = lane >> 5;
+    int8_t* Yh = (int8_t*)smem;                       // [64][YS]
+    int8_t* Yl = Yh + 64 * YS;                        // [64][YS]
+    u32* wrow = (u32*)(Yl + 64 * YS) + (size_t)lane * (NW + 1);   // [64][NW + 1] (modes 1, 2)
 
     const i64 Bq = rows * w;
-    const i64 q0 = ((i64)blockIdx.x * (nthreads >> 6) + wave) * 32;
-    if (q0 >= Bq) return;                             // whole wave: no barrier follows
-    const i64 q = q0 + e;
+    const i64 q = (i64)blockIdx.x * 64 + lane;
     const bool qok = q < Bq;
-    const i64 off = qok ? (q / w) * N + j0 + q % w : 0;
+    // byte offset of the entry in a prime plane (< 2^32: the host checks ld_g < 2^31); lanes past
+    // the block read entry 0 and store nothing
+    const u32 boff = qok ? (u32)(((q / w) * N + j0 + q % w) * 2) : 0u;
 
     // ---- phase 1
-    const int hp = npad >> 1;                         // primes per lane half (multiple of 16)
     double s = 0.0;
-    for (int gc = 0; gc < hp; gc += 16) {
-        const int gb = half * hp + gc;
+    int8_t* yhr = Yh + (size_t)lane * YS;
+    int8_t* ylr = Yl + (size_t)lane * YS;
+    for (int gc = 0; gc < npad; gc += 16) {
+        const int cnt = n - gc;                       // primes in this chunk (uniform)
         int a[16];
+        if (cnt >= 16) {
 #pragma unroll
-        for (int u = 0; u < 16; u++) {                  // independent loads (clamped prime, valid address)
-            const int g = gb + u;
-            const int gg = g < n ? g : n - 1;
-            a[u] = (int)acc[(i64)gg * ld_g + off];
+            for (int u = 0; u < 16; u++)
+                a[u] = *(const int16_t*)((const char*)(acc + (i64)(gc + u) * ld_g) + boff);
+        } else {
+#pragma unroll
+            for (int u = 0; u < 16; u++)
+                a[u] = u < cnt ? *(const int16_t*)((const char*)(acc + (i64)(gc + u) * ld_g) + boff) : 0;
         }
-#pragma unroll
-        for (int u = 0; u < 16; u++) a[u] = (gb + u < n && qok) ? a[u] : 0;
         int hi[16], lo[16];
 #pragma unroll
         for (int u = 0; u < 16; u++) {
-            const int g = gb + u;
-            const double mi = cmi[g];
-            const int x = a[u] * cinv[g];                    // 0 <= x < m^2 < 2^28
-            const int y = x - (int)rint((double)x * mi) * cm[g];   // centred residue, exact
-            s += (double)y * mi;
-            const int tt = y + 64;
-            hi[u] = tt >> 7;
-            lo[u] = (tt & 127) - 64;
+            const int g = gc + u < n ? gc + u : n - 1;  // padding primes: a = 0, so y = 0
+            int y = a[u];
+            if (!SCALED) {                              // x < m^2 < 2^28: exact nearest quotient
+                const double x = (double)y * (double)inv[g];
+                y = (int)fma(-rint(x * minv[g]), (double)mods[g], x);
+            }
+            s = fma((double)y, minv[g], s);
+            hi[u] = (y + 64) >> 7;
+            lo[u] = y - (hi[u] << 7);
         }
         v4i vh, vl;
 #pragma unroll
@@ -529,73 +521,91 @@ __global__ void __launch_bounds__(256) k_rns_crt(
             vh[c] = (int)pack4(hi[4 * c], hi[4 * c + 1], hi[4 * c + 2], hi[4 * c + 3]);
             vl[c] = (int)pack4(lo[4 * c], lo[4 * c + 1], lo[4 * c + 2], lo[4 * c + 3]);
         }
-        *(v4i*)(Yh + (size_t)e * YS + gb) = vh;
-        *(v4i*)(Yl + (size_t)e * YS + gb) = vl;
+        *(v4i*)(yhr + gc) = vh;
+        *(v4i*)(ylr + gc) = vl;
     }
-    s += __shfl_xor(s, 32);
-    const int kq = (int)floor(s + 0.5);
+    {                                                 // column n: -k = 128*hi + lo (|k| <= n/2)
+        const int nk = -(int)rint(s);
+        const int kl = ((nk + 64) & 127) - 64;
+        yhr[n] = (int8_t)((nk - kl) >> 7);
+        ylr[n] = (int8_t)kl;
+    }
     wave_sync();
 
     // ---- phase 2
     const int nDB = Epad / 32, nK = npad / 32;
-    const int8_t* yh = Yh + (size_t)e * YS + 16 * half;
-    const int8_t* yl = Yl + (size_t)e * YS + 16 * half;
+    const int8_t* y0h = Yh + (size_t)l32 * YS + 16 * half;   // B operand of entry group 0
+    const int8_t* y0l = Yl + (size_t)l32 * YS + 16 * half;
+    const int8_t* y1h = y0h + 32 * YS;                        // group 1
+    const int8_t* y1l = y0l + 32 * YS;
     i64 carry = 0;
     int last0 = -1, last1 = -1;
     u32 w0 = 0, w1 = 0, lastw = 0;
-    u32* wrow = Wl + (size_t)e * (NW + 1);
     for (int db = 0; db < nDB; db++) {
-        v16i zh = zero16(), zl = zero16();
-        const int8_t* crow = Ct + (i64)(db * 32 + e) * npad + 16 * half;
-        for (int kb = 0; kb < nK; kb += 8) {
-            v4i cf[8];
+        const int8_t* crow = Ct + (i64)(db * 32 + l32) * npad + 16 * half;
+        const v4i c0 = *(const v4i*)crow;
+        v16i zh0 = mfma(c0, *(const v4i*)y0h, zero16()), zl0 = mfma(c0, *(const v4i*)y0l, zero16());
+        v16i zh1 = mfma(c0, *(const v4i*)y1h, zero16()), zl1 = mfma(c0, *(const v4i*)y1l, zero16());
+        for (int kb = 1; kb < nK; kb += 4) {
+            v4i cf[4];
 #pragma unroll
-            for (int u = 0; u < 8; u++) cf[u] = (kb + u < nK) ? *(const v4i*)(crow + (kb + u) * 32) : zero4();
+            for (int u = 0; u < 4; u++) cf[u] = (kb + u < nK) ? *(const v4i*)(crow + (kb + u) * 32) : zero4();
 #pragma unroll
-            for (int u = 0; u < 8; u++) {
+            for (int u = 0; u < 4; u++) {
                 if (kb + u < nK) {
-                    zh = mfma(cf[u], *(const v4i*)(yh + (kb + u) * 32), zh);
-                    zl = mfma(cf[u], *(const v4i*)(yl + (kb + u) * 32), zl);
+                    const int o = (kb + u) * 32;
+                    zh0 = mfma(cf[u], *(const v4i*)(y0h + o), zh0);
+                    zl0 = mfma(cf[u], *(const v4i*)(y0l + o), zl0);
+                    zh1 = mfma(cf[u], *(const v4i*)(y1h + o), zh1);
+                    zl1 = mfma(cf[u], *(const v4i*)(y1l + o), zl1);
                 }
             }
         }
-        // word partial: sum_u Z_u 256^u - k * (word of M's digits), Z_u = 128 zh + zl (|Z_u| < 2^28)
-        i64 P[4];
+        // word partials sum_u Z_u 256^u, Z_u = 128 zh + zl (|Z_u| < 2^31); X: group 0, Y: group 1
+        u32 xl[4], xh[4], yl[4], yh[4];
 #pragma unroll
         for (int wq = 0; wq < 4; wq++) {
-            const int w4 = (db * 32 + 8 * wq + 4 * half) >> 2;
-            i64 v = (i64)kq * -MdW[w4];
+            i64 v0 = 0, v1 = 0;
 #pragma unroll
-            for (int u = 0; u < 4; u++) v += (i64)(128 * zh[4 * wq + u] + zl[4 * wq + u]) << (8 * u);
-            P[wq] = v;
+            for (int u = 0; u < 4; u++) {
+                v0 += (i64)(128 * zh0[4 * wq + u] + zl0[4 * wq + u]) << (8 * u);
+                v1 += (i64)(128 * zh1[4 * wq + u] + zl1[4 * wq + u]) << (8 * u);
+            }
+            xl[wq] = (u32)v0;
+            xh[wq] = (u32)((u64)v0 >> 32);
+            yl[wq] = (u32)v1;
+            yh[wq] = (u32)((u64)v1 >> 32);
+            // lanes 0-31 of X hold the even words of group 0's entries (their owners), lanes 32-63
+            // of X the odd ones; Y likewise for group 1, owned by lanes 32-63. After the swap, X
+            // holds the even and Y the odd word partial of the lane's own entry.
+            swap_halves(xl[wq], yl[wq]);
+            swap_halves(xh[wq], yh[wq]);
         }
-        i64 Q[4];
 #pragma unroll
-        for (int wq = 0; wq < 4; wq++) Q[wq] = __shfl_xor(P[wq], 32);
-        if (half == 0) {
-#pragma unroll
-            for (int h2 = 0; h2 < 8; h2++) {
-                const i64 v = carry + ((h2 & 1) ? Q[h2 >> 1] : P[h2 >> 1]);
-                const u32 word = (u32)v;
-                carry = v >> 32;
-                const int wi = db * 8 + h2;
-                if (wi < NW) {
-                    if (mode == SDPV_CRT_WORDS) {
-                        if (qok) W[(i64)wi * Bq + q] = (i32)word;
-                        if (word != 0u) { last0 = wi; w0 = word; }
-                        if (word != 0xffffffffu) { last1 = wi; w1 = ~word; }
-                        lastw = word;
-                    } else {
-                        wrow[wi] = word;
-                    }
+        for (int h2 = 0; h2 < 8; h2++) {
+            const int wq = h2 >> 1;
+            const u64 p = (h2 & 1) ? ((u64)yh[wq] << 32 | yl[wq]) : ((u64)xh[wq] << 32 | xl[wq]);
+            const i64 v = carry + (i64)p;
+            const u32 word = (u32)v;
+            carry = v >> 32;
+            const int wi = db * 8 + h2;
+            if (wi < NW) {
+                if (mode == SDPV_CRT_WORDS) {
+                    if (qok) W[(i64)wi * Bq + q] = (i32)word;
+                    if (word != 0u) { last0 = wi; w0 = word; }
+                    if (word != 0xffffffffu) { last1 = wi; w1 = ~word; }
+                    lastw = word;
+                } else {
+                    wrow[wi] = word;
                 }
             }
         }
     }
-    // ---- phase 3: entry q of lanes e and e + 32 (its words in lane e's LDS row)
+
+    // ---- phase 3 (lane = entry; reads only its own LDS row)
+    if (!qok) return;
     const i64 i = q / w, jj = q % w;
     if (mode == SDPV_CRT_WORDS) {
-        if (half != 0 || !qok) return;
         const bool neg = (lastw >> 31) != 0;
         const int last = neg ? last1 : last0;
         const u32 top = neg ? w1 : w0;
@@ -606,79 +616,19 @@ __global__ void __launch_bounds__(256) k_rns_crt(
     const i64 cidx = (i / Mb) * ldo + jo + jj;
     LdsWords src{wrow};
     if (mode == SDPV_CRT_NORMALIZE) {
-        // the two halves write output limbs [0, K2) and [K2, L); the rounding carry and the
-        // negation carry into limb K2 come from half 0 afterwards (usually zero) and are added
-        // to half 1's limbs in a short fix-up
-        wave_sync();                                    // half 1 reads half 0's LDS row
         MagView<LdsWords> mag(src, NW);
         const int bl = bitlen_mag(mag, NW);
-        const int K2 = L >> 1;
-        u32 cr = 0, cn = 0;                             // half 0: carries out of limb K2 - 1
-        int sh = 0;
-        if (qok && bl != 0) {
-            sh = bl - (32 * L - 2);
-            const int kb = half ? K2 : 0, ke = half ? L : K2;
-            u64 c = 0;                                  // rounding carry (half 0: the bit below)
-            const i64 q0s = (i64)sh >> 5;
-            const int r = (int)((i64)sh & 31);
-            if (sh > 0 && half == 0) {
-                const i64 sb = (i64)sh - 1, qb = sb >> 5;
-                c = (qb < NW) ? ((mag((int)qb) >> (int)(sb & 31)) & 1u) : 0u;
-            }
-            u64 nc = half ? 0u : 1u;                    // negation carry (~v + 1 starts at limb 0)
-            for (int k = kb; k < ke; k++) {
-                u32 d;
-                if (sh > 0) {
-                    const i64 i0 = q0s + k, i1 = q0s + k + 1;
-                    const u32 lo = i0 < NW ? mag((int)i0) : 0u;
-                    const u32 hi = i1 < NW ? mag((int)i1) : 0u;
-                    d = r ? ((lo >> r) | (hi << (32 - r))) : lo;
-                } else {
-                    const i64 t2 = -(i64)sh, qq = t2 >> 5;
-                    const int rr = (int)(t2 & 31);
-                    const i64 i0 = k - qq, i1 = k - qq - 1;
-                    const u32 lo = (i0 >= 0 && i0 < NW) ? mag((int)i0) : 0u;
-                    const u32 lo2 = (i1 >= 0 && i1 < NW) ? mag((int)i1) : 0u;
-                    d = rr ? ((lo << rr) | (lo2 >> (32 - rr))) : lo;
-                }
-                const u64 tt = (u64)d + c;
-                c = tt >> 32;
-                u32 v = (u32)tt;
-                if (mag.neg) {
-                    const u64 ng = (u64)(~v) + nc;
-                    nc = ng >> 32;
-                    v = (u32)ng;
-                }
-                outm[(i64)k * plane + o] = (i32)v;
-            }
-            cr = (u32)c;
-            cn = (u32)nc;
-        }
-        const u32 rcr = (u32)__shfl_xor((int)cr, 32), rcn = (u32)__shfl_xor((int)cn, 32);
-        if (!qok) return;
         if (bl == 0) {
-            for (int k = half ? K2 : 0; k < (half ? L : K2); k++) outm[(i64)k * plane + o] = 0;
-            if (half == 0) oute[o] = EZV;
+            for (int k = 0; k < L; k++) outm[(i64)k * plane + o] = 0;
+            oute[o] = EZV;
             return;
         }
-        if (half == 0) {
-            const i64 ein = (erow ? erow[i] : 0) + (ecol ? ecol[cidx] : 0);
-            oute[o] = ein + sh;
-            return;
-        }
-        // half 1: value written = D_hi (or ~D_hi); the true limbs are D_hi + cr (or ~(D_hi + cr) + cn)
-        const int delta = mag.neg ? (int)rcn - (int)rcr : (int)rcr;
-        if (delta != 0) {
-            for (int k = K2; k < L; k++) {
-                i32* pk = outm + (i64)k * plane + o;
-                const u32 v = (u32)*pk;
-                const u32 nv = v + (u32)delta;
-                *pk = (i32)nv;
-                if (delta > 0 ? nv != 0u : v != 0u) break;   // carry / borrow absorbed
-            }
-        }
+        const int sh = bl - (32 * L - 2);
+        NegStream ng(mag.neg);
+        shift_mag(mag, NW, sh, L, [&](int k, u32 v) { outm[(i64)k * plane + o] = (i32)ng(v); });
+        const i64 ein = (erow ? erow[i] : 0) + (ecol ? ecol[cidx] : 0);
+        oute[o] = ein + sh;
     } else {
-        if (half != 0 || !qok) return;
         const int bad = shr_round(src, NW, scol[cidx], L, [&](int k, u32 v) { outm[(i64)k * plane + o] = (i32)v; });
         if (bad && flag) atomicOr(flag, 1);
     }
@@ -740,7 +690,7 @@ int sdpv_rns_split(const int32_t* X, const int64_t* e, const int64_t* Erow, int6
 int sdpv_rns_gemm(const int8_t* Ah, const int8_t* Al, const int8_t* Bh, const int8_t* Bl, int64_t G, int64_t nb,
                   int64_t M, int64_t N, int64_t kp, int16_t* acc, int64_t ld_g, const int32_t* mods,
                   const double* minv, const int32_t* kcoef, int64_t g0, int32_t mode, int32_t config, void* stream) {
-    if (G < 0 || nb < 0 || M < 0 || N < 0 || kp < 0 || g0 < 0 || (mode & ~3)) return SDPV_EARG;
+    if (G < 0 || nb < 0 || M < 0 || N < 0 || kp < 0 || g0 < 0 || (mode & ~7)) return SDPV_EARG;
     if (config < -1 || config > 7) return SDPV_EARG;
     if (kp % 64 != 0 || kp > 65536) return SDPV_ESIZE;
     if ((mode & SDPV_GEMM_SYRK_LOWER) && (M != N || config != -1)) return SDPV_EARG;
@@ -766,41 +716,44 @@ int sdpv_rns_gemm(const int8_t* Ah, const int8_t* Al, const int8_t* Bh, const in
 }
 
 int sdpv_rns_crt(const int16_t* acc, int64_t n, int64_t ld_g, int64_t rows, int64_t N, int64_t j0, int64_t w,
-                 const int32_t* mods, const double* minv, const int32_t* inv, const int8_t* Ct, const int32_t* Md,
-                 int64_t E, int64_t npad, int32_t mode, int32_t* W, int64_t* bitlen, int32_t* outm, int64_t* oute,
-                 int64_t L, int64_t plane, int64_t ldo, int64_t jo, int64_t Mb, const int64_t* erow,
-                 const int64_t* ecol, const int64_t* scol, int32_t* flag, void* stream) {
-    if (n < 1 || rows < 0 || N < 0 || j0 < 0 || w < 0 || E < 4 || E % 4 != 0 || npad < n || npad % 32 != 0)
+                 const int32_t* mods, const double* minv, const int32_t* inv, const int8_t* Ct, int64_t E,
+                 int64_t npad, int32_t mode, int32_t* W, int64_t* bitlen, int32_t* outm, int64_t* oute, int64_t L,
+                 int64_t plane, int64_t ldo, int64_t jo, int64_t Mb, const int64_t* erow, const int64_t* ecol,
+                 const int64_t* scol, int32_t* flag, void* stream) {
+    if (n < 1 || rows < 0 || N < 0 || j0 < 0 || w < 0 || E < 4 || E % 4 != 0 || npad <= n || npad % 32 != 0)
         return SDPV_EARG;
-    if (mode < 0 || mode > 2) return SDPV_EARG;
+    const int omode = mode & ~SDPV_CRT_SCALED;
+    if (omode < 0 || omode > 2) return SDPV_EARG;
     const i64 Bq = rows * w;
     if (Bq == 0) return SDPV_OK;
-    if (!acc || !mods || !minv || !inv || !Ct || !Md) return SDPV_EARG;
-    if (mode == SDPV_CRT_WORDS && (!W || !bitlen)) return SDPV_EARG;
-    if (mode != SDPV_CRT_WORDS && (!outm || L < 1 || Mb < 1)) return SDPV_EARG;
-    if (mode == SDPV_CRT_NORMALIZE && !oute) return SDPV_EARG;
-    if (mode == SDPV_CRT_ROUND && !scol) return SDPV_EARG;
+    if (!acc || !mods || !minv || !inv || !Ct) return SDPV_EARG;
+    if (omode == SDPV_CRT_WORDS && (!W || !bitlen)) return SDPV_EARG;
+    if (omode != SDPV_CRT_WORDS && (!outm || L < 1 || Mb < 1)) return SDPV_EARG;
+    if (omode == SDPV_CRT_NORMALIZE && !oute) return SDPV_EARG;
+    if (omode == SDPV_CRT_ROUND && !scol) return SDPV_EARG;
     if (npad > 1024) return SDPV_ESIZE;  // digit sums 128*Zh + Zl stay within int32
+    if ((rows - 1) * N + j0 + w > ld_g || ld_g >= (1LL << 31)) return SDPV_ESIZE;  // 32-bit plane offsets
     const i64 NW = E / 4;
-    const i64 Epad = 32 * ((E + 31) / 32);
-    const size_t cbytes = (size_t)Epad * 4 + (size_t)npad * 16;
-    const size_t wbytes = (size_t)64 * (npad + 16) + (mode != SDPV_CRT_WORDS ? (size_t)128 * (NW + 1) : 0);
-    int nwave = 4;
-    while (nwave > 1 && cbytes + nwave * wbytes > 160 * 1024) nwave >>= 1;
-    const size_t lds = cbytes + nwave * wbytes;
+    const size_t lds = (size_t)128 * (npad + 16) + (omode != SDPV_CRT_WORDS ? (size_t)256 * (NW + 1) : 0);
     if (lds > 160 * 1024) return SDPV_ESIZE;
-    const i64 per = 32 * (i64)nwave;
-    const i64 grid = (Bq + per - 1) / per;
+    const i64 grid = (Bq + 63) / 64;
     if (grid > 0x7fffffffLL) return SDPV_ESIZE;
     hipStream_t st = as_stream(stream);
-    static bool attr = false;
-    if (!attr) {
-        (void)hipFuncSetAttribute((const void*)k_rns_crt, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-        attr = true;
+    const bool scaled = (mode & SDPV_CRT_SCALED) != 0;
+    static bool attr[2] = {false, false};
+    if (!attr[scaled]) {
+        (void)hipFuncSetAttribute(scaled ? (const void*)k_rns_crt<true> : (const void*)k_rns_crt<false>,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+        attr[scaled] = true;
     }
-    hipLaunchKernelGGL(k_rns_crt, dim3((unsigned)grid), dim3(64 * nwave), lds, st, acc, (int)n, (i64)ld_g, (i64)rows,
-                       (i64)N, (i64)j0, (i64)w, mods, minv, inv, Ct, Md, (int)E, (int)npad, (int)mode, W, bitlen, outm,
-                       oute, (int)L, (i64)plane, (i64)ldo, (i64)jo, (i64)Mb, erow, ecol, scol, flag);
+    if (scaled)
+        hipLaunchKernelGGL((k_rns_crt<true>), dim3((unsigned)grid), dim3(64), lds, st, acc, (int)n, (i64)ld_g,
+                           (i64)rows, (i64)N, (i64)j0, (i64)w, mods, minv, inv, Ct, (int)E, (int)npad, omode, W,
+                           bitlen, outm, oute, (int)L, (i64)plane, (i64)ldo, (i64)jo, (i64)Mb, erow, ecol, scol, flag);
+    else
+        hipLaunchKernelGGL((k_rns_crt<false>), dim3((unsigned)grid), dim3(64), lds, st, acc, (int)n, (i64)ld_g,
+                           (i64)rows, (i64)N, (i64)j0, (i64)w, mods, minv, inv, Ct, (int)E, (int)npad, omode, W,
+                           bitlen, outm, oute, (int)L, (i64)plane, (i64)ldo, (i64)jo, (i64)Mb, erow, ecol, scol, flag);
     SDPV_TRY_LAUNCH();
     return SDPV_OK;
 }

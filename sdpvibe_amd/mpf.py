@@ -386,7 +386,7 @@ def gemm(A, B, transA=False, transB=False):
 
     for j0, jw, acc in residue_gemm(R, fa, fb, nb, M, N, K, ACC_BUDGET):
         R.crt(acc, nb * M, jw, 0, jw, 1, outm=out_m, oute=out_e, L=L, plane=nb * M * N, ldo=N, jo=j0, Mb=M,
-              erow=EA, ecol=EB)
+              erow=EA, ecol=EB, scaled=True)
     return MPF(out_m, out_e)
 
 
@@ -538,7 +538,7 @@ class GramAcc:
             Rl = torch.empty_like(Rh)
             # rows of the split = columns j of P (stride 1), inner index = rows of P (stride N)
             R.split(Pm, Pe, Erow, L, rows * N, 1, N, 0, 1, N, k0, kk, kp, Rh, Rl)
-            R.product(Rh, Rl, Rh, Rl, self.acc, 1, N, N, N * N, accumulate=True, syrk=True)
+            R.product(Rh, Rl, Rh, Rl, self.acc, 1, N, N, N * N, accumulate=True, syrk=True, scaled=True)
             del Rh, Rl
 
     def _crt(self):
@@ -547,7 +547,7 @@ class GramAcc:
         out_e = torch.empty((N, N), dtype=torch.int64, device=dev)
         rE = torch.where(self.E > EZ // 2, self.E - NBITS(L), torch.zeros_like(self.E)).contiguous()
         R.crt(self.acc, N, N, 0, N, 1, outm=out_m, oute=out_e, L=L, plane=N * N, ldo=N, jo=0, Mb=N, erow=rE,
-              ecol=rE)
+              ecol=rE, scaled=True)
         return MPF(out_m, out_e)
 
     def _flush(self):

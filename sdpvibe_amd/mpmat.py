@@ -185,8 +185,9 @@ def ctx(L):
 
 
 def residue_gemm(R, A_desc, B_desc, nb, M, N, K, acc_rows_budget=ACC_BUDGET):
-    """Generator of (j0, jw, acc) panels: acc (n, nb*M, jw) int16 holding the residues of the exact
-    products over the whole inner dimension K. A_desc/B_desc: callables (k0, kc, kp, j0, jw) ->
+    """Generator of (j0, jw, acc) panels: acc (n, nb*M, jw) int16 holding the CRT coefficients
+    (centred residue * (M/m)^-1 mod m, for crt(..., scaled=True)) of the exact products over the
+    whole inner dimension K. A_desc/B_desc: callables (k0, kc, kp, j0, jw) ->
     (Rh, Rl) residues (n, nb*rows, kp) of op(A) rows / op(B)^T rows j0..j0+jw."""
     n = R.n
     w = N if n * nb * M * N * 2 <= acc_rows_budget else max(64, acc_rows_budget // (n * nb * M * 2) // 64 * 64)
@@ -204,7 +205,7 @@ def residue_gemm(R, A_desc, B_desc, nb, M, N, K, acc_rows_budget=ACC_BUDGET):
             kp = R.kp(kk)
             Ah, Al = A_desc(k0, kk, kp, j0, jw)
             Bh, Bl = B_desc(k0, kk, kp, j0, jw)
-            R.product(Ah, Al, Bh, Bl, acc, nb, M, jw, nb * M * jw, accumulate=not first)
+            R.product(Ah, Al, Bh, Bl, acc, nb, M, jw, nb * M * jw, accumulate=not first, scaled=True)
             first = False
             del Ah, Al, Bh, Bl
         yield j0, jw, acc
@@ -271,10 +272,10 @@ def _crt_round_panel(R, acc, M, jw, j0, N, e_in, out_exp, L, out, exps, fixed):
     exponents."""
     if fixed:
         s = torch.full((N,), out_exp - e_in, dtype=torch.int64, device=dev)
-        R.crt(acc, M, jw, 0, jw, 2, outm=out, L=L, plane=M * N, ldo=N, jo=j0, Mb=M, scol=s,
+        R.crt(acc, M, jw, 0, jw, 2, outm=out, L=L, plane=M * N, ldo=N, jo=j0, Mb=M, scol=s, scaled=True,
               flag=_flag())
         return
-    Wd, bl, Bq = R.crt_block(acc, jw, 0, jw, M)
+    Wd, bl, Bq = R.crt_block(acc, jw, 0, jw, M, scaled=True)
     if isinstance(out_exp, str):
         sj = bl.view(M, jw).amax(0) - headroom_bits(L)
         ej = e_in[j0:j0 + jw] + sj

@@ -7,7 +7,9 @@ An exact product C = A @ Bt^T of L-limb integers is computed as:
       r = 128*h + l (h, l int8), as a byte dot product with a table of 256^j mod m on INT8 MFMA;
    2. products: per prime, three INT8 GEMMs (HH = h.h, SS = (h+l).(h+l), LL = l.l; h + l fits
       int8 because |r| <= 8128), combined as 16256*HH + 128*SS - 127*LL mod m;
-   3. CRT: y_g = r_g (M/m_g)^-1 mod m_g, Z = Y . digits(M/m_g) on INT8 MFMA, carry to words.
+   3. CRT: y_g = r_g (M/m_g)^-1 mod m_g (centred; the products can store it directly, their
+      coefficients scaled by (M/m_g)^-1), k = round(sum y_g/m_g), Z = [Y, -k] . [digits(M/m_g),
+      digits(M)] on INT8 MFMA, carry to words.
 Prime count n: the smallest with M > 4 * kmax * 2^(64L-2), so every product with inner dimension
 <= kmax is reconstructed exactly and |C| < M/4.
 
@@ -143,7 +145,7 @@ class RNS:
             raise ValueError(f"L={L}: not enough primes below {PMAX}")
         self.mods, self.M, self.n = mods, M, len(mods)
         n = self.n
-        self.npad = -(-n // 32) * 32
+        self.npad = -(-(n + 1) // 32) * 32                 # column n of Ct: digits of M
         self.NBp = 32 * (-(-4 * L // 32))
         t = lambda a, dt: torch.tensor(a, dtype=dt, device=device)
         self.m32 = t(mods, torch.int32)
@@ -159,16 +161,21 @@ class RNS:
         self.C1 = t([(128 * int(T[i].sum())) % m for i, m in enumerate(mods)], torch.int32)
         self.C2 = t([pow(2, 32 * L, m) for m in mods], torch.int32)
         self.kcoef = t([[16256 % m, 128 % m, (-127) % m] for m in mods], torch.int32).contiguous()
+        inv = [pow(M // m % m, -1, m) for m in mods]
+        # coefficients scaled by (M/m)^-1: the product then stores the CRT's y_g directly
+        self.kcoef_crt = t([[16256 * v % m, 128 * v % m, (-127) * v % m] for m, v in zip(mods, inv)],
+                           torch.int32).contiguous()
         # CRT in base 256
         self.E = (math.ceil(M.bit_length() / 8) + 2 + 7) // 8 * 8
         self.Epad = -(-self.E // 32) * 32
-        self.inv32 = t([pow(M // m % m, -1, m) for m in mods], torch.int32)
+        self.inv32 = t(inv, torch.int32)
         Ct = np.zeros((self.Epad, self.npad), np.int64)
         for i, m in enumerate(mods):
             Ct[:self.E, i] = _sdigits(M // m, 256, self.E)
-        self.Ct = t(Ct, torch.int8).contiguous()
         Md = np.zeros(self.Epad, np.int64)
         Md[:self.E] = _sdigits(M, 256, self.E)
+        Ct[:, n] = Md                                      # the CRT subtracts k*M with the digit products
+        self.Ct = t(Ct, torch.int8).contiguous()
         self.Md = t(Md, torch.int32)
 
     # ---- step 1: residues of rows x kc entries (limbs) -> (G, rows, kp) hi, lo
@@ -214,8 +221,11 @@ class RNS:
              ptr(self.minv), g0, G, ptr(Rh), ptr(Rl), ptr(flag), stream())
 
     # ---- step 2: accumulate residues of A @ Bt^T
-    def product(self, Ah, Al, Bh, Bl, acc, nb, M, N, ld_g, g0=0, accumulate=False, syrk=False, config=None):
-        """residues in the kernels' chunk-major layout (G, kp/16, rows, 16)"""
+    def product(self, Ah, Al, Bh, Bl, acc, nb, M, N, ld_g, g0=0, accumulate=False, syrk=False, config=None,
+                scaled=False):
+        """residues in the kernels' chunk-major layout (G, kp/16, rows, 16). scaled: store the CRT
+        coefficients y_g = centred(residue * (M/m_g)^-1 mod m_g) instead of residues in [0, m)
+        (crt(..., scaled=True) then skips that step)."""
         if Ah.dim() != 4:
             raise ValueError("product: residues must be in the chunk-major layout (G, kp/16, rows, 16)")
         G, kp = Ah.shape[0], Ah.shape[1] * 16
@@ -225,11 +235,12 @@ class RNS:
                                  f"got {tuple(X.shape)}")
         if g0 < 0 or g0 + G > self.n or ld_g < nb * M * N or acc.numel() < (g0 + G - 1) * ld_g + nb * M * N:
             raise ValueError("product: accumulator too small for the primes and output block")
-        mode = (1 if accumulate else 0) | (2 if syrk else 0)
+        mode = (1 if accumulate else 0) | (2 if syrk else 0) | (4 if scaled else 0)
         if config is None:
             config = -1 if syrk else tuned_config(self, nb, M, N, kp)
         call("sdpv_rns_gemm", ptr(Ah), ptr(Al), ptr(Bh), ptr(Bl), G, nb, M, N, kp, ptr(acc), ld_g,
-             ptr(self.m32), ptr(self.minv), ptr(self.kcoef), g0, mode, config, stream())
+             ptr(self.m32), ptr(self.minv), ptr(self.kcoef_crt if scaled else self.kcoef), g0, mode, config,
+             stream())
 
     def accumulate(self, Ahi, Alo, Bhi, Blo, acc, g0=0):
         """acc[g0:g0+G] += residues of A @ B^T for the G moduli whose residues are given in the
@@ -242,7 +253,7 @@ class RNS:
 
     # ---- step 3: CRT of a block of entries
     def crt(self, acc, rows, N, j0, w, mode, ld_g=None, W=None, bitlen=None, outm=None, oute=None, L=None,
-            plane=0, ldo=0, jo=0, Mb=1, erow=None, ecol=None, scol=None, flag=None):
+            plane=0, ldo=0, jo=0, Mb=1, erow=None, ecol=None, scol=None, flag=None, scaled=False):
         ld_g = acc.stride(0) if ld_g is None else ld_g
         if acc.numel() < (self.n - 1) * ld_g + (rows - 1) * N + j0 + w or j0 + w > N:
             raise ValueError("crt: block exceeds the residue accumulator")
@@ -253,17 +264,17 @@ class RNS:
                                  + jo + w):
             raise ValueError("crt: limb output too small")
         call("sdpv_rns_crt", ptr(acc), self.n, ld_g, rows, N, j0, w, ptr(self.m32), ptr(self.minv),
-             ptr(self.inv32), ptr(self.Ct), ptr(self.Md), self.E, self.npad, mode, ptr(W), ptr(bitlen),
+             ptr(self.inv32), ptr(self.Ct), self.E, self.npad, mode | (4 if scaled else 0), ptr(W), ptr(bitlen),
              ptr(outm), ptr(oute), L or self.L, plane, ldo, jo, Mb, ptr(erow), ptr(ecol), ptr(scol),
              ptr(flag), stream())
 
-    def crt_block(self, acc, N, j0, w, rows):
-        """acc: (n, rows_total, N) int16 residues in [0, m); block = rows < `rows`, columns
-        j0..j0+w; entries q = i*w + jj. Returns (W (E/4, Bq) int32 words, bitlen (Bq,), Bq)
-        (rns.py:227-245)."""
+    def crt_block(self, acc, N, j0, w, rows, scaled=False):
+        """acc: (n, rows_total, N) int16 residues in [0, m) (scaled: the product's CRT coefficients);
+        block = rows < `rows`, columns j0..j0+w; entries q = i*w + jj. Returns (W (E/4, Bq) int32
+        words, bitlen (Bq,), Bq) (rns.py:227-245)."""
         Bq = rows * w
         acc = acc.contiguous()
         W = torch.empty((self.E // 4, Bq), dtype=torch.int32, device=acc.device)
         bl = torch.empty((Bq,), dtype=torch.int64, device=acc.device)
-        self.crt(acc, rows, N, j0, w, 0, W=W, bitlen=bl)
+        self.crt(acc, rows, N, j0, w, 0, W=W, bitlen=bl, scaled=scaled)
         return W, bl, Bq

@@ -53,6 +53,15 @@ long long emu_shfl_xor64(long long v, int mask);
 inline long long __shfl_xor(long long v, int mask, int width = 64) { (void)width; return emu_shfl_xor64(v, mask); }
 inline long __shfl_xor(long v, int mask, int width = 64) { (void)width; return (long)emu_shfl_xor64(v, mask); }
 inline int __shfl_xor(int v, int mask, int width = 64) { (void)width; return (int)emu_shfl_xor64(v, mask); }
+inline float __shfl_xor(float v, int mask, int width = 64) {
+    (void)width;
+    int b;
+    std::memcpy(&b, &v, 4);
+    b = (int)emu_shfl_xor64(b, mask);
+    float r;
+    std::memcpy(&r, &b, 4);
+    return r;
+}
 inline double __shfl_xor(double v, int mask, int width = 64) {
     (void)width;
     long long b;
@@ -61,6 +70,14 @@ inline double __shfl_xor(double v, int mask, int width = 64) {
     double r;
     std::memcpy(&r, &b, 8);
     return r;
+}
+struct emu_u2 {
+    unsigned v[2];
+    unsigned operator[](int i) const { return v[i]; }
+};
+inline emu_u2 __builtin_amdgcn_permlane32_swap(unsigned x, unsigned y, bool, bool) {
+    const unsigned tx = (unsigned)emu_shfl_xor64((long long)x, 32), ty = (unsigned)emu_shfl_xor64((long long)y, 32);
+    return (threadIdx.x & 63) < 32 ? emu_u2{{x, tx}} : emu_u2{{ty, y}};
 }
 void emu_wave_barrier();
 unsigned long long emu_ballot(int pred);

@@ -151,6 +151,9 @@ def crt_cases():
         sel = acc[:, :, start:start + width].reshape(R.n, -1)
         rW, rbl = O.crt(sel, np.array(R.mods), R.E // 4)
         good = np.array_equal(W.numpy(), rW) and np.array_equal(bl.numpy(), rbl) and fp(W.numpy()) == exp
+        from tests.test_gpu_kernels import scaled_residues
+        W2, bl2, _ = R.crt_block(torch.from_numpy(scaled_residues(R, acc)), N, start, width, rows, scaled=True)
+        good &= np.array_equal(W2.numpy(), rW) and np.array_equal(bl2.numpy(), rbl)
         print(f"crt_block L={L} n={R.n}: {'ok' if good else 'MISMATCH'}", flush=True)
         ok &= good
     return ok

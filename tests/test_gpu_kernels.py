@@ -181,6 +181,16 @@ def test_crt_block(L, width):
     _eq(host(W), rW, "crt words")
     _eq(host(bl), rbl, "crt bitlen")
     assert from_planes(host(W)) == exp
+    # the same integers from the product's CRT-ready form y = centred(residue * (M/m)^-1 mod m)
+    W2, bl2, _ = R.crt_block(cu(scaled_residues(R, acc)), N, start, width, rows, scaled=True)
+    _eq(host(W2), rW, "crt words (scaled input)")
+    _eq(host(bl2), rbl, "crt bitlen (scaled input)")
+
+
+def scaled_residues(R, acc):
+    m = np.array(R.mods, np.int64)[:, None, None]
+    y = acc.astype(np.int64) * np.array([int(v) for v in R.inv32.cpu()], np.int64)[:, None, None] % m
+    return np.where(y > m // 2, y - m, y).astype(np.int16)
 
 
 def test_product_accumulate():
@@ -228,10 +238,20 @@ def product_configs_case(dev_put, M=300, N=200, K=130, nb=2, L=10):
     A = [chunk_major(dev_put(x.astype(np.int8))) for x in (ha, la)]
     B = [chunk_major(dev_put(x.astype(np.int8))) for x in (hb, lb)]
     ref = np.einsum("gbik,gbjk->gbij", ra, rb) % m
+    inv = np.array([int(v) for v in R.inv32.cpu()], np.int64)[:, None, None, None]
+    yref = ref * inv % m
+    yref = np.where(yref > m // 2, yref - m, yref)
     for cfg in GEMM_CONFIGS:
         acc = dev_put(np.zeros((G, nb * M, N), np.int16))
         R.product(A[0], A[1], B[0], B[1], acc, nb, M, N, nb * M * N, config=cfg)
         _eq(acc.cpu().numpy().astype(np.int64).reshape(G, nb, M, N), ref, f"config {cfg}")
+        # CRT-ready output: centred residue * (M/m)^-1, then accumulated once more (2y)
+        R.product(A[0], A[1], B[0], B[1], acc, nb, M, N, nb * M * N, config=cfg, scaled=True)
+        _eq(acc.cpu().numpy().astype(np.int64).reshape(G, nb, M, N), yref, f"config {cfg} scaled")
+        R.product(A[0], A[1], B[0], B[1], acc, nb, M, N, nb * M * N, config=cfg, scaled=True, accumulate=True)
+        y2 = 2 * yref % m
+        _eq(acc.cpu().numpy().astype(np.int64).reshape(G, nb, M, N), np.where(y2 > m // 2, y2 - m, y2),
+            f"config {cfg} scaled accumulate")
     # syrk: A = B, lower tiles (tile size by N as the reference: 128 here)
     acc = dev_put(np.zeros((G, M, M), np.int16))
     A1 = [chunk_major(dev_put(x.astype(np.int8)[:, :M])) for x in (ha, la)]

@@ -66,6 +66,11 @@ def test_tables(ctxs):
             assert sum(int(d) << (8 * k) for k, d in enumerate(dig)) == R.M // m
         md = R.Md.numpy().astype(np.int64)
         assert sum(int(d) << (8 * k) for k, d in enumerate(md)) == R.M
+        assert R.npad > R.n and (R.Ct.numpy()[:, R.n] == md).all() and (R.Ct.numpy()[:, R.n + 1:] == 0).all()
+        kc, ks = R.kcoef.numpy().astype(np.int64), R.kcoef_crt.numpy().astype(np.int64)
+        for gi in (0, R.n - 1):
+            m, v = R.mods[gi], int(R.inv32[gi])
+            assert [int(x) for x in ks[gi]] == [int(x) * v % m for x in kc[gi]]
         assert R.E % 8 == 0 and 8 * R.E >= R.M.bit_length() + 16
 
 
@@ -110,9 +115,16 @@ def test_pipeline_model_against_integers(ctxs):
     # CRT: y = acc * inv mod m (centred), k = round(sum y/m), Z = digits . y, carry
     y = (acc * R.inv32.numpy().astype(np.int64)[:, None, None]) % m
     y = np.where(y > m // 2, y - m, y)
+    # the scaled coefficients give y directly (the product's CRT-ready output)
+    ks = R.kcoef_crt.numpy().astype(np.int64)
+    ys = (ks[:, 0, None, None] * HH + ks[:, 1, None, None] * SS + ks[:, 2, None, None] * LL) % m
+    assert (np.where(ys > m // 2, ys - m, ys) == y).all()
     k = np.floor((y / m).sum(0) + 0.5).astype(np.int64)
-    Ct = R.Ct.numpy()[:R.E, :R.n].astype(np.int64)
-    Z = np.einsum("dg,gij->dij", Ct, y) - k[None] * R.Md.numpy()[:R.E, None, None].astype(np.int64)
+    assert (np.abs(k) <= R.n // 2).all()
+    # -k in column n of Y against the digits of M in column n of Ct
+    Yx = np.concatenate([y, -k[None]], 0)
+    Ct = R.Ct.numpy()[:R.E, :R.n + 1].astype(np.int64)
+    Z = np.einsum("dg,gij->dij", Ct, Yx)
     for i in range(M_):
         for j in range(N_):
             v = sum(int(Z[d, i, j]) << (8 * d) for d in range(R.E))
