@@ -68,6 +68,7 @@ __device__ __forceinline__ u32 pack4(int a, int b, int c, int d) {  // low bytes
     return __builtin_amdgcn_perm(cd, ab, 0x05040100u);
 }
 
+template <int KSC>   // table chunks of 32 bytes: KSC = NBp / 32 (0: any, read from NBp)
 __global__ void __launch_bounds__(256) k_rns_split(
     const i32* __restrict__ X, const i64* __restrict__ e, const i64* __restrict__ Erow, int L, int Lout, i64 plane,
     i64 nb, i64 rows, i64 ldb, i64 ldr, i64 ldk, i64 k0, i64 kc, i64 kp, const int8_t* __restrict__ Thi,
@@ -149,7 +150,7 @@ __global__ void __launch_bounds__(256) k_rns_split(
     const u64 negmask = __ballot(negs[wave * 64 + lane] != 0);
     const unsigned char* xw = xs + (size_t)wave * 64 * XS;
     const int l32 = lane & 31, hsel = lane >> 5;
-    const int KS = NBp / 32, nPB = (G + 31) / 32;
+    const int KS = KSC ? KSC : NBp / 32, nPB = (G + 31) / 32;
     const i64 nck = kp >> 4;
     // prime-block operands: the first four 32-byte table chunks and the constants, loaded once per
     // block of primes and shared by both 32-slot halves
@@ -183,7 +184,7 @@ __global__ void __launch_bounds__(256) k_rns_split(
         const int8_t* th = Thi + (i64)g * NBp + 16 * hsel;
         const int8_t* tl = Tlo + (i64)g * NBp + 16 * hsel;
         const int m = cur.m;
-        const double mi = cur.mi;
+        const double mi = cur.mi, md = (double)m;
         const int c1 = cur.c1, c2 = cur.c2;
         // both 32-slot blocks against the same table chunks: four independent MFMA chains
         v16i ah[2] = {zero16(), zero16()}, al[2] = {zero16(), zero16()};
@@ -218,11 +219,10 @@ __global__ void __launch_bounds__(256) k_rns_split(
                 for (int u = 0; u < 4; u++) {
                     const int j = 4 * jq + u;
                     const int nb_ = (int)((negmask >> (s0 + u)) & 1ull);
-                    const int x = ah[eb][j] * 128 + al[eb][j] + c1 - (nb_ ? c2 : 0);
-                    const int c = x - (int)rint((double)x * mi) * m;      // centred residue
-                    const int tt = c + 64;
-                    hi[u] = tt >> 7;
-                    lo[u] = (tt & 127) - 64;
+                    const double x = (double)(ah[eb][j] * 128 + al[eb][j] + c1 - (nb_ ? c2 : 0));
+                    const int c = (int)fma(-rint(x * mi), md, x);        // centred residue
+                    hi[u] = (c + 64) >> 7;
+                    lo[u] = c - (hi[u] << 7);
                 }
                 dh[eb][jq] = pack4(hi[0], hi[1], hi[2], hi[3]);
                 dl[eb][jq] = pack4(lo[0], lo[1], lo[2], lo[3]);
@@ -602,7 +602,7 @@ __global__ void __launch_bounds__(64) k_rns_crt(
         }
     }
 
-    // ---- phase 3 (lane = entry; reads only its own LDS row)
+    // ---- phase 3 (lane = entry; reads only its own LDS row, in batches of independent reads)
     if (!qok) return;
     const i64 i = q / w, jj = q % w;
     if (mode == SDPV_CRT_WORDS) {
@@ -614,23 +614,89 @@ __global__ void __launch_bounds__(64) k_rns_crt(
     }
     const i64 o = i * ldo + jo + jj;
     const i64 cidx = (i / Mb) * ldo + jo + jj;
-    LdsWords src{wrow};
+    const u32 ext = (wrow[NW - 1] >> 31) ? 0xffffffffu : 0u;   // sign extension word
+    // word t of the value's two's complement: zero below 0, the sign extension above NW - 1
+    auto wd = [&](int t) -> u32 {
+        const u32 v = wrow[t < 0 ? 0 : (t >= NW ? NW - 1 : t)];
+        return t < 0 ? 0u : (t >= NW ? ext : v);
+    };
+    i64 sv;                                           // shift: out = (v + rounding) / 2^sv
+    u32 c = 0;                                        // rounding carry into bit sv
     if (mode == SDPV_CRT_NORMALIZE) {
-        MagView<LdsWords> mag(src, NW);
-        const int bl = bitlen_mag(mag, NW);
+        // bit length of |v|: top word T of v ^ ext; for v < 0 also the lowest nonzero word z, as
+        // |v| = ~v above z, -v_z at z (mpf.py normalize via bitlen of the magnitude)
+        const bool neg = ext != 0u;
+        int T = -1;
+        u32 wT = 0;
+        for (int b = NW - 1; b >= 0 && T < 0; b -= 8) {
+            u32 x[8];
+#pragma unroll
+            for (int u = 0; u < 8; u++) x[u] = (b - u >= 0) ? wd(b - u) ^ ext : 0u;
+#pragma unroll
+            for (int u = 7; u >= 0; u--)
+                if (x[u] != 0u) { T = b - u; wT = x[u]; }
+        }
+        int z = -1;
+        u32 vz = 0;
+        if (neg) {
+            for (int b = 0; b < NW && z < 0; b += 8) {
+                u32 x[8];
+#pragma unroll
+                for (int u = 0; u < 8; u++) x[u] = wd(b + u);
+#pragma unroll
+                for (int u = 7; u >= 0; u--)
+                    if (x[u] != 0u && b + u < NW) { z = b + u; vz = x[u]; }
+            }
+        }
+        int bl;
+        if (!neg) bl = T < 0 ? 0 : 32 * T + 32 - clz32(wT);
+        else bl = T > z ? 32 * T + 32 - clz32(wT) : 32 * z + 32 - clz32(0u - vz);
         if (bl == 0) {
             for (int k = 0; k < L; k++) outm[(i64)k * plane + o] = 0;
             oute[o] = EZV;
             return;
         }
-        const int sh = bl - (32 * L - 2);
-        NegStream ng(mag.neg);
-        shift_mag(mag, NW, sh, L, [&](int k, u32 v) { outm[(i64)k * plane + o] = (i32)ng(v); });
+        sv = bl - (32 * L - 2);
+        if (sv > 0) {
+            // -round_half_up(|v| / 2^sv) = floor((v + 2^(sv-1) - 1) / 2^sv) for v < 0: the carry into
+            // bit sv needs bit sv-1 and, for v < 0, a nonzero bit below it
+            const u32 rb = (wd((int)((sv - 1) >> 5)) >> (int)((sv - 1) & 31)) & 1u;
+            const bool sticky = !neg || 32 * (i64)z + __builtin_ctz(vz) < sv - 1;
+            c = (rb && sticky) ? 1u : 0u;
+        }
         const i64 ein = (erow ? erow[i] : 0) + (ecol ? ecol[cidx] : 0);
-        oute[o] = ein + sh;
+        oute[o] = ein + sv;
     } else {
-        const int bad = shr_round(src, NW, scol[cidx], L, [&](int k, u32 v) { outm[(i64)k * plane + o] = (i32)v; });
-        if (bad && flag) atomicOr(flag, 1);
+        // round_half_up(v / 2^sv) = floor((v + 2^(sv-1)) / 2^sv) (limbops._shift_round_kernel); shifts
+        // beyond the value or the output give the same words as these clamped ones
+        sv = scol[cidx];
+        const i64 smax = 32 * (i64)NW + 32, smin = -32 * ((i64)L + 2);
+        sv = sv > smax ? smax : (sv < smin ? smin : sv);
+        if (sv > 0) c = (wd((int)((sv - 1) >> 5)) >> (int)((sv - 1) & 31)) & 1u;
+    }
+    // out word k = bits [32k + sv, 32k + sv + 32) of v, plus the carry; sv < 0 shifts left
+    const int qs = (int)(sv >> 5), r = (int)(sv & 31);
+    u32 prev = 0;
+    for (int k0 = 0; k0 < L; k0 += 8) {
+        u32 x[9];
+#pragma unroll
+        for (int u = 0; u < 9; u++) x[u] = wd(qs + k0 + u);
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+            if (k0 + u < L) {
+                const u32 d = (u32)((((u64)x[u + 1] << 32) | x[u]) >> r);
+                const u32 t = d + c;
+                c = t < d ? 1u : 0u;
+                outm[(i64)(k0 + u) * plane + o] = (i32)t;
+                prev = t;
+            }
+        }
+    }
+    if (mode == SDPV_CRT_ROUND) {
+        // overflow: word L of the rounded value differs from the sign extension of word L - 1
+        const u32 d = (u32)((((u64)wd(qs + L + 1) << 32) | wd(qs + L)) >> r);
+        const u32 want = (prev >> 31) ? 0xffffffffu : 0u;
+        if (d + c != want && flag) atomicOr(flag, 1);
     }
 }
 
@@ -674,15 +740,28 @@ int sdpv_rns_split(const int32_t* X, const int64_t* e, const int64_t* Erow, int6
     const int stage = lds + stage_bytes <= 160 * 1024;   // stage the limbs when they fit
     if (stage) lds += stage_bytes;
     if (lds > 160 * 1024) return SDPV_ESIZE;
-    static bool attr_set = false;
-    if (!attr_set) {
-        (void)hipFuncSetAttribute((const void*)k_rns_split, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-        attr_set = true;
+    static bool attr_set[5] = {false, false, false, false, false};
+    const int ks = NBp / 32 <= 4 ? (int)(NBp / 32) : 0;
+#define SDPV_SPLIT_LAUNCH(KSC)                                                                                 \
+    do {                                                                                                       \
+        if (!attr_set[KSC]) {                                                                                  \
+            (void)hipFuncSetAttribute((const void*)k_rns_split<KSC>, hipFuncAttributeMaxDynamicSharedMemorySize, \
+                                      160 * 1024);                                                             \
+            attr_set[KSC] = true;                                                                              \
+        }                                                                                                      \
+        hipLaunchKernelGGL((k_rns_split<KSC>), dim3((unsigned)grid), dim3(256), lds, as_stream(stream), X, e,   \
+                           Erow, (int)L, (int)Lout, (i64)plane, (i64)nb, (i64)rows, (i64)ldb, (i64)ldr, (i64)ldk, \
+                           (i64)k0, (i64)kc, (i64)kp, Thi, Tlo, (int)NBp, C1, C2, mods, minv, (int)g0, (int)G, Rh, \
+                           Rl, flag, ntiles_k, stage);                                                         \
+    } while (0)
+    switch (ks) {
+        case 1: SDPV_SPLIT_LAUNCH(1); break;
+        case 2: SDPV_SPLIT_LAUNCH(2); break;
+        case 3: SDPV_SPLIT_LAUNCH(3); break;
+        case 4: SDPV_SPLIT_LAUNCH(4); break;
+        default: SDPV_SPLIT_LAUNCH(0); break;
     }
-    hipLaunchKernelGGL(k_rns_split, dim3((unsigned)grid), dim3(256), lds, as_stream(stream), X, e, Erow, (int)L,
-                       (int)Lout, (i64)plane, (i64)nb, (i64)rows, (i64)ldb, (i64)ldr, (i64)ldk, (i64)k0, (i64)kc,
-                       (i64)kp, Thi, Tlo, (int)NBp, C1, C2, mods, minv, (int)g0, (int)G, Rh, Rl, flag, ntiles_k,
-                       stage);
+#undef SDPV_SPLIT_LAUNCH
     SDPV_TRY_LAUNCH();
     return SDPV_OK;
 }

@@ -252,6 +252,17 @@ def carry_cases():
 CASES["carry"] = carry_cases
 
 
+def small_round_cases():
+    from tests.test_gpu_kernels import small_and_round_case
+    for L in (1, 2, 24):
+        small_and_round_case(L)
+    print("small values and rounding: ok", flush=True)
+    return True
+
+
+CASES["small_round"] = small_round_cases
+
+
 if __name__ == "__main__":  # noqa: E305
     ok = True
     for c in sys.argv[1:] or list(CASES):

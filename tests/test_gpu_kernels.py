@@ -284,6 +284,40 @@ def test_gemm_carry_edges(L):
     carry_edge_case(cu, L)
 
 
+def small_and_round_case(L):
+    """products that normalise by a left shift (1, -1, -2^k), zero, and the fixed-exponent rounding
+    (round half up of the two's complement value) at right, left and out-of-range shifts"""
+    from tests.intutil import norm_closed_form, rnd_half_up, wrap
+    NB = 32 * L - 2
+    a = [[1, 0, 0], [-1, 0, 0], [0, 0, 0], [-(1 << (NB - 1)), 3, 0], [5, -7, 1 << (NB - 3)]]
+    b = [[1, 0, 0], [1 << 5, 0, 0], [-3, 5, -1], [1 << (NB - 2), 1, -1]]
+    exact = [[sum(x * y for x, y in zip(ra, rb)) for rb in b] for ra in a]
+    A = mpmat.from_ints(a, 0, L)
+    Bt = mpmat.from_ints(b, 0, L)
+    # normalisation: value = m * 2^e with |m| of NB bits (zero: EZ)
+    C = F.gemm(F.MPF(A.limbs.contiguous(), torch.zeros((5, 3), dtype=torch.int64, device=A.limbs.device)),
+               F.MPF(Bt.limbs.permute(0, 2, 1).contiguous(),
+                     torch.zeros((3, 4), dtype=torch.int64, device=A.limbs.device)))
+    cm, ce = C.m.cpu().numpy(), C.e.cpu().numpy()
+    got = from_planes(cm.reshape(L, -1))
+    for idx, v in enumerate(v for row in exact for v in row):
+        e = int(ce.reshape(-1)[idx])
+        if v == 0:
+            assert got[idx] == 0 and e == EZ
+        else:
+            assert (got[idx], e) == norm_closed_form(v, NB), (idx, v)
+    for s in (0, 1, 31, 33, NB, 64 * L + 40, -1, -33):
+        D = mpmat.gemm_nt(A, Bt, out_exp=s)
+        dv = from_planes(D.limbs.cpu().numpy().reshape(L, -1))
+        for idx, v in enumerate(v for row in exact for v in row):
+            assert dv[idx] == wrap(rnd_half_up(v, s), L), (s, idx)
+
+
+@pytest.mark.parametrize("L", [1, 2, 24])
+def test_small_values_and_rounding(L):
+    small_and_round_case(L)
+
+
 def test_product_configs():
     product_configs_case(cu)
 
