@@ -487,7 +487,7 @@ __global__ void __launch_bounds__(64) k_rns_crt(
     const u32 boff = qok ? (u32)(((q / w) * N + j0 + q % w) * 2) : 0u;
 
     // ---- phase 1
-    double s = 0.0;
+    double sp[4] = {0.0, 0.0, 0.0, 0.0};             // four independent partial sums of y_g / m_g
     int8_t* yhr = Yh + (size_t)lane * YS;
     int8_t* ylr = Yl + (size_t)lane * YS;
     for (int gc = 0; gc < npad; gc += 16) {
@@ -511,7 +511,7 @@ __global__ void __launch_bounds__(64) k_rns_crt(
                 const double x = (double)y * (double)inv[g];
                 y = (int)fma(-rint(x * minv[g]), (double)mods[g], x);
             }
-            s = fma((double)y, minv[g], s);
+            sp[u & 3] = fma((double)y, minv[g], sp[u & 3]);
             hi[u] = (y + 64) >> 7;
             lo[u] = y - (hi[u] << 7);
         }
@@ -525,7 +525,7 @@ __global__ void __launch_bounds__(64) k_rns_crt(
         *(v4i*)(ylr + gc) = vl;
     }
     {                                                 // column n: -k = 128*hi + lo (|k| <= n/2)
-        const int nk = -(int)rint(s);
+        const int nk = -(int)rint((sp[0] + sp[1]) + (sp[2] + sp[3]));
         const int kl = ((nk + 64) & 127) - 64;
         yhr[n] = (int8_t)((nk - kl) >> 7);
         ylr[n] = (int8_t)kl;
@@ -541,25 +541,38 @@ __global__ void __launch_bounds__(64) k_rns_crt(
     i64 carry = 0;
     int last0 = -1, last1 = -1;
     u32 w0 = 0, w1 = 0, lastw = 0;
-    for (int db = 0; db < nDB; db++) {
+    // the first KF 32-byte digit-table chunks of a digit block live in registers, the next
+    // block's loaded while this one is folded and carried
+    constexpr int KF = 4;
+    v4i cfc[KF], cfn[KF];
+    auto load_chunks = [&](int db, v4i* cf) {
         const int8_t* crow = Ct + (i64)(db * 32 + l32) * npad + 16 * half;
-        const v4i c0 = *(const v4i*)crow;
-        v16i zh0 = mfma(c0, *(const v4i*)y0h, zero16()), zl0 = mfma(c0, *(const v4i*)y0l, zero16());
-        v16i zh1 = mfma(c0, *(const v4i*)y1h, zero16()), zl1 = mfma(c0, *(const v4i*)y1l, zero16());
-        for (int kb = 1; kb < nK; kb += 4) {
-            v4i cf[4];
 #pragma unroll
-            for (int u = 0; u < 4; u++) cf[u] = (kb + u < nK) ? *(const v4i*)(crow + (kb + u) * 32) : zero4();
+        for (int u = 0; u < KF; u++) cf[u] = u < nK ? *(const v4i*)(crow + u * 32) : zero4();
+    };
+    load_chunks(0, cfc);
+    for (int db = 0; db < nDB; db++) {
+        if (db + 1 < nDB) load_chunks(db + 1, cfn);
+        v16i zh0 = mfma(cfc[0], *(const v4i*)y0h, zero16()), zl0 = mfma(cfc[0], *(const v4i*)y0l, zero16());
+        v16i zh1 = mfma(cfc[0], *(const v4i*)y1h, zero16()), zl1 = mfma(cfc[0], *(const v4i*)y1l, zero16());
 #pragma unroll
-            for (int u = 0; u < 4; u++) {
-                if (kb + u < nK) {
-                    const int o = (kb + u) * 32;
-                    zh0 = mfma(cf[u], *(const v4i*)(y0h + o), zh0);
-                    zl0 = mfma(cf[u], *(const v4i*)(y0l + o), zl0);
-                    zh1 = mfma(cf[u], *(const v4i*)(y1h + o), zh1);
-                    zl1 = mfma(cf[u], *(const v4i*)(y1l + o), zl1);
-                }
+        for (int u = 1; u < KF; u++) {
+            if (u < nK) {
+                const int o = u * 32;
+                zh0 = mfma(cfc[u], *(const v4i*)(y0h + o), zh0);
+                zl0 = mfma(cfc[u], *(const v4i*)(y0l + o), zl0);
+                zh1 = mfma(cfc[u], *(const v4i*)(y1h + o), zh1);
+                zl1 = mfma(cfc[u], *(const v4i*)(y1l + o), zl1);
             }
+        }
+        const int8_t* crow = Ct + (i64)(db * 32 + l32) * npad + 16 * half;
+        for (int kb = KF; kb < nK; kb++) {           // wide prime sets (more than 32 KF primes)
+            const v4i cf = *(const v4i*)(crow + kb * 32);
+            const int o = kb * 32;
+            zh0 = mfma(cf, *(const v4i*)(y0h + o), zh0);
+            zl0 = mfma(cf, *(const v4i*)(y0l + o), zl0);
+            zh1 = mfma(cf, *(const v4i*)(y1h + o), zh1);
+            zl1 = mfma(cf, *(const v4i*)(y1l + o), zl1);
         }
         // word partials sum_u Z_u 256^u, Z_u = 128 zh + zl (|Z_u| < 2^31); X: group 0, Y: group 1
         u32 xl[4], xh[4], yl[4], yh[4];
@@ -581,25 +594,34 @@ __global__ void __launch_bounds__(64) k_rns_crt(
             swap_halves(xl[wq], yl[wq]);
             swap_halves(xh[wq], yh[wq]);
         }
+        u32 word[8];
 #pragma unroll
         for (int h2 = 0; h2 < 8; h2++) {
             const int wq = h2 >> 1;
             const u64 p = (h2 & 1) ? ((u64)yh[wq] << 32 | yl[wq]) : ((u64)xh[wq] << 32 | xl[wq]);
             const i64 v = carry + (i64)p;
-            const u32 word = (u32)v;
+            word[h2] = (u32)v;
             carry = v >> 32;
-            const int wi = db * 8 + h2;
-            if (wi < NW) {
-                if (mode == SDPV_CRT_WORDS) {
-                    if (qok) W[(i64)wi * Bq + q] = (i32)word;
-                    if (word != 0u) { last0 = wi; w0 = word; }
-                    if (word != 0xffffffffu) { last1 = wi; w1 = ~word; }
-                    lastw = word;
-                } else {
-                    wrow[wi] = word;
+        }
+        const int nw = NW - db * 8;                    // words of this block inside the value
+        if (mode == SDPV_CRT_WORDS) {
+#pragma unroll
+            for (int h2 = 0; h2 < 8; h2++) {
+                if (h2 < nw) {
+                    const int wi = db * 8 + h2;
+                    if (qok) W[(i64)wi * Bq + q] = (i32)word[h2];
+                    if (word[h2] != 0u) { last0 = wi; w0 = word[h2]; }
+                    if (word[h2] != 0xffffffffu) { last1 = wi; w1 = ~word[h2]; }
+                    lastw = word[h2];
                 }
             }
+        } else {
+#pragma unroll
+            for (int h2 = 0; h2 < 8; h2++)
+                if (h2 < nw) wrow[db * 8 + h2] = word[h2];
         }
+#pragma unroll
+        for (int u = 0; u < KF; u++) cfc[u] = cfn[u];
     }
 
     // ---- phase 3 (lane = entry; reads only its own LDS row, in batches of independent reads)
