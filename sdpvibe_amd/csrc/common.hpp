@@ -97,6 +97,53 @@ __device__ __forceinline__ int shr_round(const Src& src, int ns, i64 s, int Lout
     return 0;
 }
 
+// Words k < Lout of floor(v / 2^s) + c (s < 0: exact left shift), v the two's complement value
+// of ns words read through src (zero below word 0, sign words above ns - 1), in batches of 8
+// independent reads; returns word Lout of that sum. |s| is clamped to where the words no longer
+// change (beyond the value, or beyond the output).
+template <class Src, class Out>
+__device__ __forceinline__ u32 shift_words(const Src& src, int ns, u32 ext, i64 s, u32 c, int Lout, Out out) {
+    const i64 smax = 32 * (i64)ns + 32, smin = -32 * ((i64)Lout + 2);
+    s = s > smax ? smax : (s < smin ? smin : s);
+    auto wd = [&](int t) -> u32 {
+        const u32 v = src(t < 0 ? 0 : (t >= ns ? ns - 1 : t));
+        return t < 0 ? 0u : (t >= ns ? ext : v);
+    };
+    const int qs = (int)(s >> 5), r = (int)(s & 31);
+    for (int k0 = 0; k0 < Lout; k0 += 8) {
+        u32 x[9];
+#pragma unroll
+        for (int u = 0; u < 9; u++) x[u] = wd(qs + k0 + u);
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+            if (k0 + u < Lout) {
+                const u32 d = (u32)((((u64)x[u + 1] << 32) | x[u]) >> r);
+                const u32 t = d + c;
+                c = t < d ? 1u : 0u;
+                out(k0 + u, t);
+            }
+        }
+    }
+    return (u32)((((u64)wd(qs + Lout + 1) << 32) | wd(qs + Lout)) >> r) + c;
+}
+
+// shr_round with the source read in batches (shift_words): same words, same return value.
+template <class Src, class Out>
+__device__ __forceinline__ int shr_round_batched(const Src& src, int ns, i64 s, int Lout, Out out) {
+    const u32 ext = (src(ns - 1) >> 31) ? 0xffffffffu : 0u;
+    u32 c = 0;
+    if (s >= 1) {
+        const i64 sb = s - 1;
+        c = (sb >> 5) >= ns ? (ext & 1u) : (src((int)(sb >> 5)) >> (int)(sb & 31)) & 1u;
+    }
+    u32 prev = 0;
+    const u32 top = shift_words(src, ns, ext, s, c, Lout, [&](int k, u32 v) {
+        out(k, v);
+        prev = v;
+    });
+    return top != ((prev >> 31) ? 0xffffffffu : 0u);
+}
+
 // Bit length of v (of ~v when v < 0), limbops._bitlen_kernel (sdpvibe/limbops.py:197-210).
 template <class Src>
 __device__ __forceinline__ int bitlen_tc(const Src& src, int n) {

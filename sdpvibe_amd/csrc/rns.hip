@@ -98,15 +98,15 @@ __global__ void __launch_bounds__(256) k_rns_split(
         const i64 Rc = valid ? R : 0;
         const i64 off = valid ? (Rc / rows) * ldb + (Rc % rows) * ldr + (k0 + kg) * ldk : 0;
         if (stage) {
-            for (int w0 = 0; w0 < L; w0 += 8) {          // independent loads, 8 in flight
-                u32 v[8];
+            for (int w0 = 0; w0 < L; w0 += 32) {         // independent loads, up to 32 in flight
+                u32 v[32];
 #pragma unroll
-                for (int u = 0; u < 8; u++) {
+                for (int u = 0; u < 32; u++) {
                     const int w = w0 + u < L ? w0 + u : L - 1;
                     v[u] = (u32)X[off + (i64)w * plane];
                 }
 #pragma unroll
-                for (int u = 0; u < 8; u++)
+                for (int u = 0; u < 32; u++)
                     if (w0 + u < L) raw[w0 + u] = v[u];
             }
         }
@@ -134,8 +134,8 @@ __global__ void __launch_bounds__(256) k_rns_split(
                 }
                 if (wi == Lout - 1) neg = (int)(v >> 31);
             };
-            const int bad = stage ? shr_round(LdsStrided{raw, 1}, L, s, Lout, put)
-                                  : shr_round(PlaneSrc{X + off, plane}, L, s, Lout, put);
+            const int bad = stage ? shr_round_batched(LdsStrided{raw, 1}, L, s, Lout, put)
+                                  : shr_round_batched(PlaneSrc{X + off, plane}, L, s, Lout, put);
             if (bad && flag) atomicOr(flag, 1);
             if (Lout & 3) flush4(Lout >> 2);                  // last partial group, zero-padded
         } else {
@@ -175,9 +175,10 @@ __global__ void __launch_bounds__(256) k_rns_split(
         o.c1 = C1[g];
         o.c2 = C2[g];
     };
-    PB cur;
+    PB cur, nxt;
+    load_pb(0, cur);
     for (int pb = 0; pb < nPB; pb++) {
-        load_pb(pb, cur);
+        if (pb + 1 < nPB) load_pb(pb + 1, nxt);        // next block's table chunks in flight
         const int pr = pb * 32 + l32;
         const bool pok = pr < G;
         const int g = g0 + (pok ? pr : 0);
@@ -251,6 +252,7 @@ __global__ void __launch_bounds__(256) k_rns_split(
                 *(v4i*)(Rl + o) = vl;
             }
         }
+        cur = nxt;
     }
 }
 
@@ -637,6 +639,7 @@ __global__ void __launch_bounds__(64) k_rns_crt(
     const i64 o = i * ldo + jo + jj;
     const i64 cidx = (i / Mb) * ldo + jo + jj;
     const u32 ext = (wrow[NW - 1] >> 31) ? 0xffffffffu : 0u;   // sign extension word
+    const LdsWords src{wrow};
     // word t of the value's two's complement: zero below 0, the sign extension above NW - 1
     auto wd = [&](int t) -> u32 {
         const u32 v = wrow[t < 0 ? 0 : (t >= NW ? NW - 1 : t)];
@@ -688,37 +691,11 @@ __global__ void __launch_bounds__(64) k_rns_crt(
         }
         const i64 ein = (erow ? erow[i] : 0) + (ecol ? ecol[cidx] : 0);
         oute[o] = ein + sv;
+        (void)shift_words(src, NW, ext, sv, c, L, [&](int k, u32 v) { outm[(i64)k * plane + o] = (i32)v; });
     } else {
-        // round_half_up(v / 2^sv) = floor((v + 2^(sv-1)) / 2^sv) (limbops._shift_round_kernel); shifts
-        // beyond the value or the output give the same words as these clamped ones
-        sv = scol[cidx];
-        const i64 smax = 32 * (i64)NW + 32, smin = -32 * ((i64)L + 2);
-        sv = sv > smax ? smax : (sv < smin ? smin : sv);
-        if (sv > 0) c = (wd((int)((sv - 1) >> 5)) >> (int)((sv - 1) & 31)) & 1u;
-    }
-    // out word k = bits [32k + sv, 32k + sv + 32) of v, plus the carry; sv < 0 shifts left
-    const int qs = (int)(sv >> 5), r = (int)(sv & 31);
-    u32 prev = 0;
-    for (int k0 = 0; k0 < L; k0 += 8) {
-        u32 x[9];
-#pragma unroll
-        for (int u = 0; u < 9; u++) x[u] = wd(qs + k0 + u);
-#pragma unroll
-        for (int u = 0; u < 8; u++) {
-            if (k0 + u < L) {
-                const u32 d = (u32)((((u64)x[u + 1] << 32) | x[u]) >> r);
-                const u32 t = d + c;
-                c = t < d ? 1u : 0u;
-                outm[(i64)(k0 + u) * plane + o] = (i32)t;
-                prev = t;
-            }
-        }
-    }
-    if (mode == SDPV_CRT_ROUND) {
-        // overflow: word L of the rounded value differs from the sign extension of word L - 1
-        const u32 d = (u32)((((u64)wd(qs + L + 1) << 32) | wd(qs + L)) >> r);
-        const u32 want = (prev >> 31) ? 0xffffffffu : 0u;
-        if (d + c != want && flag) atomicOr(flag, 1);
+        // round_half_up(v / 2^s) = floor((v + 2^(s-1)) / 2^s) (limbops._shift_round_kernel)
+        const int bad = shr_round_batched(src, NW, scol[cidx], L, [&](int k, u32 v) { outm[(i64)k * plane + o] = (i32)v; });
+        if (bad && flag) atomicOr(flag, 1);
     }
 }
 
