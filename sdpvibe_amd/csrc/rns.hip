@@ -769,7 +769,7 @@ int sdpv_rns_gemm(const int8_t* Ah, const int8_t* Al, const int8_t* Bh, const in
                   int64_t M, int64_t N, int64_t kp, int16_t* acc, int64_t ld_g, const int32_t* mods,
                   const double* minv, const int32_t* kcoef, int64_t g0, int32_t mode, int32_t config, void* stream) {
     if (G < 0 || nb < 0 || M < 0 || N < 0 || kp < 0 || g0 < 0 || (mode & ~7)) return SDPV_EARG;
-    if (config < -1 || config > 7) return SDPV_EARG;
+    if (config < -1 || config > 9) return SDPV_EARG;
     if (kp % 64 != 0 || kp > 65536) return SDPV_ESIZE;
     if ((mode & SDPV_GEMM_SYRK_LOWER) && (M != N || config != -1)) return SDPV_EARG;
     if (G == 0 || nb == 0 || M == 0 || N == 0) return SDPV_OK;
@@ -789,7 +789,9 @@ int sdpv_rns_gemm(const int8_t* Ah, const int8_t* Al, const int8_t* Bh, const in
         case 4: return launch_gemm<4, 2, 2, 2, 2, 0>(Ah, Al, Bh, Bl, g, b, M, N, kp, acc, ld_g, mods, minv, kcoef, q0, mode, st);
         case 5: return launch_gemm<2, 4, 2, 2, 2, 0>(Ah, Al, Bh, Bl, g, b, M, N, kp, acc, ld_g, mods, minv, kcoef, q0, mode, st);
         case 6: return launch_gemm<2, 2, 2, 2, 2, 1>(Ah, Al, Bh, Bl, g, b, M, N, kp, acc, ld_g, mods, minv, kcoef, q0, mode, st);
-        default: return launch_gemm<2, 4, 2, 2, 2, 1>(Ah, Al, Bh, Bl, g, b, M, N, kp, acc, ld_g, mods, minv, kcoef, q0, mode, st);
+        case 7: return launch_gemm<2, 4, 2, 2, 2, 1>(Ah, Al, Bh, Bl, g, b, M, N, kp, acc, ld_g, mods, minv, kcoef, q0, mode, st);
+        case 8: return launch_gemm<2, 4, 2, 2, 3, 1>(Ah, Al, Bh, Bl, g, b, M, N, kp, acc, ld_g, mods, minv, kcoef, q0, mode, st);
+        default: return launch_gemm<2, 4, 2, 2, 3, 0>(Ah, Al, Bh, Bl, g, b, M, N, kp, acc, ld_g, mods, minv, kcoef, q0, mode, st);
     }
 }
 
