@@ -41,6 +41,27 @@ def parse():
     return ap.parse_args()
 
 
+TRAFFIC_PROFILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                               "mpgemm_512_L24_hbm_traffic.md")
+
+
+def measured_traffic(kernel, config, n, L):
+    """HBM bytes per launch of `kernel` (product tile configuration `config`) at this workload, from
+    the rocprofv3 FETCH_SIZE/WRITE_SIZE summary, if it was measured on this library's sources;
+    None otherwise. Lines: `traffic <kernel> config=<c> n=<n> L=<L> bytes=<B> source=<hash>`."""
+    from sdpvibe_amd import _lib
+    if not os.path.exists(TRAFFIC_PROFILE):
+        return None
+    src = "%#x" % _lib.lib().sdpv_source_hash()
+    pat = re.compile(r"^traffic (\S+) config=(-?\d+) n=(\d+) L=(\d+) bytes=(\d+) source=(0x[0-9a-f]+)\s*$")
+    for line in open(TRAFFIC_PROFILE):
+        m = pat.match(line.strip())
+        if m and m.group(1) == kernel and int(m.group(2)) == config and int(m.group(3)) == n \
+                and int(m.group(4)) == L and m.group(6) == src:
+            return int(m.group(5))
+    return None
+
+
 def cpu_baseline(n, threads, L):
     """The reference's CPU path on the same shape: mpf_baseline <prec> <n> <nrhs> <threads>
     (GMP mpf at 32L bits; its GEMM line is an n x n x n product)."""
@@ -163,6 +184,11 @@ def main():
     roof["kernel"] = {"product": "k_rns_gemm", "split_A": "k_rns_split", "split_B": "k_rns_split",
                       "crt": "k_rns_crt", "row_exp": "k_row_exp_*"}[dom]
     roof["avg_launch_ms"] = ph[dom]
+    from sdpvibe_amd import rns as RN
+    cfg = RN.tuned_config(R, 1, M, N, kp) if dom == "product" else None
+    if cfg is not None:
+        roof["config"] = cfg
+    roof["traffic"] = measured_traffic(roof["kernel"], cfg if cfg is not None else -1, n, L)
 
     cpu = None if args.no_cpu_baseline else cpu_baseline(n, args.cpu_threads, L)
     line = {

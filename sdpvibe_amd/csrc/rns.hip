@@ -408,6 +408,7 @@ __global__ void __launch_bounds__(WM* WN * 64, (WM * WN <= 4) ? 2 : 1) k_rns_gem
     const bool accum = (mode & SDPV_GEMM_ACCUMULATE) != 0;
     const bool centred = (mode & SDPV_GEMM_CENTRED) != 0;
     int16_t* out = acc + (i64)g * ld_g + b * M * N;
+    const bool full = r0 + BM <= M && c0 + BN <= N;  // uniform: no per-entry tests inside the matrix
 #pragma unroll
     for (int i = 0; i < BLM; i++)
 #pragma unroll
@@ -416,7 +417,7 @@ __global__ void __launch_bounds__(WM* WN * 64, (WM * WN <= 4) ? 2 : 1) k_rns_gem
 #pragma unroll
             for (int q = 0; q < 16; q++) {
                 const i64 row = r0 + (wm * BLM + i) * 32 + 8 * (q >> 2) + 4 * hsel + (q & 3);
-                if (row < M && col < N) {
+                if (full || (row < M && col < N)) {
                     double v = fma(k0c, (double)cHH[i][j][q], fma(k1c, (double)cSS[i][j][q], k2c * (double)cLL[i][j][q]));
                     if (accum) v += (double)out[row * N + col];
                     // nearest quotient, exact: v*mi errs by < 2^-17, v/m is >= 1/(2m) > 2^-15 from a half-integer

@@ -68,6 +68,10 @@ def gemm_config(M, N, K):
 GEMM_CONFIGS = (0, 1, 2, 3, 4, 5, 6, 7, 8, 9)   # see sdpv_rns_gemm: tile shapes, stages, three or four products
 _tuned = {}
 TUNE = os.environ.get("SDPV_TUNE", "1") != "0"
+# SDPV_GEMM_CONFIG=<c> pins every tuned product to configuration c (measurements of one tile shape)
+FORCED = int(os.environ["SDPV_GEMM_CONFIG"]) if os.environ.get("SDPV_GEMM_CONFIG") else None
+if FORCED is not None and FORCED not in GEMM_CONFIGS:
+    raise ValueError(f"SDPV_GEMM_CONFIG={FORCED}: not one of {GEMM_CONFIGS}")
 
 
 def _bucket(x):
@@ -81,6 +85,8 @@ def tuned_config(R, nb, M, N, kp):
     tuning is off or the product is small"""
     if not TUNE or nb * M * N * kp < (1 << 24):
         return -1
+    if FORCED is not None:
+        return FORCED
     key = (R.L, _bucket(nb), _bucket(M), _bucket(N), _bucket(kp))
     if key not in _tuned:
         _tuned[key] = _measure(R, nb, M, N, kp)
