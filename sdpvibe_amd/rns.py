@@ -94,7 +94,8 @@ def tuned_config(R, nb, M, N, kp):
 
 
 def _measure(R, nb, M, N, kp, reps=3):
-    G = min(R.n, 32)
+    # every prime when the scratch operands stay small (the launch's wave count decides between tiles)
+    G = R.n if R.n * nb * (2 * kp * (M + N) + 2 * M * N) <= (1 << 29) else min(R.n, 32)
     dev_ = R.m32.device
     gen = torch.Generator(device=dev_)
     gen.manual_seed(0)
