@@ -38,7 +38,59 @@ def parse():
     ap.add_argument("--L", type=int, default=24)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
+    ap.add_argument("--no-batched", action="store_true", help="skip the uneven-block Cholesky comparison")
+    ap.add_argument("--blocks", type=int, default=400)
     return ap.parse_args()
+
+
+def batched_chol(L, nblocks, seed=0, rhs=4):
+    """Uneven-block Cholesky + solve at L limbs: nblocks SPD blocks A = D (B B^T + n I) D, n uniform
+    in 8..96, B Gaussian MPF, D = diag(10^u) with u uniform in [-20, 20], `rhs` Gaussian right-hand
+    sides per block. Times the plain per-block form (one Chol and one solve per block) against the
+    bucketed form (one batched factorisation and solve per size class); results are bitwise equal."""
+    import numpy as np
+    import torch
+    from sdpvibe_amd import mpf as F
+    from sdpvibe_amd.synthetic import gaussian_mpf
+
+    rng = np.random.default_rng(seed)
+    dims = [int(x) for x in rng.integers(8, 97, nblocks)]
+    put = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()
+    blocks, rhs_l = [], []
+    p10 = F.from_strings([f"1e{k}" for k in range(-40, 41)], L)      # 10^k rounded to NB bits
+    for i, n in enumerate(dims):
+        bm, be = gaussian_mpf((n, n), L, seed=10_000 + i)
+        B = F.MPF(put(bm), put(be))
+        G = F.add(F.gemm(B, B, transB=True), F.eye(n, L, val=n))
+        u = rng.integers(-20, 21, n)
+        idx = put((u[:, None] + u[None, :] + 40).reshape(-1))
+        D = F.MPF(p10.m[:, idx].reshape(L, n, n).contiguous(), p10.e[idx].reshape(n, n).contiguous())
+        blocks.append(F.mul(G, D))
+        rm, re_ = gaussian_mpf((n, rhs), L, seed=20_000 + i)
+        rhs_l.append(F.MPF(put(rm), put(re_)))
+    torch.cuda.synchronize()
+
+    def plain():
+        return [F.Chol(A).solve(b) for A, b in zip(blocks, rhs_l)]
+
+    def bucketed():
+        return F.BucketedChol(blocks).solve(rhs_l)
+
+    out = {}
+    for name, f in (("plain", plain), ("bucketed", bucketed)):
+        f()                                               # warm (autotuner, attributes)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        res = f()
+        torch.cuda.synchronize()
+        out[name] = (time.perf_counter() - t0, res)
+    same = all(torch.equal(a.m, b.m) and torch.equal(a.e, b.e) for a, b in zip(out["plain"][1], out["bucketed"][1]))
+    classes = len({max(32, -(-n // 32) * 32) for n in dims})
+    return {"workload": f"{nblocks} SPD blocks, n uniform in 8..96, A = D (B B^T + n I) D, D = diag(10^u), "
+                        f"u uniform in [-20, 20]; factor + solve with {rhs} RHS each, L={L}",
+            "plain_ms": round(out["plain"][0] * 1e3, 2), "bucketed_ms": round(out["bucketed"][0] * 1e3, 2),
+            "speedup": round(out["plain"][0] / out["bucketed"][0], 2), "size_classes": classes,
+            "bitwise_equal": bool(same)}
 
 
 TRAFFIC_PROFILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
@@ -191,6 +243,12 @@ def main():
     roof["traffic"] = measured_traffic(roof["kernel"], cfg if cfg is not None else -1, n, L)
 
     cpu = None if args.no_cpu_baseline else cpu_baseline(n, args.cpu_threads, L)
+    bchol = None
+    if not args.no_batched:
+        try:                                              # auxiliary measurement: never loses the line
+            bchol = batched_chol(L, args.blocks)
+        except Exception as exc:                          # noqa: BLE001
+            bchol = {"error": f"{type(exc).__name__}: {exc}"}
     line = {
         "metric": METRIC, "value": round(value, 3), "unit": "% of dense INT8 MFMA peak (mp-GEMM)",
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": dt * 1e3,
@@ -204,7 +262,7 @@ def main():
         "executed_int8_ops_per_step": ops_total,
         "pct_peak_executed_ops": round(100.0 * ops_total / dt / (PEAK_INT8_TOPS * 1e12), 3),
         "phase_ms": {p: round(v, 4) for p, v in ph.items()},
-        "roofline": roof, "cpu_baseline": cpu,
+        "roofline": roof, "cpu_baseline": cpu, "batched_chol": bchol,
     }
     print(json.dumps(line))
 

@@ -411,7 +411,10 @@ class Chol:
 
     SOLVE_REFINE = int(os.environ.get("SDPB_GPU_SOLVE_REFINE", "0"))
 
-    def __init__(self, A, bt=32, refine=1):
+    def __init__(self, A, bt=32, refine=1, dims=None):
+        """dims: optional (nb,) true dimensions of batched blocks embedded in the top-left corner of
+        A (zeros elsewhere); the rows beyond a block's dimension get the identity padding, as rows
+        beyond n do, so each block factors exactly as it would alone."""
         from .batched import BMP, bchol
         self.two_d = A.e.dim() == 2
         if self.two_d:
@@ -422,9 +425,13 @@ class Chol:
         d = torch.arange(n, device=A.e.device)
         top_d = _top(A)[:, d, d]                                    # (nb, n)
         neg_d = A.m[-1][:, d, d] < 0
-        if bool((neg_d | (top_d <= EZ // 2)).any()):
+        live = None if dims is None else d[None, :] < torch.as_tensor(dims, device=A.e.device)[:, None]
+        bad = neg_d | (top_d <= EZ // 2)
+        if bool((bad if live is None else bad & live).any()):
             raise ValueError("matrix is not positive definite (diagonal)")
         k = torch.div(top_d + 1, 2, rounding_mode="floor")          # A_ii < 2^(2 k_i)
+        if live is not None:
+            k = torch.where(live, k, torch.zeros_like(k))
         self.k = k
         self.npad = npad = max(bt, -(-n // bt) * bt)
         efx = fx_exp(L)
@@ -434,12 +441,17 @@ class Chol:
         Af = shift_round(A.m.reshape(L, -1), s, L).view(L, nb, n, n)
         P = torch.zeros((L, nb, npad, npad), dtype=torch.int32, device=A.m.device)
         P[:, :, :n, :n] = Af
-        if npad > n:                                                # identity padding: 1 = 2^-efx units
-            one = 1 << (-efx)
+        one = 1 << (-efx)                                           # identity padding: 1 = 2^-efx units
+        words = [(one >> (32 * w)) & 0xFFFFFFFF for w in range(L)]
+        words = [v - (1 << 32) if v >> 31 else v for v in words]
+        if npad > n:
             pad = torch.arange(n, npad, device=A.m.device)
             for w in range(L):
-                v = (one >> (32 * w)) & 0xFFFFFFFF
-                P[w][:, pad, pad] = v - (1 << 32) if v >> 31 else v
+                P[w][:, pad, pad] = words[w]
+        if live is not None and not bool(live.all()):
+            bi, ri = (~live).nonzero(as_tuple=True)
+            for w in range(L):
+                P[w][bi, ri, ri] = words[w]
         self.Lt, self.tiles = bchol(BMP(P, efx), bt, refine)
 
     def _to_bmp(self, B):
@@ -485,6 +497,67 @@ class Chol:
     def solve(self, B):
         """A^-1 B"""
         return self.solve_upper(self.solve_lower(B))
+
+
+class BucketedChol:
+    """Cholesky factors and solves of SPD blocks of uneven dimension. Blocks whose dimension
+    rounds up to the same multiple of bt share one batched factorisation: each is embedded with
+    the identity padding Chol(block) uses on its own (the `dims` argument of Chol), so every factor
+    and every solve is bitwise Chol(block)'s, with one launch sequence per size class instead of
+    one per block. (The reference batches equal dimensions with inert padding, solver.py:10-12.)"""
+
+    def __init__(self, blocks, bt=32, refine=1):
+        if not blocks:
+            raise ValueError("no blocks")
+        self.dims = [int(b.shape[-1]) for b in blocks]
+        self.L, self.bt = blocks[0].L, bt
+        for b, n in zip(blocks, self.dims):
+            if b.e.dim() != 2 or b.shape[0] != n or b.L != self.L:
+                raise ValueError("blocks must be square 2-d MPF of one limb count")
+        classes = {}
+        for i, n in enumerate(self.dims):
+            classes.setdefault(max(bt, -(-n // bt) * bt), []).append(i)
+        self.groups = []
+        for npad, idx in sorted(classes.items()):
+            A = self._embed([blocks[i] for i in idx], npad, npad)
+            dims = torch.tensor([self.dims[i] for i in idx], dtype=torch.int64, device=A.e.device)
+            self.groups.append((idx, Chol(A, bt, refine, dims=dims)))
+
+    def _embed(self, Bs, rows, cols):
+        dev_ = Bs[0].m.device
+        m = torch.zeros((self.L, len(Bs), rows, cols), dtype=torch.int32, device=dev_)
+        e = torch.full((len(Bs), rows, cols), EZ, dtype=torch.int64, device=dev_)
+        for j, B in enumerate(Bs):
+            r, c = B.shape
+            m[:, j, :r, :c] = B.m
+            e[j, :r, :c] = B.e
+        return MPF(m, e)
+
+    def _solve(self, Bs, which):
+        if len(Bs) != len(self.dims):
+            raise ValueError("one right-hand side per block")
+        out = [None] * len(Bs)
+        for idx, ch in self.groups:
+            cols = {int(Bs[i].shape[-1]) for i in idx}
+            if len(cols) != 1 or any(Bs[i].shape[0] != self.dims[i] for i in idx):
+                raise ValueError("right-hand sides of one size class need the block's rows and one width")
+            R = getattr(ch, which)(self._embed([Bs[i] for i in idx], ch.n, cols.pop()))
+            for j, i in enumerate(idx):
+                n = self.dims[i]
+                out[i] = MPF(R.m[:, j, :n].contiguous(), R.e[j, :n].contiguous())
+        return out
+
+    def solve_lower(self, Bs):
+        """[L_b^-1 B_b] for the list of right-hand sides B_b (n_b, m)"""
+        return self._solve(Bs, "solve_lower")
+
+    def solve_upper(self, Bs):
+        """[L_b^-T B_b]"""
+        return self._solve(Bs, "solve_upper")
+
+    def solve(self, Bs):
+        """[A_b^-1 B_b]"""
+        return self._solve(Bs, "solve")
 
 
 # ------------------------------------------------------------------ Gram accumulation (Q = sum_c P_c^T P_c)

@@ -263,6 +263,16 @@ def small_round_cases():
 CASES["small_round"] = small_round_cases
 
 
+def bucketed_cases():
+    from tests.test_gpu_bucketed import bucketed_case
+    bucketed_case(lambda a: torch.from_numpy(np.ascontiguousarray(a)), 2, [3, 17, 33, 9], 2, seed=5)
+    print("bucketed chol: ok", flush=True)
+    return True
+
+
+CASES["bucketed"] = bucketed_cases
+
+
 if __name__ == "__main__":  # noqa: E305
     ok = True
     for c in sys.argv[1:] or list(CASES):
