@@ -299,7 +299,24 @@ __global__ void __launch_bounds__(WM* WN * 64, (WM * WN <= 4) ? 2 : 1) k_rns_gem
     const i64 offA = ((i64)gi * nck * rowsA + b * M + r0) * 16;  // tile origin, chunk 0
     const i64 offB = ((i64)gi * nck * rowsB + b * N + c0) * 16;
 
-    // direct loads of stage s into LDS buffer s % NST
+    // direct loads of stage s into LDS buffer s % NST; rows past the matrix load its last row (kept
+    // out of the results), so every stage issues exactly LPW loads per wave (the waits count them).
+    // Per-lane byte offsets inside a stage are fixed (32-bit: the host keeps 64 rows x 16 B of a
+    // plane below 2^32); the stage origin is uniform.
+    const int rlimA = (int)(M - 1 - r0 < BM - 1 ? M - 1 - r0 : BM - 1);
+    const int rlimB = (int)(N - 1 - c0 < BN - 1 ? N - 1 - c0 : BN - 1);
+    u32 loff[LPW];
+#pragma unroll
+    for (int u = 0; u < LPW; u++) {
+        const int q = wave * LPW + u;
+        if (q < 2 * QA) {
+            const int e = (q % QA) * 64 + lane;
+            loff[u] = (u32)(((e / BM) * rowsA + min(e % BM, rlimA)) * 16);
+        } else {
+            const int e = ((q - 2 * QA) % QB) * 64 + lane;
+            loff[u] = (u32)(((e / BN) * rowsB + min(e % BN, rlimB)) * 16);
+        }
+    }
     auto issue = [&](int s) {
         int8_t* sbase = smem8 + (s % NST) * SBYTES;
         const i64 kc0 = (i64)s * 4;
@@ -308,15 +325,13 @@ __global__ void __launch_bounds__(WM* WN * 64, (WM * WN <= 4) ? 2 : 1) k_rns_gem
             const int q = wave * LPW + u;                // uniform per wave
             if (q < 2 * QA) {
                 const int p = q / QA, qq = q % QA;
-                const int e = qq * 64 + lane, c = e / BM, r = e % BM;
-                const int8_t* src = (p ? Al : Ah) + offA + ((kc0 + c) * rowsA + r) * 16;
-                if (r0 + r < M) SDPV_LDS_DMA16(src, sbase + p * PA + qq * 1024);
+                const int8_t* base = (p ? Al : Ah) + offA + kc0 * rowsA * 16;
+                SDPV_LDS_DMA16(base + loff[u], sbase + p * PA + qq * 1024);
             } else {
                 const int q2 = q - 2 * QA;
                 const int p = q2 / QB, qq = q2 % QB;
-                const int e = qq * 64 + lane, c = e / BN, r = e % BN;
-                const int8_t* src = (p ? Bl : Bh) + offB + ((kc0 + c) * rowsB + r) * 16;
-                if (c0 + r < N) SDPV_LDS_DMA16(src, sbase + 2 * PA + p * PB + qq * 1024);
+                const int8_t* base = (p ? Bl : Bh) + offB + kc0 * rowsB * 16;
+                SDPV_LDS_DMA16(base + loff[u], sbase + 2 * PA + p * PB + qq * 1024);
             }
         }
     };
@@ -772,6 +787,7 @@ int sdpv_rns_gemm(const int8_t* Ah, const int8_t* Al, const int8_t* Bh, const in
     if (G < 0 || nb < 0 || M < 0 || N < 0 || kp < 0 || g0 < 0 || (mode & ~7)) return SDPV_EARG;
     if (config < -1 || config > 9) return SDPV_EARG;
     if (kp % 64 != 0 || kp > 65536) return SDPV_ESIZE;
+    if (nb * M > (1LL << 26) || nb * N > (1LL << 26)) return SDPV_ESIZE;   // 32-bit offsets in a stage
     if ((mode & SDPV_GEMM_SYRK_LOWER) && (M != N || config != -1)) return SDPV_EARG;
     if (G == 0 || nb == 0 || M == 0 || N == 0) return SDPV_OK;
     if (!Ah || !Al || !Bh || !Bl || !acc || !mods || !minv || !kcoef) return SDPV_EARG;
