@@ -11,6 +11,8 @@
 //   D:         lane l, register j holds (row 8*(j/4) + 4*(l/32) + j%4, column l % 32).
 // A and B use the same k-slot assignment, so the dot product is exact whatever order the
 // hardware pairs the bytes in.
+#include <type_traits>
+
 #include "common.hpp"
 
 namespace {
@@ -508,22 +510,18 @@ __global__ void __launch_bounds__(64) k_rns_crt(
     double sp[4] = {0.0, 0.0, 0.0, 0.0};             // four independent partial sums of y_g / m_g
     int8_t* yhr = Yh + (size_t)lane * YS;
     int8_t* ylr = Yl + (size_t)lane * YS;
-    for (int gc = 0; gc < npad; gc += 16) {
+    // 16 primes gc .. gc + 15; FULL: all below n (consecutive constants: wide scalar loads)
+    auto chunk = [&](int gc, auto full_t) {
+        constexpr bool FULL = decltype(full_t)::value;
         const int cnt = n - gc;                       // primes in this chunk (uniform)
         int a[16];
-        if (cnt >= 16) {
 #pragma unroll
-            for (int u = 0; u < 16; u++)
-                a[u] = *(const int16_t*)((const char*)(acc + (i64)(gc + u) * ld_g) + boff);
-        } else {
-#pragma unroll
-            for (int u = 0; u < 16; u++)
-                a[u] = u < cnt ? *(const int16_t*)((const char*)(acc + (i64)(gc + u) * ld_g) + boff) : 0;
-        }
+        for (int u = 0; u < 16; u++)
+            a[u] = (FULL || u < cnt) ? *(const int16_t*)((const char*)(acc + (i64)(gc + u) * ld_g) + boff) : 0;
         int hi[16], lo[16];
 #pragma unroll
         for (int u = 0; u < 16; u++) {
-            const int g = gc + u < n ? gc + u : n - 1;  // padding primes: a = 0, so y = 0
+            const int g = FULL ? gc + u : (gc + u < n ? gc + u : n - 1);  // padding primes: a = 0, y = 0
             int y = a[u];
             if (!SCALED) {                              // x < m^2 < 2^28: exact nearest quotient
                 const double x = (double)y * (double)inv[g];
@@ -541,7 +539,10 @@ __global__ void __launch_bounds__(64) k_rns_crt(
         }
         *(v4i*)(yhr + gc) = vh;
         *(v4i*)(ylr + gc) = vl;
-    }
+    };
+    int gc = 0;
+    for (; gc + 16 <= n; gc += 16) chunk(gc, std::true_type{});
+    for (; gc < npad; gc += 16) chunk(gc, std::false_type{});
     {                                                 // column n: -k = 128*hi + lo (|k| <= n/2)
         const int nk = -(int)rint((sp[0] + sp[1]) + (sp[2] + sp[3]));
         const int kl = ((nk + 64) & 127) - 64;
