@@ -560,39 +560,41 @@ __global__ void __launch_bounds__(64) k_rns_crt(
     i64 carry = 0;
     int last0 = -1, last1 = -1;
     u32 w0 = 0, w1 = 0, lastw = 0;
-    // the first KF 32-byte digit-table chunks of a digit block live in registers, the next
-    // block's loaded while this one is folded and carried
+    // the first KF 32-byte digit-table chunks of a digit block live in registers. Software pipeline
+    // over digit blocks: the MFMAs of block d + 1 run while block d is folded and carried, and the
+    // table chunks of block d + 2 load meanwhile.
     constexpr int KF = 4;
-    v4i cfc[KF], cfn[KF];
     auto load_chunks = [&](int db, v4i* cf) {
         const int8_t* crow = Ct + (i64)(db * 32 + l32) * npad + 16 * half;
 #pragma unroll
         for (int u = 0; u < KF; u++) cf[u] = u < nK ? *(const v4i*)(crow + u * 32) : zero4();
     };
-    load_chunks(0, cfc);
-    for (int db = 0; db < nDB; db++) {
-        if (db + 1 < nDB) load_chunks(db + 1, cfn);
-        v16i zh0 = mfma(cfc[0], *(const v4i*)y0h, zero16()), zl0 = mfma(cfc[0], *(const v4i*)y0l, zero16());
-        v16i zh1 = mfma(cfc[0], *(const v4i*)y1h, zero16()), zl1 = mfma(cfc[0], *(const v4i*)y1l, zero16());
+    auto contract = [&](int db, const v4i* cf, v16i* z) {   // z: zh0, zl0, zh1, zl1
+        z[0] = mfma(cf[0], *(const v4i*)y0h, zero16());
+        z[1] = mfma(cf[0], *(const v4i*)y0l, zero16());
+        z[2] = mfma(cf[0], *(const v4i*)y1h, zero16());
+        z[3] = mfma(cf[0], *(const v4i*)y1l, zero16());
 #pragma unroll
         for (int u = 1; u < KF; u++) {
             if (u < nK) {
                 const int o = u * 32;
-                zh0 = mfma(cfc[u], *(const v4i*)(y0h + o), zh0);
-                zl0 = mfma(cfc[u], *(const v4i*)(y0l + o), zl0);
-                zh1 = mfma(cfc[u], *(const v4i*)(y1h + o), zh1);
-                zl1 = mfma(cfc[u], *(const v4i*)(y1l + o), zl1);
+                z[0] = mfma(cf[u], *(const v4i*)(y0h + o), z[0]);
+                z[1] = mfma(cf[u], *(const v4i*)(y0l + o), z[1]);
+                z[2] = mfma(cf[u], *(const v4i*)(y1h + o), z[2]);
+                z[3] = mfma(cf[u], *(const v4i*)(y1l + o), z[3]);
             }
         }
         const int8_t* crow = Ct + (i64)(db * 32 + l32) * npad + 16 * half;
         for (int kb = KF; kb < nK; kb++) {           // wide prime sets (more than 32 KF primes)
-            const v4i cf = *(const v4i*)(crow + kb * 32);
+            const v4i c = *(const v4i*)(crow + kb * 32);
             const int o = kb * 32;
-            zh0 = mfma(cf, *(const v4i*)(y0h + o), zh0);
-            zl0 = mfma(cf, *(const v4i*)(y0l + o), zl0);
-            zh1 = mfma(cf, *(const v4i*)(y1h + o), zh1);
-            zl1 = mfma(cf, *(const v4i*)(y1l + o), zl1);
+            z[0] = mfma(c, *(const v4i*)(y0h + o), z[0]);
+            z[1] = mfma(c, *(const v4i*)(y0l + o), z[1]);
+            z[2] = mfma(c, *(const v4i*)(y1h + o), z[2]);
+            z[3] = mfma(c, *(const v4i*)(y1l + o), z[3]);
         }
+    };
+    auto fold = [&](int db, const v16i* z) {
         // word partials sum_u Z_u 256^u, Z_u = 128 zh + zl (|Z_u| < 2^31); X: group 0, Y: group 1
         u32 xl[4], xh[4], yl[4], yh[4];
 #pragma unroll
@@ -600,8 +602,8 @@ __global__ void __launch_bounds__(64) k_rns_crt(
             i64 v0 = 0, v1 = 0;
 #pragma unroll
             for (int u = 0; u < 4; u++) {
-                v0 += (i64)(128 * zh0[4 * wq + u] + zl0[4 * wq + u]) << (8 * u);
-                v1 += (i64)(128 * zh1[4 * wq + u] + zl1[4 * wq + u]) << (8 * u);
+                v0 += (i64)(128 * z[0][4 * wq + u] + z[1][4 * wq + u]) << (8 * u);
+                v1 += (i64)(128 * z[2][4 * wq + u] + z[3][4 * wq + u]) << (8 * u);
             }
             xl[wq] = (u32)v0;
             xh[wq] = (u32)((u64)v0 >> 32);
@@ -639,8 +641,25 @@ __global__ void __launch_bounds__(64) k_rns_crt(
             for (int h2 = 0; h2 < 8; h2++)
                 if (h2 < nw) wrow[db * 8 + h2] = word[h2];
         }
-#pragma unroll
-        for (int u = 0; u < KF; u++) cfc[u] = cfn[u];
+    };
+    v4i cfA[KF], cfB[KF];
+    v16i zA[4], zB[4];
+    load_chunks(0, cfA);
+    contract(0, cfA, zA);
+    if (nDB > 1) load_chunks(1, cfB);
+    for (int db = 0; db < nDB; db += 2) {
+        if (db + 1 < nDB) {
+            contract(db + 1, cfB, zB);
+            if (db + 2 < nDB) load_chunks(db + 2, cfA);
+        }
+        fold(db, zA);
+        if (db + 1 < nDB) {
+            if (db + 2 < nDB) {
+                contract(db + 2, cfA, zA);
+                if (db + 3 < nDB) load_chunks(db + 3, cfB);
+            }
+            fold(db + 1, zB);
+        }
     }
 
     // ---- phase 3 (lane = entry; reads only its own LDS row, in batches of independent reads)
