@@ -108,7 +108,9 @@ int sdpv_row_exp(const int32_t* X, const int64_t* e, int64_t L, int64_t plane, i
  * Rh, Rl: (G, kp/16, nb*rows, 16) int8, chunk-major: byte k of row r of prime g at
  * ((g*(kp/16) + k/16)*nb*rows + r)*16 + k%16; columns kc <= k < kp hold residue 0.
  * Tables (built by the caller for Lout): Thi/Tlo (npad, NBp) int8 with NBp = 32*ceil(4 Lout/32),
- * C1, C2, mods (n,) int32, minv (n,) double = 1/m.
+ * T = 128*Thi + Tlo the centred 256^j mod m; C1 (n,) int32 = 128 * sum_{j < 4 Lout - 1} T_j mod m
+ * (the bias of every byte but the top one, which enters signed); mods (n,) int32, minv (n,)
+ * double = 1/m.
  * flag: optional int32 set to 1 when the shifted value overflows Lout limbs.
  * Replaces RNS.residues / _residue_kernel (sdpvibe/rns.py:196-214, :67-103) fused with the
  * shift_round of mpf._align_rows / GramAcc.add (sdpvibe/mpf.py:390-398, :574-576). */
@@ -116,7 +118,7 @@ int sdpv_rns_split(const int32_t* X, const int64_t* e, const int64_t* Erow, int6
                    int64_t Lout, int64_t plane, int64_t nb, int64_t rows, int64_t ldb,
                    int64_t ldr, int64_t ldk, int64_t k0, int64_t kc, int64_t kp,
                    const int8_t* Thi, const int8_t* Tlo, int64_t NBp, const int32_t* C1,
-                   const int32_t* C2, const int32_t* mods, const double* minv, int64_t g0,
+                   const int32_t* mods, const double* minv, int64_t g0,
                    int64_t G, int8_t* Rh, int8_t* Rl, int32_t* flag, void* stream);
 
 #define SDPV_GEMM_ACCUMULATE 1 /* acc = (acc + product) mod m instead of acc = product mod m */

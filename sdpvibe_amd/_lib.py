@@ -26,7 +26,7 @@ SIGNATURES = {
     "sdpv_mpf_norm": ([_P, _P, _I64, _P, _P, _I64, _I64, _P], _I32),
     "sdpv_row_exp": ([_P, _P, _I64, _I64, _I64, _I64, _I64, _I64, _I64, _I64, _P, _P], _I32),
     "sdpv_rns_split": ([_P, _P, _P, _I64, _I64, _I64, _I64, _I64, _I64, _I64, _I64, _I64, _I64, _I64,
-                        _P, _P, _I64, _P, _P, _P, _P, _I64, _I64, _P, _P, _P, _P], _I32),
+                        _P, _P, _I64, _P, _P, _P, _I64, _I64, _P, _P, _P, _P], _I32),
     "sdpv_rns_gemm": ([_P, _P, _P, _P, _I64, _I64, _I64, _I64, _I64, _P, _I64, _P, _P, _P, _I64, _I32, _I32, _P],
                       _I32),
     "sdpv_rns_crt": ([_P, _I64, _I64, _I64, _I64, _I64, _I64, _P, _P, _P, _P, _I64, _I64, _I32, _P, _P,

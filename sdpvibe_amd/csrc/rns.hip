@@ -49,12 +49,13 @@ __device__ __forceinline__ v16i mfma(v4i a, v4i b, v16i c) {
 // row of the residue planes).
 // Phase 1: thread t takes one entry, with the lanes along the contiguous index (k when ldk == 1,
 //   rows otherwise) so the limb loads coalesce; the limbs are staged in LDS with independent
-//   loads, shift-rounded to the row exponent into Lout words and written as bytes u - 128 into
-//   the LDS slot of the wave that owns the entry's row in phase 2 (wave w: rows 4w..4w+3, slot
-//   (r % 4) * 16 + k); the sign goes into a byte array.
+//   loads, shift-rounded to the row exponent into Lout words and written as bytes into the LDS
+//   slot of the wave that owns the entry's row in phase 2 (wave w: rows 4w..4w+3, slot
+//   (r % 4) * 16 + k): u - 128 for every byte but the top one, which goes in as the signed top
+//   byte of the two's complement value, so the bytes carry the sign themselves.
 // Phase 2, per wave and block of 32 primes: the byte dot products x . T_hi and x . T_lo run on MFMA
 //   with the wave's 64 slots as rows and the primes as columns, so each lane holds four runs of 4
-//   consecutive k of one row; r = 128 T_hi.x + T_lo.x + C1 - neg C2 is reduced to the centred
+//   consecutive k of one row; r = 128 T_hi.x + T_lo.x + C1 is reduced to the centred
 //   residue (nearest quotient in double, exact for |r| < 2^30) and split into 128 hi + lo. The two
 //   lane halves swap 4-byte runs so that each lane holds whole 16-byte chunks (one row, one prime)
 //   and stores them with 16-byte writes: 64 contiguous bytes per prime and wave.
@@ -74,7 +75,7 @@ template <int KSC>   // table chunks of 32 bytes: KSC = NBp / 32 (0: any, read f
 __global__ void __launch_bounds__(256) k_rns_split(
     const i32* __restrict__ X, const i64* __restrict__ e, const i64* __restrict__ Erow, int L, int Lout, i64 plane,
     i64 nb, i64 rows, i64 ldb, i64 ldr, i64 ldk, i64 k0, i64 kc, i64 kp, const int8_t* __restrict__ Thi,
-    const int8_t* __restrict__ Tlo, int NBp, const i32* __restrict__ C1, const i32* __restrict__ C2,
+    const int8_t* __restrict__ Tlo, int NBp, const i32* __restrict__ C1,
     const i32* __restrict__ mods, const double* __restrict__ minv, int g0, int G, int8_t* __restrict__ Rh,
     int8_t* __restrict__ Rl, i32* __restrict__ flag, i64 ntiles_k, int stage) {
     SDPV_DYNAMIC_LDS(unsigned char, smem);
@@ -83,8 +84,7 @@ __global__ void __launch_bounds__(256) k_rns_split(
     const int t = threadIdx.x, lane = t & 63;
     const int wave = __builtin_amdgcn_readfirstlane(t >> 6);
     unsigned char* xs = smem;                           // [256 slots][XS]
-    unsigned char* negs = xs + (size_t)256 * XS;        // [256]
-    u32* raw = (u32*)(negs + 256) + (size_t)t * RS;     // [256 threads][RS] when staging
+    u32* raw = (u32*)(xs + (size_t)256 * XS) + (size_t)t * RS;   // [256 threads][RS] when staging
     const i64 tr = blockIdx.x / ntiles_k, tk = blockIdx.x % ntiles_k;
     const i64 row0 = tr * 16, kk0 = tk * 16;
     const i64 nrows = nb * rows;
@@ -114,7 +114,6 @@ __global__ void __launch_bounds__(256) k_rns_split(
         }
         const i64 ex = e ? e[off] : 0;
         const i64 er = e ? Erow[Rc] : 0;
-        int neg = 0;
         // words leave in groups of four (16-byte LDS writes: conflict-free at this row stride)
         u32 b0 = 0x80808080u, b1 = 0x80808080u, b2 = 0x80808080u, b3 = 0x80808080u;
         auto flush4 = [&](int w4) {
@@ -123,33 +122,30 @@ __global__ void __launch_bounds__(256) k_rns_split(
             *(v4i*)(row + 4 * w4) = q4;
             b0 = b1 = b2 = b3 = 0x80808080u;
         };
+        auto put = [&](int wi, u32 v) {                 // bias every byte but the top one
+            const u32 x = v ^ (wi == Lout - 1 ? 0x00808080u : 0x80808080u);
+            switch (wi & 3) {
+                case 0: b0 = x; break;
+                case 1: b1 = x; break;
+                case 2: b2 = x; break;
+                default: b3 = x; flush4(wi >> 2);
+            }
+        };
         if (valid) {
             i64 s = 0;
             if (e) s = (ex == EZV) ? (i64)(1 << 30) : er - ex;
-            auto put = [&](int wi, u32 v) {
-                const u32 x = v ^ 0x80808080u;
-                switch (wi & 3) {
-                    case 0: b0 = x; break;
-                    case 1: b1 = x; break;
-                    case 2: b2 = x; break;
-                    default: b3 = x; flush4(wi >> 2);
-                }
-                if (wi == Lout - 1) neg = (int)(v >> 31);
-            };
             const int bad = stage ? shr_round_batched(LdsStrided{raw, 1}, L, s, Lout, put)
                                   : shr_round_batched(PlaneSrc{X + off, plane}, L, s, Lout, put);
             if (bad && flag) atomicOr(flag, 1);
-            if (Lout & 3) flush4(Lout >> 2);                  // last partial group, zero-padded
         } else {
-            for (int w4 = 0; w4 < (Lout + 3) / 4; w4++) flush4(w4);   // zero entry
+            for (int wi = 0; wi < Lout; wi++) put(wi, 0u);  // zero entry
         }
+        if (Lout & 3) flush4(Lout >> 2);                  // last partial group (zero bytes beyond)
         for (int w4 = (Lout + 3) / 4; w4 < NBp / 16; w4++) flush4(w4);
-        negs[S] = (unsigned char)neg;
     }
     __syncthreads();
 
     // ---- phase 2: wave w owns slots 64w .. 64w + 63 (rows 4w .. 4w + 3 of the tile)
-    const u64 negmask = __ballot(negs[wave * 64 + lane] != 0);
     const unsigned char* xw = xs + (size_t)wave * 64 * XS;
     const int l32 = lane & 31, hsel = lane >> 5;
     const int KS = KSC ? KSC : NBp / 32, nPB = (G + 31) / 32;
@@ -158,7 +154,7 @@ __global__ void __launch_bounds__(256) k_rns_split(
     // block of primes and shared by both 32-slot halves
     struct PB {
         v4i fh[4], fl[4];
-        int m, c1, c2;
+        int m, c1;
         double mi;
     };
     auto load_pb = [&](int pb, PB& o) {
@@ -175,7 +171,6 @@ __global__ void __launch_bounds__(256) k_rns_split(
         o.m = mods[g];
         o.mi = minv[g];
         o.c1 = C1[g];
-        o.c2 = C2[g];
     };
     PB cur, nxt;
     load_pb(0, cur);
@@ -188,7 +183,7 @@ __global__ void __launch_bounds__(256) k_rns_split(
         const int8_t* tl = Tlo + (i64)g * NBp + 16 * hsel;
         const int m = cur.m;
         const double mi = cur.mi, md = (double)m;
-        const int c1 = cur.c1, c2 = cur.c2;
+        const int c1 = cur.c1;
         // both 32-slot blocks against the same table chunks: four independent MFMA chains
         v16i ah[2] = {zero16(), zero16()}, al[2] = {zero16(), zero16()};
 #pragma unroll
@@ -216,13 +211,11 @@ __global__ void __launch_bounds__(256) k_rns_split(
         for (int eb = 0; eb < 2; eb++) {
 #pragma unroll
             for (int jq = 0; jq < 4; jq++) {
-                const int s0 = eb * 32 + 8 * jq + 4 * hsel;
                 int hi[4], lo[4];
 #pragma unroll
                 for (int u = 0; u < 4; u++) {
                     const int j = 4 * jq + u;
-                    const int nb_ = (int)((negmask >> (s0 + u)) & 1ull);
-                    const double x = (double)(ah[eb][j] * 128 + al[eb][j] + c1 - (nb_ ? c2 : 0));
+                    const double x = (double)(ah[eb][j] * 128 + al[eb][j] + c1);
                     const int c = (int)fma(-rint(x * mi), md, x);        // centred residue
                     hi[u] = (c + 64) >> 7;
                     lo[u] = c - (hi[u] << 7);
@@ -758,19 +751,19 @@ extern "C" {
 int sdpv_rns_split(const int32_t* X, const int64_t* e, const int64_t* Erow, int64_t L, int64_t Lout, int64_t plane,
                    int64_t nb, int64_t rows, int64_t ldb, int64_t ldr, int64_t ldk, int64_t k0, int64_t kc,
                    int64_t kp, const int8_t* Thi, const int8_t* Tlo, int64_t NBp, const int32_t* C1,
-                   const int32_t* C2, const int32_t* mods, const double* minv, int64_t g0, int64_t G, int8_t* Rh,
+                   const int32_t* mods, const double* minv, int64_t g0, int64_t G, int8_t* Rh,
                    int8_t* Rl, int32_t* flag, void* stream) {
     if (L < 1 || Lout < 1 || nb < 0 || rows < 0 || kc < 0 || kp < kc || G < 0 || g0 < 0) return SDPV_EARG;
     if (kp % 64 != 0 || NBp != 32 * ((4 * Lout + 31) / 32)) return SDPV_EARG;
     if (Lout > 128 || L > 128) return SDPV_ESIZE;
     const i64 nrows = nb * rows;
     if (nrows == 0 || kp == 0 || G == 0) return SDPV_OK;
-    if (!X || !Thi || !Tlo || !C1 || !C2 || !mods || !minv || !Rh || !Rl) return SDPV_EARG;
+    if (!X || !Thi || !Tlo || !C1 || !mods || !minv || !Rh || !Rl) return SDPV_EARG;
     if (e && !Erow) return SDPV_EARG;
     const i64 ntiles_k = kp / 16, ntiles_r = (nrows + 15) / 16;
     const i64 grid = ntiles_r * ntiles_k;
     if (grid > 0x7fffffffLL) return SDPV_ESIZE;
-    size_t lds = (size_t)256 * (NBp + 16) + 256;
+    size_t lds = (size_t)256 * (NBp + 16);
     const size_t stage_bytes = (size_t)256 * 4 * (L + 1);
     const int stage = lds + stage_bytes <= 160 * 1024;   // stage the limbs when they fit
     if (stage) lds += stage_bytes;
@@ -786,8 +779,8 @@ int sdpv_rns_split(const int32_t* X, const int64_t* e, const int64_t* Erow, int6
         }                                                                                                      \
         hipLaunchKernelGGL((k_rns_split<KSC>), dim3((unsigned)grid), dim3(256), lds, as_stream(stream), X, e,   \
                            Erow, (int)L, (int)Lout, (i64)plane, (i64)nb, (i64)rows, (i64)ldb, (i64)ldr, (i64)ldk, \
-                           (i64)k0, (i64)kc, (i64)kp, Thi, Tlo, (int)NBp, C1, C2, mods, minv, (int)g0, (int)G, Rh, \
-                           Rl, flag, ntiles_k, stage);                                                         \
+                           (i64)k0, (i64)kc, (i64)kp, Thi, Tlo, (int)NBp, C1, mods, minv, (int)g0, (int)G, Rh, Rl, \
+                           flag, ntiles_k, stage);                                                             \
     } while (0)
     switch (ks) {
         case 1: SDPV_SPLIT_LAUNCH(1); break;

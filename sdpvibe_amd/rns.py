@@ -165,8 +165,8 @@ class RNS:
         T_hi = (T - T_lo) // 128
         self.Thi = t(T_hi, torch.int8).contiguous()
         self.Tlo = t(T_lo, torch.int8).contiguous()
-        self.C1 = t([(128 * int(T[i].sum())) % m for i, m in enumerate(mods)], torch.int32)
-        self.C2 = t([pow(2, 32 * L, m) for m in mods], torch.int32)
+        # every byte but the top one enters as u - 128, the top one signed: C1 restores the bias
+        self.C1 = t([(128 * int(T[i, :4 * L - 1].sum())) % m for i, m in enumerate(mods)], torch.int32)
         self.kcoef = t([[16256 % m, 128 % m, (-127) % m] for m in mods], torch.int32).contiguous()
         inv = [pow(M // m % m, -1, m) for m in mods]
         # coefficients scaled by (M/m)^-1: the product then stores the CRT's y_g directly
@@ -224,7 +224,7 @@ class RNS:
             if e is not None and (e.numel() < plane or Erow is None or Erow.numel() < nrows):
                 raise ValueError("split: exponent arrays too small")
         call("sdpv_rns_split", ptr(X), ptr(e), ptr(Erow), Lin, self.L, plane, nb, rows, ldb, ldr, ldk, k0, kc,
-             kp, ptr(self.Thi), ptr(self.Tlo), self.NBp, ptr(self.C1), ptr(self.C2), ptr(self.m32),
+             kp, ptr(self.Thi), ptr(self.Tlo), self.NBp, ptr(self.C1), ptr(self.m32),
              ptr(self.minv), g0, G, ptr(Rh), ptr(Rl), ptr(flag), stream())
 
     # ---- step 2: accumulate residues of A @ Bt^T

@@ -59,8 +59,7 @@ def test_tables(ctxs):
             m = R.mods[gi]
             assert all((int(T[gi, j]) - pow(256, j, m)) % m == 0 and abs(int(T[gi, j])) <= m // 2
                        for j in range(4 * L))
-            assert int(R.C1[gi]) == (128 * int(T[gi].sum())) % m
-            assert int(R.C2[gi]) == pow(2, 32 * L, m)
+            assert int(R.C1[gi]) == (128 * int(T[gi, :4 * L - 1].sum())) % m
             assert (int(R.inv32[gi]) * (R.M // m)) % m == 1
             dig = R.Ct.numpy()[:, gi].astype(np.int64)
             assert sum(int(d) << (8 * k) for k, d in enumerate(dig)) == R.M // m
@@ -75,14 +74,15 @@ def test_tables(ctxs):
 
 
 def _model_residues(R, vals, L):
-    """numpy model of the split kernel: bytes (u - 128) . T + C1 - neg C2 (mod m), centred"""
+    """numpy model of the split kernel: bytes u - 128 (the top byte signed) . T + C1 (mod m), centred"""
     X = to_planes(vals, L).T.copy().view(np.uint8).reshape(len(vals), 4 * L).astype(np.int64)
     xt = X - 128
-    neg = (X[:, -1] >= 128).astype(np.int64)
+    xt[:, -1] = np.where(X[:, -1] >= 128, X[:, -1] - 256, X[:, -1])   # signed top byte
+    assert xt.min() >= -128 and xt.max() <= 127
     Th = R.Thi.numpy()[:R.n, :4 * L].astype(np.int64)
     Tl = R.Tlo.numpy()[:R.n, :4 * L].astype(np.int64)
     m = np.array(R.mods, np.int64)
-    x = 128 * (xt @ Th.T) + xt @ Tl.T + R.C1.numpy()[None, :] - neg[:, None] * R.C2.numpy()[None, :]
+    x = 128 * (xt @ Th.T) + xt @ Tl.T + R.C1.numpy()[None, :]
     r = x % m[None, :]
     return np.where(r > m // 2, r - m, r)                     # (entries, n)
 
