@@ -40,7 +40,46 @@ def parse():
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
     ap.add_argument("--no-batched", action="store_true", help="skip the uneven-block Cholesky comparison")
     ap.add_argument("--blocks", type=int, default=400)
+    ap.add_argument("--no-gram", action="store_true", help="skip the Gram accumulation measurement")
     return ap.parse_args()
+
+
+def gram_acc(L, N=3000, rows=1024, chunks=4):
+    """Schur-complement assembly Q = sum_c P_c^T P_c (mpf.GramAcc: residues of ctx(L + 1), syrk
+    products on the lower tiles, one CRT at the end) for N columns and `chunks` chunks of `rows`
+    Gaussian MPF rows (one chunk drawn, added `chunks` times). Reports mp-MAC/s over the lower
+    triangle and the time of the final reconstruction."""
+    import numpy as np
+    import torch
+    from sdpvibe_amd import mpf as F
+    from sdpvibe_amd.synthetic import gaussian_mpf
+
+    pm, pe = gaussian_mpf((rows, N), L, seed=7)
+    P = F.MPF(torch.from_numpy(pm).cuda(), torch.from_numpy(pe).cuda())
+    del pm, pe
+
+    def run():
+        G = F.GramAcc(N, L)
+        for _ in range(chunks):
+            G.add(P)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        Q = G.result()
+        torch.cuda.synchronize()
+        return Q, time.perf_counter() - t1
+
+    run()                                                 # warm (autotuner, attributes)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    Q, t_crt = run()
+    dt = time.perf_counter() - t0
+    mac = chunks * rows * N * (N + 1) // 2
+    R = F.ctx(L + 1)
+    ops = 2 * (4 * R.n * mac + 2 * R.n * 4 * (L + 1) * chunks * rows * N + 2 * R.E * R.n * N * (N + 1) // 2)
+    del Q
+    return {"workload": f"GramAcc Q = sum P^T P, N={N}, {chunks} chunks of {rows} rows, L={L} (residues of L+1 "
+                        f"limbs, {R.n} primes)", "seconds": round(dt, 4), "crt_seconds": round(t_crt, 4),
+            "mp_mac_per_s": mac / dt, "pct_int8_peak": round(100.0 * ops / dt / (PEAK_INT8_TOPS * 1e12), 3)}
 
 
 def batched_chol(L, nblocks, seed=0, rhs=4):
@@ -243,6 +282,12 @@ def main():
     roof["traffic"] = measured_traffic(roof["kernel"], cfg if cfg is not None else -1, n, L)
 
     cpu = None if args.no_cpu_baseline else cpu_baseline(n, args.cpu_threads, L)
+    gram = None
+    if not args.no_gram:
+        try:                                              # auxiliary measurement: never loses the line
+            gram = gram_acc(L)
+        except Exception as exc:                          # noqa: BLE001
+            gram = {"error": f"{type(exc).__name__}: {exc}"}
     bchol = None
     if not args.no_batched:
         try:                                              # auxiliary measurement: never loses the line
@@ -262,7 +307,7 @@ def main():
         "executed_int8_ops_per_step": ops_total,
         "pct_peak_executed_ops": round(100.0 * ops_total / dt / (PEAK_INT8_TOPS * 1e12), 3),
         "phase_ms": {p: round(v, 4) for p, v in ph.items()},
-        "roofline": roof, "cpu_baseline": cpu, "batched_chol": bchol,
+        "roofline": roof, "cpu_baseline": cpu, "batched_chol": bchol, "gram_acc": gram,
     }
     print(json.dumps(line))
 

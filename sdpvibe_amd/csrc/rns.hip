@@ -808,7 +808,7 @@ int sdpv_rns_gemm(const int8_t* Ah, const int8_t* Al, const int8_t* Bh, const in
     if (kp == 0) return SDPV_EARG;
     if (config == -1) {
         const i64 mn = M < N ? M : N;  // syrk: square tiles of 128 / 64 / 32 by N, as the reference
-        config = mn >= 128 ? 0 : (mn >= 64 ? 2 : 3);
+        config = mn >= 128 ? 6 : (mn >= 64 ? 2 : 3);   // 128x128 four-product tile: the fastest
     }
     const int g = (int)G, b = (int)nb, q0 = (int)g0;
     switch (config) {
