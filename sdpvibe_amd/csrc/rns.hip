@@ -476,7 +476,11 @@ int launch_gemm(const int8_t* Ah, const int8_t* Al, const int8_t* Bh, const int8
 // Phase 3 (lane = entry): words -> W + bitlen (mode 0), normalised MPF limbs (1), or a
 //   fixed-exponent rounding (2).
 
-template <bool SCALED>
+// NDB > 0 (modes 1, 2 with that many digit blocks): the words of an entry stay in registers
+// during phase 2 and move into the then unused byte rows afterwards, so a wave needs only the
+// byte rows of LDS (more waves per CU); NDB = 0: words in their own LDS rows, digit blocks
+// pipelined.
+template <bool SCALED, int NDB>
 __global__ void __launch_bounds__(64) k_rns_crt(
     const int16_t* __restrict__ acc, int n, i64 ld_g, i64 rows, i64 N, i64 j0, i64 w, const i32* __restrict__ mods,
     const double* __restrict__ minv, const i32* __restrict__ inv, const int8_t* __restrict__ Ct, int E, int npad,
@@ -490,7 +494,8 @@ __global__ void __launch_bounds__(64) k_rns_crt(
     const int lane = threadIdx.x, l32 = lane & 31, half = lane >> 5;
     int8_t* Yh = (int8_t*)smem;                       // [64][YS]
     int8_t* Yl = Yh + 64 * YS;                        // [64][YS]
-    u32* wrow = (u32*)(Yl + 64 * YS) + (size_t)lane * (NW + 1);   // [64][NW + 1] (modes 1, 2)
+    u32* wrow = (u32*)(NDB > 0 ? (unsigned char*)smem : (unsigned char*)(Yl + 64 * YS)) + (size_t)lane * (NW + 1);
+                                                      // [64][NW + 1] words (modes 1, 2)
 
     const i64 Bq = rows * w;
     const i64 q = (i64)blockIdx.x * 64 + lane;
@@ -587,7 +592,7 @@ __global__ void __launch_bounds__(64) k_rns_crt(
             z[3] = mfma(c, *(const v4i*)(y1l + o), z[3]);
         }
     };
-    auto fold = [&](int db, const v16i* z) {
+    auto fold = [&](int db, const v16i* z, u32* word) {
         // word partials sum_u Z_u 256^u, Z_u = 128 zh + zl (|Z_u| < 2^31); X: group 0, Y: group 1
         u32 xl[4], xh[4], yl[4], yh[4];
 #pragma unroll
@@ -608,7 +613,6 @@ __global__ void __launch_bounds__(64) k_rns_crt(
             swap_halves(xl[wq], yl[wq]);
             swap_halves(xh[wq], yh[wq]);
         }
-        u32 word[8];
 #pragma unroll
         for (int h2 = 0; h2 < 8; h2++) {
             const int wq = h2 >> 1;
@@ -617,6 +621,8 @@ __global__ void __launch_bounds__(64) k_rns_crt(
             word[h2] = (u32)v;
             carry = v >> 32;
         }
+    };
+    auto store_words = [&](int db, const u32* word) {
         const int nw = NW - db * 8;                    // words of this block inside the value
         if (mode == SDPV_CRT_WORDS) {
 #pragma unroll
@@ -635,23 +641,45 @@ __global__ void __launch_bounds__(64) k_rns_crt(
                 if (h2 < nw) wrow[db * 8 + h2] = word[h2];
         }
     };
-    v4i cfA[KF], cfB[KF];
-    v16i zA[4], zB[4];
-    load_chunks(0, cfA);
-    contract(0, cfA, zA);
-    if (nDB > 1) load_chunks(1, cfB);
-    for (int db = 0; db < nDB; db += 2) {
-        if (db + 1 < nDB) {
-            contract(db + 1, cfB, zB);
-            if (db + 2 < nDB) load_chunks(db + 2, cfA);
+    if constexpr (NDB > 0) {
+        u32 wv[NDB * 8];                                // the entry's words (static indices)
+        v4i cfc[KF], cfn[KF];
+        load_chunks(0, cfc);
+#pragma unroll
+        for (int db = 0; db < NDB; db++) {
+            if (db + 1 < NDB) load_chunks(db + 1, cfn);
+            v16i z[4];
+            contract(db, cfc, z);
+            fold(db, z, wv + 8 * db);
+#pragma unroll
+            for (int u = 0; u < KF; u++) cfc[u] = cfn[u];
         }
-        fold(db, zA);
-        if (db + 1 < nDB) {
-            if (db + 2 < nDB) {
-                contract(db + 2, cfA, zA);
-                if (db + 3 < nDB) load_chunks(db + 3, cfB);
+        wave_sync();                                    // the byte rows are read: reuse them
+#pragma unroll
+        for (int i = 0; i < NDB * 8; i++)
+            if (i < NW) wrow[i] = wv[i];
+    } else {
+        v4i cfA[KF], cfB[KF];
+        v16i zA[4], zB[4];
+        u32 word[8];
+        load_chunks(0, cfA);
+        contract(0, cfA, zA);
+        if (nDB > 1) load_chunks(1, cfB);
+        for (int db = 0; db < nDB; db += 2) {
+            if (db + 1 < nDB) {
+                contract(db + 1, cfB, zB);
+                if (db + 2 < nDB) load_chunks(db + 2, cfA);
             }
-            fold(db + 1, zB);
+            fold(db, zA, word);
+            store_words(db, word);
+            if (db + 1 < nDB) {
+                if (db + 2 < nDB) {
+                    contract(db + 2, cfA, zA);
+                    if (db + 3 < nDB) load_chunks(db + 3, cfB);
+                }
+                fold(db + 1, zB, word);
+                store_words(db + 1, word);
+            }
         }
     }
 
@@ -844,26 +872,39 @@ int sdpv_rns_crt(const int16_t* acc, int64_t n, int64_t ld_g, int64_t rows, int6
     if (npad > 1024) return SDPV_ESIZE;  // digit sums 128*Zh + Zl stay within int32
     if ((rows - 1) * N + j0 + w > ld_g || ld_g >= (1LL << 31)) return SDPV_ESIZE;  // 32-bit plane offsets
     const i64 NW = E / 4;
-    const size_t lds = (size_t)128 * (npad + 16) + (omode != SDPV_CRT_WORDS ? (size_t)256 * (NW + 1) : 0);
+    const int ndb = (int)((E + 31) / 32);
+    const bool scaled = (mode & SDPV_CRT_SCALED) != 0;
+    // register-held words: scaled input, limb output, 5..8 digit blocks (about L = 16..31; more
+    // words would cost the wave count that the smaller LDS gains)
+    const int reg = (scaled && omode != SDPV_CRT_WORDS && ndb >= 5 && ndb <= 8) ? ndb : 0;
+    const size_t ybytes = (size_t)128 * (npad + 16), wbytes = (size_t)256 * (NW + 1);
+    const size_t lds = omode == SDPV_CRT_WORDS ? ybytes : (reg ? (ybytes > wbytes ? ybytes : wbytes) : ybytes + wbytes);
     if (lds > 160 * 1024) return SDPV_ESIZE;
     const i64 grid = (Bq + 63) / 64;
     if (grid > 0x7fffffffLL) return SDPV_ESIZE;
     hipStream_t st = as_stream(stream);
-    const bool scaled = (mode & SDPV_CRT_SCALED) != 0;
-    static bool attr[2] = {false, false};
-    if (!attr[scaled]) {
-        (void)hipFuncSetAttribute(scaled ? (const void*)k_rns_crt<true> : (const void*)k_rns_crt<false>,
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-        attr[scaled] = true;
+#define SDPV_CRT_LAUNCH(SC, ND)                                                                                \
+    do {                                                                                                       \
+        static bool attr = false;                                                                              \
+        if (!attr) {                                                                                           \
+            (void)hipFuncSetAttribute((const void*)k_rns_crt<SC, ND>, hipFuncAttributeMaxDynamicSharedMemorySize, \
+                                      160 * 1024);                                                             \
+            attr = true;                                                                                       \
+        }                                                                                                      \
+        hipLaunchKernelGGL((k_rns_crt<SC, ND>), dim3((unsigned)grid), dim3(64), lds, st, acc, (int)n, (i64)ld_g, \
+                           (i64)rows, (i64)N, (i64)j0, (i64)w, mods, minv, inv, Ct, (int)E, (int)npad, omode, W,  \
+                           bitlen, outm, oute, (int)L, (i64)plane, (i64)ldo, (i64)jo, (i64)Mb, erow, ecol, scol,  \
+                           flag);                                                                              \
+    } while (0)
+    if (!scaled) SDPV_CRT_LAUNCH(false, 0);
+    else switch (reg) {
+        case 5: SDPV_CRT_LAUNCH(true, 5); break;
+        case 6: SDPV_CRT_LAUNCH(true, 6); break;
+        case 7: SDPV_CRT_LAUNCH(true, 7); break;
+        case 8: SDPV_CRT_LAUNCH(true, 8); break;
+        default: SDPV_CRT_LAUNCH(true, 0); break;
     }
-    if (scaled)
-        hipLaunchKernelGGL((k_rns_crt<true>), dim3((unsigned)grid), dim3(64), lds, st, acc, (int)n, (i64)ld_g,
-                           (i64)rows, (i64)N, (i64)j0, (i64)w, mods, minv, inv, Ct, (int)E, (int)npad, omode, W,
-                           bitlen, outm, oute, (int)L, (i64)plane, (i64)ldo, (i64)jo, (i64)Mb, erow, ecol, scol, flag);
-    else
-        hipLaunchKernelGGL((k_rns_crt<false>), dim3((unsigned)grid), dim3(64), lds, st, acc, (int)n, (i64)ld_g,
-                           (i64)rows, (i64)N, (i64)j0, (i64)w, mods, minv, inv, Ct, (int)E, (int)npad, omode, W,
-                           bitlen, outm, oute, (int)L, (i64)plane, (i64)ldo, (i64)jo, (i64)Mb, erow, ecol, scol, flag);
+#undef SDPV_CRT_LAUNCH
     SDPV_TRY_LAUNCH();
     return SDPV_OK;
 }

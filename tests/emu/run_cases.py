@@ -43,6 +43,8 @@ CASES = {
     "gemm_small": lambda: gemm_case(1, 1, 3, 4, 5, False, False),
     "gemm_trans": lambda: all(gemm_case(2, 1, 5, 7, 6, ta, tb, 3) for ta in (False, True) for tb in (False, True)),
     "gemm_batched": lambda: gemm_case(3, 2, 33, 20, 40, False, True, 4),
+    # the CRT's register-held words: 5, 6 and 8 digit blocks (7: the L = 24 cases)
+    "gemm_regwords": lambda: all(gemm_case(L, 1, 9, 12, 10, False, True, L) for L in (16, 20, 28)),
 }
 
 def gpu_test_gemm_cases():
