@@ -103,22 +103,23 @@ def _measure(R, nb, M, N, kp, reps=3):
                                     generator=gen)
     Ah, Al, Bh, Bl = mk(M), mk(M), mk(N), mk(N)
     acc = torch.empty((G, nb * M, N), dtype=torch.int16, device=dev_)
-    best, best_t = -1, None
-    for cfg in GEMM_CONFIGS:
-        st = torch.cuda.current_stream()
-        call("sdpv_rns_gemm", ptr(Ah), ptr(Al), ptr(Bh), ptr(Bl), G, nb, M, N, kp, ptr(acc), nb * M * N,
-             ptr(R.m32), ptr(R.minv), ptr(R.kcoef), 0, 0, cfg, stream())          # warm (attributes)
-        ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        ev0.record(st)
-        for _ in range(reps):
-            call("sdpv_rns_gemm", ptr(Ah), ptr(Al), ptr(Bh), ptr(Bl), G, nb, M, N, kp, ptr(acc), nb * M * N,
-                 ptr(R.m32), ptr(R.minv), ptr(R.kcoef), 0, 0, cfg, stream())
-        ev1.record(st)
-        ev1.synchronize()
-        t = ev0.elapsed_time(ev1)
-        if best_t is None or t < best_t:
-            best, best_t = cfg, t
-    return best
+    st = torch.cuda.current_stream()
+    launch = lambda cfg: call("sdpv_rns_gemm", ptr(Ah), ptr(Al), ptr(Bh), ptr(Bl), G, nb, M, N, kp, ptr(acc),
+                              nb * M * N, ptr(R.m32), ptr(R.minv), ptr(R.kcoef), 0, 0, cfg, stream())
+    for cfg in GEMM_CONFIGS:                          # warm every configuration (attributes, clocks)
+        launch(cfg)
+    best_t = {}
+    for _ in range(2):                                # two passes, the faster of each: clock drift
+        for cfg in GEMM_CONFIGS:
+            ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            ev0.record(st)
+            for _ in range(reps):
+                launch(cfg)
+            ev1.record(st)
+            ev1.synchronize()
+            t = ev0.elapsed_time(ev1)
+            best_t[cfg] = min(t, best_t.get(cfg, t))
+    return min(GEMM_CONFIGS, key=lambda c: best_t[c])
 
 
 def row_major(R):
