@@ -224,22 +224,18 @@ __global__ void __launch_bounds__(256) k_rns_split(
                 dl[eb][jq] = pack4(lo[0], lo[1], lo[2], lo[3]);
             }
         }
-        // swap runs across the halves: half 0 keeps rows 2 eb (jq 0, 1), half 1 rows 2 eb + 1 (jq 2, 3)
+        // swap runs across the halves: half 0 stores row 2 eb (jq 0, 1 of both halves), half 1 row
+        // 2 eb + 1 (jq 2, 3). Swapping (jq 0, jq 2) gives half 0 (own jq 0 = k 0-3, partner's = k 4-7)
+        // and half 1 (partner's jq 2 = k 0-3, own = k 4-7); (jq 1, jq 3) likewise for k 8-15.
 #pragma unroll
         for (int eb = 0; eb < 2; eb++) {
-            // runs this lane gives away (half 0: jq 2, 3; half 1: jq 0, 1) and keeps (the others)
-            const u32 gh0 = hsel ? dh[eb][0] : dh[eb][2], gh1 = hsel ? dh[eb][1] : dh[eb][3];
-            const u32 gl0 = hsel ? dl[eb][0] : dl[eb][2], gl1 = hsel ? dl[eb][1] : dl[eb][3];
-            const u32 kh0 = hsel ? dh[eb][2] : dh[eb][0], kh1 = hsel ? dh[eb][3] : dh[eb][1];
-            const u32 kl0 = hsel ? dl[eb][2] : dl[eb][0], kl1 = hsel ? dl[eb][3] : dl[eb][1];
-            const u32 rh0 = (u32)__shfl_xor((int)gh0, 32), rh1 = (u32)__shfl_xor((int)gh1, 32);
-            const u32 rl0 = (u32)__shfl_xor((int)gl0, 32), rl1 = (u32)__shfl_xor((int)gl1, 32);
-            // half 0: k 0-3 own, 4-7 partner, 8-11 own, 12-15 partner; half 1 the other way round
+            swap_halves(dh[eb][0], dh[eb][2]);
+            swap_halves(dh[eb][1], dh[eb][3]);
+            swap_halves(dl[eb][0], dl[eb][2]);
+            swap_halves(dl[eb][1], dl[eb][3]);
             v4i vh, vl;
-            vh[0] = (int)(hsel ? rh0 : kh0); vh[1] = (int)(hsel ? kh0 : rh0);
-            vh[2] = (int)(hsel ? rh1 : kh1); vh[3] = (int)(hsel ? kh1 : rh1);
-            vl[0] = (int)(hsel ? rl0 : kl0); vl[1] = (int)(hsel ? kl0 : rl0);
-            vl[2] = (int)(hsel ? rl1 : kl1); vl[3] = (int)(hsel ? kl1 : rl1);
+            vh[0] = (int)dh[eb][0]; vh[1] = (int)dh[eb][2]; vh[2] = (int)dh[eb][1]; vh[3] = (int)dh[eb][3];
+            vl[0] = (int)dl[eb][0]; vl[1] = (int)dl[eb][2]; vl[2] = (int)dl[eb][1]; vl[3] = (int)dl[eb][3];
             const i64 R = row0 + 4 * wave + 2 * eb + hsel;
             if (pok && R < nrows) {
                 const i64 o = (((i64)pr * nck + tk) * nrows + R) * 16;
