@@ -140,8 +140,9 @@ int sdpv_rns_split(const int32_t* X, const int64_t* e, const int64_t* Erow, int6
  * 128x128 (2 stages), 128x128 (3 stages), 64x64, 32x32, 256x128 and 128x256 tiles with three
  * INT8 products per prime (the sum plane h + l formed in registers); 6 and 7 are the 128x128 and
  * 128x256 tiles with four products (HH, LL, HL + LH; no bytewise adds); 8 and 9 the 128x256 tile
- * with 3 stages, four and three products. Every configuration computes the same residues (an
- * autotuner picks by time). */
+ * with 3 stages, four and three products; 10 and 11 the 128x64 and 64x128 tiles with four
+ * products (fewer registers, more workgroups per CU). Every configuration computes the same
+ * residues (an autotuner picks by time). */
 int sdpv_rns_gemm(const int8_t* Ah, const int8_t* Al, const int8_t* Bh, const int8_t* Bl,
                   int64_t G, int64_t nb, int64_t M, int64_t N, int64_t kp, int16_t* acc,
                   int64_t ld_g, const int32_t* mods, const double* minv, const int32_t* kcoef,

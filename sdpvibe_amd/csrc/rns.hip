@@ -822,7 +822,7 @@ int sdpv_rns_gemm(const int8_t* Ah, const int8_t* Al, const int8_t* Bh, const in
                   int64_t M, int64_t N, int64_t kp, int16_t* acc, int64_t ld_g, const int32_t* mods,
                   const double* minv, const int32_t* kcoef, int64_t g0, int32_t mode, int32_t config, void* stream) {
     if (G < 0 || nb < 0 || M < 0 || N < 0 || kp < 0 || g0 < 0 || (mode & ~7)) return SDPV_EARG;
-    if (config < -1 || config > 9) return SDPV_EARG;
+    if (config < -1 || config > 11) return SDPV_EARG;
     if (kp % 64 != 0 || kp > 65536) return SDPV_ESIZE;
     if (nb * M > (1LL << 26) || nb * N > (1LL << 26)) return SDPV_ESIZE;   // 32-bit offsets in a stage
     if ((mode & SDPV_GEMM_SYRK_LOWER) && (M != N || config != -1)) return SDPV_EARG;
@@ -845,7 +845,9 @@ int sdpv_rns_gemm(const int8_t* Ah, const int8_t* Al, const int8_t* Bh, const in
         case 6: return launch_gemm<2, 2, 2, 2, 2, 1>(Ah, Al, Bh, Bl, g, b, M, N, kp, acc, ld_g, mods, minv, kcoef, q0, mode, st);
         case 7: return launch_gemm<2, 4, 2, 2, 2, 1>(Ah, Al, Bh, Bl, g, b, M, N, kp, acc, ld_g, mods, minv, kcoef, q0, mode, st);
         case 8: return launch_gemm<2, 4, 2, 2, 3, 1>(Ah, Al, Bh, Bl, g, b, M, N, kp, acc, ld_g, mods, minv, kcoef, q0, mode, st);
-        default: return launch_gemm<2, 4, 2, 2, 3, 0>(Ah, Al, Bh, Bl, g, b, M, N, kp, acc, ld_g, mods, minv, kcoef, q0, mode, st);
+        case 9: return launch_gemm<2, 4, 2, 2, 3, 0>(Ah, Al, Bh, Bl, g, b, M, N, kp, acc, ld_g, mods, minv, kcoef, q0, mode, st);
+        case 10: return launch_gemm<2, 2, 2, 1, 2, 1>(Ah, Al, Bh, Bl, g, b, M, N, kp, acc, ld_g, mods, minv, kcoef, q0, mode, st);
+        default: return launch_gemm<2, 2, 1, 2, 2, 1>(Ah, Al, Bh, Bl, g, b, M, N, kp, acc, ld_g, mods, minv, kcoef, q0, mode, st);
     }
 }
 
