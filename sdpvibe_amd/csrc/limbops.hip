@@ -291,27 +291,29 @@ __global__ void __launch_bounds__(256) k_row_exp_wave(const i32* __restrict__ X,
     if (lane == 0) Erow[R] = row_exp_of(mx, L);
 }
 
-// other layouts: 64 rows (lanes) per block of 16 waves, wave w takes k = w, w + 16, ...;
-// the 16 partial maxima per row meet in LDS
+// other layouts: 64 rows (lanes) per block of 16 waves, wave w takes k = w, w + 16, ... of the
+// block's k range (blockIdx.y: one of gridDim.y parts of kchunk); the 16 partial maxima per row
+// meet in LDS. With several k parts the rows meet by an atomic maximum over a filled Erow.
 __global__ void __launch_bounds__(1024) k_row_exp_cols(const i32* __restrict__ X, const i64* __restrict__ e, int L,
                                                        i64 plane, i64 nrows, i64 rows, i64 K, i64 ldb, i64 ldr,
-                                                       i64 ldk, i64* __restrict__ Erow) {
+                                                       i64 ldk, i64 kchunk, i64* __restrict__ Erow) {
     __shared__ i64 part[16][64];
     const int lane = threadIdx.x & 63, kg = threadIdx.x >> 6;
     const i64 R = blockIdx.x * (i64)64 + lane;
+    const i64 kb = (i64)blockIdx.y * kchunk, ke = kb + kchunk < K ? kb + kchunk : K;
     i64 mx = EZV;
     if (R < nrows) {
         const i64 b = R / rows, r = R % rows;
         const i64 base = b * ldb + r * ldr;
         const i32* top = X + (i64)(L - 1) * plane;
-        for (i64 k0 = kg; k0 < K; k0 += 8 * 16) {
+        for (i64 k0 = kb + kg; k0 < ke; k0 += 8 * 16) {
             i64 ex[8];
             u32 tw[8];
 #pragma unroll
             for (int u = 0; u < 8; u++) {
                 const i64 k = k0 + u * 16;
-                ex[u] = k < K ? e[base + k * ldk] : EZV;
-                tw[u] = k < K ? (u32)top[base + k * ldk] : 0u;
+                ex[u] = k < ke ? e[base + k * ldk] : EZV;
+                tw[u] = k < ke ? (u32)top[base + k * ldk] : 0u;
             }
 #pragma unroll
             for (int u = 0; u < 8; u++) {
@@ -324,8 +326,15 @@ __global__ void __launch_bounds__(1024) k_row_exp_cols(const i32* __restrict__ X
     __syncthreads();
     if (kg == 0 && R < nrows) {
         for (int q = 1; q < 16; q++) mx = part[q][lane] > mx ? part[q][lane] : mx;
-        Erow[R] = row_exp_of(mx, L);
+        const i64 v = row_exp_of(mx, L);               // monotone in mx: the maximum commutes
+        if (gridDim.y == 1) Erow[R] = v;
+        else if (v != EZV) atomicMax((long long*)&Erow[R], (long long)v);
     }
+}
+
+__global__ void __launch_bounds__(256) k_fill_i64(i64* __restrict__ p, i64 n, i64 v) {
+    const i64 i = blockIdx.x * (i64)256 + threadIdx.x;
+    if (i < n) p[i] = v;
 }
 
 inline unsigned nblocks(i64 n, int tpb) { return (unsigned)((n + tpb - 1) / tpb); }
@@ -441,8 +450,20 @@ int sdpv_row_exp(const int32_t* X, const int64_t* e, int64_t L, int64_t plane, i
         hipLaunchKernelGGL(k_row_exp_wave, dim3(nblocks(nrows, 4)), dim3(256), 0, st, X, e, (int)L, (i64)plane, nrows,
                            (i64)rows, (i64)K, (i64)ldb, (i64)ldr, Erow);
     } else {
-        hipLaunchKernelGGL(k_row_exp_cols, dim3(nblocks(nrows, 64)), dim3(1024), 0, st, X, e, (int)L, (i64)plane,
-                           nrows, (i64)rows, (i64)K, (i64)ldb, (i64)ldr, (i64)ldk, Erow);
+        // few row groups: split k as well, so that the launch fills the device
+        const i64 groups = (nrows + 63) / 64;
+        i64 parts = 1;
+        if (groups < 128 && K > 256) {
+            parts = (256 + groups - 1) / groups;
+            const i64 kmax = (K + 127) / 128;
+            parts = parts < kmax ? parts : kmax;
+            if (parts > 65535) parts = 65535;
+        }
+        const i64 kchunk = (K + parts - 1) / parts;
+        if (parts > 1)
+            hipLaunchKernelGGL(k_fill_i64, dim3(nblocks(nrows, 256)), dim3(256), 0, st, Erow, nrows, (i64)EZV);
+        hipLaunchKernelGGL(k_row_exp_cols, dim3((unsigned)groups, (unsigned)parts), dim3(1024), 0, st, X, e, (int)L,
+                           (i64)plane, nrows, (i64)rows, (i64)K, (i64)ldb, (i64)ldr, (i64)ldk, kchunk, Erow);
     }
     SDPV_TRY_LAUNCH();
     return SDPV_OK;

@@ -16,7 +16,7 @@ def emu_built():
     harness.build()
 
 
-@pytest.mark.parametrize("case", ["gemm_small", "gemm_trans", "gemm_batched", "bgemm", "tile", "chol", "crt", "gram", "linalg", "eig", "product", "residues", "carry", "small_round", "bucketed", "gemm_regwords"])
+@pytest.mark.parametrize("case", ["gemm_small", "gemm_trans", "gemm_batched", "bgemm", "tile", "chol", "crt", "gram", "linalg", "eig", "product", "residues", "carry", "small_round", "bucketed", "gemm_regwords", "gemm_ksplit"])
 def test_emulated(emu_built, case):
     r = subprocess.run([sys.executable, "tests/emu/run_cases.py", case], env=harness.asan_env(),
                        capture_output=True, text=True, timeout=900, cwd=harness.HERE + "/../..")
