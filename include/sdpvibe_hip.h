@@ -95,8 +95,10 @@ int sdpv_mpf_norm(const int32_t* X, const int64_t* eI, int64_t LI, int32_t* O, i
 
 /* Row alignment exponents: Erow[b*rows + r] = max_k top(b, r, k) - (32 L - 2) over the entries
  * k in [0, K) with e != SDPV_EZ; SDPV_EZ when there is none (the reference then uses 0, which no
- * product can observe: every entry of such a row is zero). Two launches: fill, then a k-split
- * reduction with atomic max. Replaces the exponent half of mpf._align_rows (sdpvibe/mpf.py:390-398). */
+ * product can observe: every entry of such a row is zero). Rows contiguous in k: one launch, one
+ * wave per row. Other layouts: one launch when there are enough groups of 64 rows, else a fill
+ * and a launch split over k that meets by atomic maximum. Replaces the exponent half of
+ * mpf._align_rows (sdpvibe/mpf.py:390-398). */
 int sdpv_row_exp(const int32_t* X, const int64_t* e, int64_t L, int64_t plane, int64_t nb,
                  int64_t rows, int64_t K, int64_t ldb, int64_t ldr, int64_t ldk, int64_t* Erow,
                  void* stream);
