@@ -392,6 +392,9 @@ __global__ void __launch_bounds__(WM* WN * 64, (WM * WN <= 4) ? 2 : 1) k_rns_gem
 #pragma unroll
     for (int s = 0; s < NST - 1; s++)
         if (s < nst) issue(s);
+    // the MFMA loop runs at a raised wave priority, the epilogue at the normal one: a wave in its
+    // epilogue yields issue slots to a workgroup still in its main loop on the same SIMD
+    __builtin_amdgcn_s_setprio(1);
     for (int s = 0; s < nst; s++) {
         // this wave's loads of stage s are done once at most (stages issued after s) x LPW remain
         if (s + NST - 2 < nst) SDPV_WAIT_VMCNT((NST - 2) * LPW);
@@ -401,6 +404,7 @@ __global__ void __launch_bounds__(WM* WN * 64, (WM * WN <= 4) ? 2 : 1) k_rns_gem
         compute(s);
     }
 
+    __builtin_amdgcn_s_setprio(0);
     // Epilogue: v = k0*HH + k1*SS + k2*LL (+ old) is exact in double (|v| < 2^48), reduced once.
     const int g = g0 + gi;
     const int m = mods[g];

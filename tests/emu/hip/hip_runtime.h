@@ -108,6 +108,7 @@ emu_v16i emu_mfma_i32_32x32x32_i8(emu_v4i a, emu_v4i b, emu_v16i c, int, int, in
 #define SDPV_LDS_DMA16(gptr, ldsbase) \
     std::memcpy((unsigned char*)(ldsbase) + 16 * (threadIdx.x & 63), (const void*)(gptr), 16)
 #define SDPV_WAIT_VMCNT(n) ((void)(n))
+#define __builtin_amdgcn_s_setprio(p) ((void)(p))
 #define __builtin_amdgcn_readfirstlane(x) (x)
 
 // Dynamic shared memory of the block being run (blocks run one at a time).
