@@ -216,39 +216,43 @@ def main():
     dt_graph = (time.perf_counter() - t0) / args.steps
     del cap
 
-    # ---- per-phase times with HIP events on the launch stream (torch's current stream)
+    # ---- per-phase times with HIP events on the launch stream (torch's current stream): every
+    # buffer allocated first, then each phase launched `reps` times back to back between two
+    # events, so the average is the phase's own kernel time (the rocprofv3 trace agrees)
     st = torch.cuda.current_stream()
     nb = 1
     kp = R.kp(K)
     phases = ["row_exp", "split_A", "split_B", "product", "crt"]
-    acc_ms = {p: 0.0 for p in phases}
-    reps = max(5, args.steps // 2)
-    for rep in range(reps + 2):
-        ev = [torch.cuda.Event(enable_timing=True) for _ in range(len(phases) + 1)]
-        ev[0].record(st)
-        EA = F._row_exp(A.m, A.e, L, A.e.numel(), 1, M, K, M * K, K, 1)
-        EB = F._row_exp(B.m, B.e, L, B.e.numel(), 1, N, K, K * N, 1, N)
-        ev[1].record(st)
-        RhA = torch.empty((R.n, kp // 16, M, 16), dtype=torch.int8, device="cuda")
-        RlA = torch.empty_like(RhA)
-        R.split(A.m, A.e, EA, L, A.e.numel(), 1, M, M * K, K, 1, 0, K, kp, RhA, RlA)
-        ev[2].record(st)
-        RhB = torch.empty((R.n, kp // 16, N, 16), dtype=torch.int8, device="cuda")
-        RlB = torch.empty_like(RhB)
-        R.split(B.m, B.e, EB, L, B.e.numel(), 1, N, K * N, 1, N, 0, K, kp, RhB, RlB)
-        ev[3].record(st)
-        acc = torch.empty((R.n, M, N), dtype=torch.int16, device="cuda")
-        R.product(RhA, RlA, RhB, RlB, acc, 1, M, N, M * N, scaled=True)
-        ev[4].record(st)
-        om = torch.empty((L, 1, M, N), dtype=torch.int32, device="cuda")
-        oe = torch.empty((1, M, N), dtype=torch.int64, device="cuda")
-        R.crt(acc, M, N, 0, N, 1, outm=om, oute=oe, L=L, plane=M * N, ldo=N, jo=0, Mb=M, erow=EA, ecol=EB,
-              scaled=True)
-        ev[5].record(st)
+    RhA = torch.empty((R.n, kp // 16, M, 16), dtype=torch.int8, device="cuda")
+    RlA = torch.empty_like(RhA)
+    RhB = torch.empty((R.n, kp // 16, N, 16), dtype=torch.int8, device="cuda")
+    RlB = torch.empty_like(RhB)
+    acc = torch.empty((R.n, M, N), dtype=torch.int16, device="cuda")
+    om = torch.empty((L, 1, M, N), dtype=torch.int32, device="cuda")
+    oe = torch.empty((1, M, N), dtype=torch.int64, device="cuda")
+    EA = F._row_exp(A.m, A.e, L, A.e.numel(), 1, M, K, M * K, K, 1)
+    EB = F._row_exp(B.m, B.e, L, B.e.numel(), 1, N, K, K * N, 1, N)
+    run = {
+        "row_exp": lambda: (F._row_exp(A.m, A.e, L, A.e.numel(), 1, M, K, M * K, K, 1),
+                            F._row_exp(B.m, B.e, L, B.e.numel(), 1, N, K, K * N, 1, N)),
+        "split_A": lambda: R.split(A.m, A.e, EA, L, A.e.numel(), 1, M, M * K, K, 1, 0, K, kp, RhA, RlA),
+        "split_B": lambda: R.split(B.m, B.e, EB, L, B.e.numel(), 1, N, K * N, 1, N, 0, K, kp, RhB, RlB),
+        "product": lambda: R.product(RhA, RlA, RhB, RlB, acc, 1, M, N, M * N, scaled=True),
+        "crt": lambda: R.crt(acc, M, N, 0, N, 1, outm=om, oute=oe, L=L, plane=M * N, ldo=N, jo=0, Mb=M,
+                             erow=EA, ecol=EB, scaled=True),
+    }
+    reps = max(10, args.steps)
+    acc_ms = {}
+    for p in phases:
+        run[p]()
         torch.cuda.synchronize()
-        if rep >= 2:
-            for i, p in enumerate(phases):
-                acc_ms[p] += ev[i].elapsed_time(ev[i + 1])
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(st)
+        for _ in range(reps):
+            run[p]()
+        e1.record(st)
+        torch.cuda.synchronize()
+        acc_ms[p] = e0.elapsed_time(e1)
     ph = {p: acc_ms[p] / reps for p in phases}
 
     # ---- algorithmic work (this algorithm: 3 INT8 products per prime; padding excluded)

@@ -23,6 +23,7 @@ def asan_env():
     env = dict(os.environ)
     env["LD_PRELOAD"] = asan_runtime()
     env["ASAN_OPTIONS"] = "detect_leaks=0:verify_asan_link_order=0:abort_on_error=1"
+    env["OMP_NUM_THREADS"] = "1"
     return env
 
 

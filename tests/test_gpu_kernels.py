@@ -218,7 +218,7 @@ def test_product_accumulate():
         _eq(host(acc).astype(np.int64), ref, f"product {M}x{N}x{K}")
 
 
-def product_configs_case(dev_put, M=300, N=200, K=130, nb=2, L=10):
+def product_configs_case(dev_put, M=300, N=200, K=130, nb=2, L=10, configs=None, syrk=True):
     """every tile configuration of the product (the autotuner's choices) gives the same residues,
     equal to numpy int64; the syrk mode too on its lower tiles"""
     from sdpvibe_amd.rns import GEMM_CONFIGS, chunk_major
@@ -241,7 +241,7 @@ def product_configs_case(dev_put, M=300, N=200, K=130, nb=2, L=10):
     inv = np.array([int(v) for v in R.inv32.cpu()], np.int64)[:, None, None, None]
     yref = ref * inv % m
     yref = np.where(yref > m // 2, yref - m, yref)
-    for cfg in GEMM_CONFIGS:
+    for cfg in GEMM_CONFIGS if configs is None else configs:
         acc = dev_put(np.zeros((G, nb * M, N), np.int16))
         R.product(A[0], A[1], B[0], B[1], acc, nb, M, N, nb * M * N, config=cfg)
         _eq(acc.cpu().numpy().astype(np.int64).reshape(G, nb, M, N), ref, f"config {cfg}")
@@ -252,6 +252,8 @@ def product_configs_case(dev_put, M=300, N=200, K=130, nb=2, L=10):
         y2 = 2 * yref % m
         _eq(acc.cpu().numpy().astype(np.int64).reshape(G, nb, M, N), np.where(y2 > m // 2, y2 - m, y2),
             f"config {cfg} scaled accumulate")
+    if not syrk:
+        return
     # syrk: A = B, lower tiles (tile size by N as the reference: 128 here)
     acc = dev_put(np.zeros((G, M, M), np.int16))
     A1 = [chunk_major(dev_put(x.astype(np.int8)[:, :M])) for x in (ha, la)]
