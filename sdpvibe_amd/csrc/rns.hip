@@ -49,7 +49,7 @@ __device__ __forceinline__ v16i mfma(v4i a, v4i b, v16i c) {
 // row of the residue planes).
 // Phase 1: thread t takes one entry, with the lanes along the contiguous index (k when ldk == 1,
 //   rows otherwise) so the limb loads coalesce; the limbs are staged in LDS with independent
-//   loads, shift-rounded to the row exponent into Lout words and written as bytes into the LDS
+//   loads (in the entry's own slot row when they fit there), shift-rounded to the row exponent into Lout words and written as bytes into the LDS
 //   slot of the wave that owns the entry's row in phase 2 (wave w: rows 4w..4w+3, slot
 //   (r % 4) * 16 + k): u - 128 for every byte but the top one, which goes in as the signed top
 //   byte of the two's complement value, so the bytes carry the sign themselves.
@@ -71,8 +71,10 @@ __device__ __forceinline__ u32 pack4(int a, int b, int c, int d) {  // low bytes
     return __builtin_amdgcn_perm(cd, ab, 0x05040100u);
 }
 
-template <int KSC>   // table chunks of 32 bytes: KSC = NBp / 32 (0: any, read from NBp)
-__global__ void __launch_bounds__(256) k_rns_split(
+// table chunks of 32 bytes: KSC = NBp / 32 (0: any, read from NBp); up to three chunks the kernel
+// fits 128 VGPRs, four workgroups per CU
+template <int KSC>
+__global__ void __launch_bounds__(256, (KSC >= 1 && KSC <= 3) ? 4 : 1) k_rns_split(
     const i32* __restrict__ X, const i64* __restrict__ e, const i64* __restrict__ Erow, int L, int Lout, i64 plane,
     i64 nb, i64 rows, i64 ldb, i64 ldr, i64 ldk, i64 k0, i64 kc, i64 kp, const int8_t* __restrict__ Thi,
     const int8_t* __restrict__ Tlo, int NBp, const i32* __restrict__ C1,
@@ -84,7 +86,6 @@ __global__ void __launch_bounds__(256) k_rns_split(
     const int t = threadIdx.x, lane = t & 63;
     const int wave = __builtin_amdgcn_readfirstlane(t >> 6);
     unsigned char* xs = smem;                           // [256 slots][XS]
-    u32* raw = (u32*)(xs + (size_t)256 * XS) + (size_t)t * RS;   // [256 threads][RS] when staging
     const i64 tr = blockIdx.x / ntiles_k, tk = blockIdx.x % ntiles_k;
     const i64 row0 = tr * 16, kk0 = tk * 16;
     const i64 nrows = nb * rows;
@@ -95,6 +96,9 @@ __global__ void __launch_bounds__(256) k_rns_split(
         const int r = kmajor ? (t >> 4) : (t & 15), k = kmajor ? (t & 15) : (t >> 4);
         const int S = r * 16 + k;                       // phase-2 slot: wave S / 64, row S % 64
         u32* row = (u32*)(xs + (size_t)S * XS);
+        // staged limbs: stage 1, a region of their own ([256 threads][RS] after the slots); stage 2,
+        // the entry's own slot row, which the rounded words then overwrite in place (see below)
+        u32* raw = stage == 2 ? row : (u32*)(xs + (size_t)256 * XS) + (size_t)t * RS;
         const i64 R = row0 + r, kg = kk0 + k;
         const bool valid = R < nrows && kg < kc;
         const i64 Rc = valid ? R : 0;
@@ -134,8 +138,13 @@ __global__ void __launch_bounds__(256) k_rns_split(
         if (valid) {
             i64 s = 0;
             if (e) s = (ex == EZV) ? (i64)(1 << 30) : er - ex;
-            const int bad = stage ? shr_round_batched(LdsStrided{raw, 1}, L, s, Lout, put)
-                                  : shr_round_batched(PlaneSrc{X + off, plane}, L, s, Lout, put);
+            // In place (stage 2), output word k is written after every read of source words <= k
+            // when s >= 0 (the reads of a batch of 8 precede its writes, and later batches read
+            // words >= k + 8). A left shift (s < 0) reads below k, so it re-reads the limbs from
+            // global memory instead.
+            const bool lds_src = stage == 1 || (stage == 2 && s >= 0);
+            const int bad = lds_src ? shr_round_batched(LdsStrided{raw, 1}, L, s, Lout, put)
+                                    : shr_round_batched(PlaneSrc{X + off, plane}, L, s, Lout, put);
             if (bad && flag) atomicOr(flag, 1);
         } else {
             for (int wi = 0; wi < Lout; wi++) put(wi, 0u);  // zero entry
@@ -184,58 +193,52 @@ __global__ void __launch_bounds__(256) k_rns_split(
         const int m = cur.m;
         const double mi = cur.mi, md = (double)m;
         const int c1 = cur.c1;
-        // both 32-slot blocks against the same table chunks: four independent MFMA chains
-        v16i ah[2] = {zero16(), zero16()}, al[2] = {zero16(), zero16()};
-#pragma unroll
-        for (int u = 0; u < 4; u++) {
-            if (u < KS) {
-#pragma unroll
-                for (int eb = 0; eb < 2; eb++) {
-                    const v4i fx = *(const v4i*)(xw + (size_t)(eb * 32 + l32) * XS + 16 * hsel + u * 32);
-                    ah[eb] = mfma(fx, cur.fh[u], ah[eb]);
-                    al[eb] = mfma(fx, cur.fl[u], al[eb]);
-                }
-            }
-        }
-        for (int ks = 4; ks < KS; ks++) {                // wide operands (L > 32)
-            const v4i fh = *(const v4i*)(th + ks * 32), fl = *(const v4i*)(tl + ks * 32);
-#pragma unroll
-            for (int eb = 0; eb < 2; eb++) {
-                const v4i fx = *(const v4i*)(xw + (size_t)(eb * 32 + l32) * XS + 16 * hsel + ks * 32);
-                ah[eb] = mfma(fx, fh, ah[eb]);
-                al[eb] = mfma(fx, fl, al[eb]);
-            }
-        }
-        u32 dh[2][4], dl[2][4];                         // [eb][jq]: 4 bytes at k = 8 (jq & 1) + 4 hsel of row 2 eb + jq / 2
+        // the two 32-slot blocks one after the other, against the same table chunks (two MFMA
+        // chains each; one block's accumulators live at a time, so four waves fit a SIMD)
 #pragma unroll
         for (int eb = 0; eb < 2; eb++) {
+            v16i ah = zero16(), al = zero16();
+            const unsigned char* xe = xw + (size_t)(eb * 32 + l32) * XS + 16 * hsel;
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                if (u < KS) {
+                    const v4i fx = *(const v4i*)(xe + u * 32);
+                    ah = mfma(fx, cur.fh[u], ah);
+                    al = mfma(fx, cur.fl[u], al);
+                }
+            }
+            for (int ks = 4; ks < KS; ks++) {            // wide operands (L > 32)
+                const v4i fh = *(const v4i*)(th + ks * 32), fl = *(const v4i*)(tl + ks * 32);
+                const v4i fx = *(const v4i*)(xe + ks * 32);
+                ah = mfma(fx, fh, ah);
+                al = mfma(fx, fl, al);
+            }
+            u32 dh[4], dl[4];                           // [jq]: 4 bytes at k = 8 (jq & 1) + 4 hsel of row 2 eb + jq / 2
 #pragma unroll
             for (int jq = 0; jq < 4; jq++) {
                 int hi[4], lo[4];
 #pragma unroll
                 for (int u = 0; u < 4; u++) {
                     const int j = 4 * jq + u;
-                    const double x = (double)(ah[eb][j] * 128 + al[eb][j] + c1);
+                    const double x = (double)(ah[j] * 128 + al[j] + c1);
                     const int c = (int)fma(-rint(x * mi), md, x);        // centred residue
                     hi[u] = (c + 64) >> 7;
                     lo[u] = c - (hi[u] << 7);
                 }
-                dh[eb][jq] = pack4(hi[0], hi[1], hi[2], hi[3]);
-                dl[eb][jq] = pack4(lo[0], lo[1], lo[2], lo[3]);
+                dh[jq] = pack4(hi[0], hi[1], hi[2], hi[3]);
+                dl[jq] = pack4(lo[0], lo[1], lo[2], lo[3]);
             }
-        }
-        // swap runs across the halves: half 0 stores row 2 eb (jq 0, 1 of both halves), half 1 row
-        // 2 eb + 1 (jq 2, 3). Swapping (jq 0, jq 2) gives half 0 (own jq 0 = k 0-3, partner's = k 4-7)
-        // and half 1 (partner's jq 2 = k 0-3, own = k 4-7); (jq 1, jq 3) likewise for k 8-15.
-#pragma unroll
-        for (int eb = 0; eb < 2; eb++) {
-            swap_halves(dh[eb][0], dh[eb][2]);
-            swap_halves(dh[eb][1], dh[eb][3]);
-            swap_halves(dl[eb][0], dl[eb][2]);
-            swap_halves(dl[eb][1], dl[eb][3]);
+            // swap runs across the halves: half 0 stores row 2 eb (jq 0, 1 of both halves), half 1
+            // row 2 eb + 1 (jq 2, 3). Swapping (jq 0, jq 2) gives half 0 (own jq 0 = k 0-3, partner's
+            // = k 4-7) and half 1 (partner's jq 2 = k 0-3, own = k 4-7); (jq 1, jq 3) likewise for
+            // k 8-15.
+            swap_halves(dh[0], dh[2]);
+            swap_halves(dh[1], dh[3]);
+            swap_halves(dl[0], dl[2]);
+            swap_halves(dl[1], dl[3]);
             v4i vh, vl;
-            vh[0] = (int)dh[eb][0]; vh[1] = (int)dh[eb][2]; vh[2] = (int)dh[eb][1]; vh[3] = (int)dh[eb][3];
-            vl[0] = (int)dl[eb][0]; vl[1] = (int)dl[eb][2]; vl[2] = (int)dl[eb][1]; vl[3] = (int)dl[eb][3];
+            vh[0] = (int)dh[0]; vh[1] = (int)dh[2]; vh[2] = (int)dh[1]; vh[3] = (int)dh[3];
+            vl[0] = (int)dl[0]; vl[1] = (int)dl[2]; vl[2] = (int)dl[1]; vl[3] = (int)dl[3];
             const i64 R = row0 + 4 * wave + 2 * eb + hsel;
             if (pok && R < nrows) {
                 const i64 o = (((i64)pr * nck + tk) * nrows + R) * 16;
@@ -791,10 +794,14 @@ int sdpv_rns_split(const int32_t* X, const int64_t* e, const int64_t* Erow, int6
     const i64 ntiles_k = kp / 16, ntiles_r = (nrows + 15) / 16;
     const i64 grid = ntiles_r * ntiles_k;
     if (grid > 0x7fffffffLL) return SDPV_ESIZE;
+    // the limbs are staged in LDS: in the slot rows themselves when L words fit in one (stage 2:
+    // no extra LDS, so four workgroups fit a CU at L = 24), else after them when that fits (1)
     size_t lds = (size_t)256 * (NBp + 16);
     const size_t stage_bytes = (size_t)256 * 4 * (L + 1);
-    const int stage = lds + stage_bytes <= 160 * 1024;   // stage the limbs when they fit
-    if (stage) lds += stage_bytes;
+    int stage = 0;
+    if (NBp + 16 >= 4 * L) stage = 2;
+    else if (lds + stage_bytes <= 160 * 1024) stage = 1;
+    if (stage == 1) lds += stage_bytes;
     if (lds > 160 * 1024) return SDPV_ESIZE;
     static bool attr_set[5] = {false, false, false, false, false};
     const int ks = NBp / 32 <= 4 ? (int)(NBp / 32) : 0;
