@@ -90,6 +90,34 @@ __global__ void __launch_bounds__(256, (KSC >= 1 && KSC <= 3) ? 4 : 1) k_rns_spl
     const i64 row0 = tr * 16, kk0 = tk * 16;
     const i64 nrows = nb * rows;
 
+    const int l32 = lane & 31, hsel = lane >> 5;
+    const int KS = KSC ? KSC : NBp / 32, nPB = (G + 31) / 32;
+    const i64 nck = kp >> 4;
+    // prime-block operands: the first four 32-byte table chunks and the constants, loaded once per
+    // block of primes and shared by both 32-slot halves
+    struct PB {
+        v4i fh[4], fl[4];
+        int m, c1;
+        double mi;
+    };
+    auto load_pb = [&](int pb, PB& o) {
+        const int pr = pb * 32 + l32;
+        const int g = g0 + (pr < G ? pr : 0);
+        const int8_t* th = Thi + (i64)g * NBp + 16 * hsel;
+        const int8_t* tl = Tlo + (i64)g * NBp + 16 * hsel;
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const bool ok = u < KS;
+            o.fh[u] = ok ? *(const v4i*)(th + u * 32) : zero4();
+            o.fl[u] = ok ? *(const v4i*)(tl + u * 32) : zero4();
+        }
+        o.m = mods[g];
+        o.mi = minv[g];
+        o.c1 = C1[g];
+    };
+    PB cur, nxt;
+    load_pb(0, cur);                                    // first block's tables: in flight during phase 1
+
     // ---- phase 1: thread t -> entry (r, k) of the tile
     {
         const bool kmajor = ldk == 1;
@@ -103,21 +131,33 @@ __global__ void __launch_bounds__(256, (KSC >= 1 && KSC <= 3) ? 4 : 1) k_rns_spl
         const bool valid = R < nrows && kg < kc;
         const i64 Rc = valid ? R : 0;
         const i64 off = valid ? (Rc / rows) * ldb + (Rc % rows) * ldr + (k0 + kg) * ldk : 0;
+        const i64 ex = e ? e[off] : 0;                  // exponents first: their latency overlaps the limbs'
+        const i64 er = e ? Erow[Rc] : 0;
         if (stage) {
+            const i32* px = X + off;
             for (int w0 = 0; w0 < L; w0 += 32) {         // independent loads, up to 32 in flight
                 u32 v[32];
 #pragma unroll
                 for (int u = 0; u < 32; u++) {
-                    const int w = w0 + u < L ? w0 + u : L - 1;
-                    v[u] = (u32)X[off + (i64)w * plane];
+                    v[u] = (u32)*px;                    // words past L - 1 re-read word L - 1
+                    if (w0 + u + 1 < L) px += plane;
                 }
+                if (stage == 2) {                       // 16-byte row: L rounded up to 4 words fits
 #pragma unroll
-                for (int u = 0; u < 32; u++)
-                    if (w0 + u < L) raw[w0 + u] = v[u];
+                    for (int q = 0; q < 8; q++)
+                        if (w0 + 4 * q < L) {
+                            v4i g4;
+                            g4[0] = (int)v[4 * q]; g4[1] = (int)v[4 * q + 1];
+                            g4[2] = (int)v[4 * q + 2]; g4[3] = (int)v[4 * q + 3];
+                            *(v4i*)(raw + w0 + 4 * q) = g4;
+                        }
+                } else {
+#pragma unroll
+                    for (int u = 0; u < 32; u++)
+                        if (w0 + u < L) raw[w0 + u] = v[u];
+                }
             }
         }
-        const i64 ex = e ? e[off] : 0;
-        const i64 er = e ? Erow[Rc] : 0;
         // words leave in groups of four (16-byte LDS writes: conflict-free at this row stride)
         u32 b0 = 0x80808080u, b1 = 0x80808080u, b2 = 0x80808080u, b3 = 0x80808080u;
         auto flush4 = [&](int w4) {
@@ -156,33 +196,6 @@ __global__ void __launch_bounds__(256, (KSC >= 1 && KSC <= 3) ? 4 : 1) k_rns_spl
 
     // ---- phase 2: wave w owns slots 64w .. 64w + 63 (rows 4w .. 4w + 3 of the tile)
     const unsigned char* xw = xs + (size_t)wave * 64 * XS;
-    const int l32 = lane & 31, hsel = lane >> 5;
-    const int KS = KSC ? KSC : NBp / 32, nPB = (G + 31) / 32;
-    const i64 nck = kp >> 4;
-    // prime-block operands: the first four 32-byte table chunks and the constants, loaded once per
-    // block of primes and shared by both 32-slot halves
-    struct PB {
-        v4i fh[4], fl[4];
-        int m, c1;
-        double mi;
-    };
-    auto load_pb = [&](int pb, PB& o) {
-        const int pr = pb * 32 + l32;
-        const int g = g0 + (pr < G ? pr : 0);
-        const int8_t* th = Thi + (i64)g * NBp + 16 * hsel;
-        const int8_t* tl = Tlo + (i64)g * NBp + 16 * hsel;
-#pragma unroll
-        for (int u = 0; u < 4; u++) {
-            const bool ok = u < KS;
-            o.fh[u] = ok ? *(const v4i*)(th + u * 32) : zero4();
-            o.fl[u] = ok ? *(const v4i*)(tl + u * 32) : zero4();
-        }
-        o.m = mods[g];
-        o.mi = minv[g];
-        o.c1 = C1[g];
-    };
-    PB cur, nxt;
-    load_pb(0, cur);
     for (int pb = 0; pb < nPB; pb++) {
         if (pb + 1 < nPB) load_pb(pb + 1, nxt);        // next block's table chunks in flight
         const int pr = pb * 32 + l32;
