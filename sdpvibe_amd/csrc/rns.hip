@@ -86,7 +86,10 @@ __global__ void __launch_bounds__(256, (KSC >= 1 && KSC <= 3) ? 4 : 1) k_rns_spl
     const int t = threadIdx.x, lane = t & 63;
     const int wave = __builtin_amdgcn_readfirstlane(t >> 6);
     unsigned char* xs = smem;                           // [256 slots][XS]
-    const i64 tr = blockIdx.x / ntiles_k, tk = blockIdx.x % ntiles_k;
+    // row tiles fastest: neighbouring workgroups write neighbouring 16-byte rows of every residue
+    // chunk (one contiguous run per prime and chunk)
+    const i64 ntr = (nb * rows + 15) / 16;
+    const i64 tr = blockIdx.x % ntr, tk = blockIdx.x / ntr;
     const i64 row0 = tr * 16, kk0 = tk * 16;
     const i64 nrows = nb * rows;
 
