@@ -81,7 +81,9 @@ __global__ void __launch_bounds__(256, (KSC >= 1 && KSC <= 3) ? 4 : 1) k_rns_spl
     const i32* __restrict__ mods, const double* __restrict__ minv, int g0, int G, int8_t* __restrict__ Rh,
     int8_t* __restrict__ Rl, i32* __restrict__ flag, i64 ntiles_k, int stage) {
     SDPV_DYNAMIC_LDS(unsigned char, smem);
-    const int XS = NBp + 16;                            // bytes per slot row (multiple of 16)
+    // slot rows of XS bytes (multiple of 16); row S starts 16 * ((S / 16) % 4) bytes in, so that the
+    // 16-row stride of phase 1 for operands with contiguous rows spreads over four bank groups
+    const int XS = NBp + 64;
     const int RS = L + 1;                               // words per staged-limb row
     const int t = threadIdx.x, lane = t & 63;
     const int wave = __builtin_amdgcn_readfirstlane(t >> 6);
@@ -126,7 +128,7 @@ __global__ void __launch_bounds__(256, (KSC >= 1 && KSC <= 3) ? 4 : 1) k_rns_spl
         const bool kmajor = ldk == 1;
         const int r = kmajor ? (t >> 4) : (t & 15), k = kmajor ? (t & 15) : (t >> 4);
         const int S = r * 16 + k;                       // phase-2 slot: wave S / 64, row S % 64
-        u32* row = (u32*)(xs + (size_t)S * XS);
+        u32* row = (u32*)(xs + (size_t)S * XS + ((S >> 4) & 3) * 16);
         // staged limbs: stage 1, a region of their own ([256 threads][RS] after the slots); stage 2,
         // the entry's own slot row, which the rounded words then overwrite in place (see below)
         u32* raw = stage == 2 ? row : (u32*)(xs + (size_t)256 * XS) + (size_t)t * RS;
@@ -198,7 +200,6 @@ __global__ void __launch_bounds__(256, (KSC >= 1 && KSC <= 3) ? 4 : 1) k_rns_spl
     __syncthreads();
 
     // ---- phase 2: wave w owns slots 64w .. 64w + 63 (rows 4w .. 4w + 3 of the tile)
-    const unsigned char* xw = xs + (size_t)wave * 64 * XS;
     for (int pb = 0; pb < nPB; pb++) {
         if (pb + 1 < nPB) load_pb(pb + 1, nxt);        // next block's table chunks in flight
         const int pr = pb * 32 + l32;
@@ -214,7 +215,8 @@ __global__ void __launch_bounds__(256, (KSC >= 1 && KSC <= 3) ? 4 : 1) k_rns_spl
 #pragma unroll
         for (int eb = 0; eb < 2; eb++) {
             v16i ah = zero16(), al = zero16();
-            const unsigned char* xe = xw + (size_t)(eb * 32 + l32) * XS + 16 * hsel;
+            const int Sx = wave * 64 + eb * 32 + l32;   // slot row of this lane's fragment
+            const unsigned char* xe = xs + (size_t)Sx * XS + ((Sx >> 4) & 3) * 16 + 16 * hsel;
 #pragma unroll
             for (int u = 0; u < 4; u++) {
                 if (u < KS) {
@@ -810,9 +812,10 @@ int sdpv_rns_split(const int32_t* X, const int64_t* e, const int64_t* Erow, int6
     const i64 ntiles_k = kp / 16, ntiles_r = (nrows + 15) / 16;
     const i64 grid = ntiles_r * ntiles_k;
     if (grid > 0x7fffffffLL) return SDPV_ESIZE;
-    // the limbs are staged in LDS: in the slot rows themselves when L words fit in one (stage 2:
-    // no extra LDS, so four workgroups fit a CU at L = 24), else after them when that fits (1)
-    size_t lds = (size_t)256 * (NBp + 16);
+    // the limbs are staged in LDS: in the slot rows themselves when L words fit in one after its
+    // offset of up to 48 bytes (stage 2: no extra LDS, 40 KB per workgroup at L = 24, so four
+    // workgroups fit a CU), else after them when that fits (1)
+    size_t lds = (size_t)256 * (NBp + 64);
     const size_t stage_bytes = (size_t)256 * 4 * (L + 1);
     int stage = 0;
     if (NBp + 16 >= 4 * L) stage = 2;
