@@ -332,6 +332,49 @@ __global__ void __launch_bounds__(1024) k_row_exp_cols(const i32* __restrict__ X
     }
 }
 
+// strided rows, few of them: 16 rows per block (lane = 16 rows x 4 k), its 16 waves over k, so that
+// few rows still give enough blocks without splitting k (no fill, no atomics); the 64 k partial
+// maxima of a row meet by lane exchange within the wave and in LDS across the waves
+__global__ void __launch_bounds__(1024) k_row_exp_cols16(const i32* __restrict__ X, const i64* __restrict__ e, int L,
+                                                         i64 plane, i64 nrows, i64 rows, i64 K, i64 ldb, i64 ldr,
+                                                         i64 ldk, i64* __restrict__ Erow) {
+    __shared__ i64 part[16][16];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int rl = lane & 15, kq = lane >> 4;
+    const i64 R = blockIdx.x * (i64)16 + rl;
+    i64 mx = EZV;
+    if (R < nrows) {
+        const i64 b = R / rows, r = R % rows;
+        const i64 base = b * ldb + r * ldr;
+        const i32* top = X + (i64)(L - 1) * plane;
+        for (i64 k0 = wave * 4 + kq; k0 < K; k0 += 8 * 64) {
+            i64 ex[8];
+            u32 tw[8];
+#pragma unroll
+            for (int u = 0; u < 8; u++) {
+                const i64 k = k0 + u * 64;
+                ex[u] = k < K ? e[base + k * ldk] : EZV;
+                tw[u] = k < K ? (u32)top[base + k * ldk] : 0u;
+            }
+#pragma unroll
+            for (int u = 0; u < 8; u++) {
+                const i64 t = entry_top_fast(X, plane, L, base + (k0 + u * 64) * ldk, ex[u], tw[u]);
+                mx = t > mx ? t : mx;
+            }
+        }
+    }
+    for (int o = 16; o <= 32; o <<= 1) {                  // the 4 k lanes of a row within the wave
+        const i64 v = __shfl_xor(mx, o, 64);
+        mx = v > mx ? v : mx;
+    }
+    if (kq == 0) part[wave][rl] = mx;
+    __syncthreads();
+    if (wave == 0 && kq == 0 && R < nrows) {
+        for (int q = 1; q < 16; q++) mx = part[q][rl] > mx ? part[q][rl] : mx;
+        Erow[R] = row_exp_of(mx, L);
+    }
+}
+
 __global__ void __launch_bounds__(256) k_fill_i64(i64* __restrict__ p, i64 n, i64 v) {
     const i64 i = blockIdx.x * (i64)256 + threadIdx.x;
     if (i < n) p[i] = v;
@@ -450,8 +493,15 @@ int sdpv_row_exp(const int32_t* X, const int64_t* e, int64_t L, int64_t plane, i
         hipLaunchKernelGGL(k_row_exp_wave, dim3(nblocks(nrows, 4)), dim3(256), 0, st, X, e, (int)L, (i64)plane, nrows,
                            (i64)rows, (i64)K, (i64)ldb, (i64)ldr, Erow);
     } else {
-        // few row groups: split k as well, so that the launch fills the device
+        // few row groups of 64: 16-row blocks when they are enough (32 or more, or a short k),
+        // else split k as well, so that the launch fills the device
         const i64 groups = (nrows + 63) / 64;
+        if (groups < 128 && K > 256 && ((nrows + 15) / 16 >= 32 || K <= 4096)) {
+            hipLaunchKernelGGL(k_row_exp_cols16, dim3(nblocks(nrows, 16)), dim3(1024), 0, st, X, e, (int)L,
+                               (i64)plane, nrows, (i64)rows, (i64)K, (i64)ldb, (i64)ldr, (i64)ldk, Erow);
+            SDPV_TRY_LAUNCH();
+            return SDPV_OK;
+        }
         i64 parts = 1;
         if (groups < 128 && K > 256) {
             parts = (256 + groups - 1) / groups;

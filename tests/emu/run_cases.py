@@ -45,8 +45,10 @@ CASES = {
     "gemm_batched": lambda: gemm_case(3, 2, 33, 20, 40, False, True, 4),
     # the CRT's register-held words: 5, 6 and 8 digit blocks (7: the L = 24 cases)
     "gemm_regwords": lambda: all(gemm_case(L, 1, 9, 12, 10, False, True, L) for L in (16, 20, 28)),
-    # column row exponents of B over k parts (few rows, long k)
+    # column row exponents of B: 16-row blocks (few rows, k <= 4096), and k split over blocks with
+    # a filled Erow and an atomic maximum (few rows, k > 4096)
     "gemm_ksplit": lambda: gemm_case(2, 1, 3, 600, 5, False, False, 9),
+    "gemm_ksplit_long": lambda: gemm_case(1, 1, 2, 5000, 3, False, False, 13),
 }
 
 def gpu_test_gemm_cases():

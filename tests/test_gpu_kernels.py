@@ -347,7 +347,7 @@ def _gemm_case(L, nb, M, K, N, transA, transB, seed):
 @pytest.mark.parametrize("L,nb,M,K,N", [(1, 1, 3, 4, 5), (2, 1, 33, 70, 31), (10, 2, 40, 33, 70),
                                          (16, 2, 20, 33, 40), (20, 1, 65, 40, 33), (24, 1, 130, 200, 129),
                                          (26, 3, 17, 64, 65), (28, 1, 33, 50, 64), (34, 1, 64, 65, 64),
-                                         (2, 1, 5, 700, 9)])
+                                         (2, 1, 5, 700, 9), (1, 1, 3, 5000, 4)])
 @pytest.mark.parametrize("transA,transB", [(False, False), (True, False), (False, True), (True, True)])
 def test_mpf_gemm(L, nb, M, K, N, transA, transB):
     Am, Ae, Bm, Be = _gemm_case(L, nb, M, K, N, transA, transB, L * 7919 + M + 2 * transA + transB)
