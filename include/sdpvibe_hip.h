@@ -143,7 +143,8 @@ int sdpv_rns_split(const int32_t* X, const int64_t* e, const int64_t* Erow, int6
  * INT8 products per prime (the sum plane h + l formed in registers); 6 and 7 are the 128x128 and
  * 128x256 tiles with four products (HH, LL, HL + LH; no bytewise adds); 8 and 9 the 128x256 tile
  * with 3 stages, four and three products; 10 and 11 the 128x64 and 64x128 tiles with four
- * products (fewer registers, more workgroups per CU). Every configuration computes the same
+ * products (fewer registers, more workgroups per CU), 12 the 64x128 tile in 3 stages of 32 bytes of
+ * k (36 KB of LDS, four workgroups per CU). Every configuration computes the same
  * residues (an autotuner picks by time). */
 int sdpv_rns_gemm(const int8_t* Ah, const int8_t* Al, const int8_t* Bh, const int8_t* Bl,
                   int64_t G, int64_t nb, int64_t M, int64_t N, int64_t kp, int16_t* acc,

@@ -271,22 +271,23 @@ __global__ void __launch_bounds__(256, (KSC >= 1 && KSC <= 3) ? 4 : 1) k_rns_spl
 // ================================================================== modular products
 // Residues are stored chunk-major: byte k of row r of prime g at ((g*(kp/16) + k/16)*rows + r)*16
 // + k%16, so 64 consecutive rows of one 16-byte k-chunk are 1 KB of contiguous memory.
-// WM x WN waves, each BLM x BLN blocks of 32x32; BK = 64 bytes (4 chunks) per stage. Stages are
+// WM x WN waves, each BLM x BLN blocks of 32x32; BKB = 64 bytes (4 chunks) per stage, or 32 (2 chunks:
+// smaller stages, more workgroups per CU). Stages are
 // copied global -> LDS by direct loads (no registers), NST stages in flight, one barrier per stage.
 // LDS stage layout: [A h | A l | B h | B l], each plane [chunk][tile row][16 B]: fragment reads of
 // 32 consecutive rows are contiguous and conflict-free. The s = h + l operand is formed in
 // registers (bytewise add) from the h and l fragments.
-template <int WM, int WN, int BLM, int BLN, int NST, int P4>
-__global__ void __launch_bounds__(WM* WN * 64, (WM * WN <= 4) ? 2 : 1) k_rns_gemm(
+template <int WM, int WN, int BLM, int BLN, int NST, int P4, int BKB = 64>
+__global__ void __launch_bounds__(WM* WN * 64, BKB == 32 ? 4 : ((WM * WN <= 4) ? 2 : 1)) k_rns_gemm(
     const int8_t* __restrict__ Ah, const int8_t* __restrict__ Al, const int8_t* __restrict__ Bh,
     const int8_t* __restrict__ Bl, int G, int nb, i64 M, i64 N, i64 kp, int16_t* __restrict__ acc, i64 ld_g,
     const i32* __restrict__ mods, const double* __restrict__ minv, const i32* __restrict__ kcoef, int g0, int mode,
     i64 tiles_m, i64 tiles_n) {
     constexpr int NWAVE = WM * WN;
     constexpr int BM = WM * BLM * 32, BN = WN * BLN * 32;
-    constexpr int PA = 64 * BM, PB = 64 * BN;       // bytes of one plane of one stage
+    constexpr int PA = BKB * BM, PB = BKB * BN;     // bytes of one plane of one stage
     constexpr int SBYTES = 2 * PA + 2 * PB;
-    constexpr int QA = BM / 16, QB = BN / 16;       // 1 KB direct loads per plane per stage
+    constexpr int QA = BM * BKB / 1024, QB = BN * BKB / 1024;   // 1 KB direct loads per plane per stage
     constexpr int LPW = (2 * QA + 2 * QB) / NWAVE;  // direct loads per wave per stage
     static_assert((2 * QA + 2 * QB) % NWAVE == 0, "load split");
     SDPV_DYNAMIC_LDS(int8_t, smem8);
@@ -331,7 +332,7 @@ __global__ void __launch_bounds__(WM* WN * 64, (WM * WN <= 4) ? 2 : 1) k_rns_gem
     }
     auto issue = [&](int s) {
         int8_t* sbase = smem8 + (s % NST) * SBYTES;
-        const i64 kc0 = (i64)s * 4;
+        const i64 kc0 = (i64)s * (BKB / 16);
 #pragma unroll
         for (int u = 0; u < LPW; u++) {
             const int q = wave * LPW + u;                // uniform per wave
@@ -362,7 +363,7 @@ __global__ void __launch_bounds__(WM* WN * 64, (WM * WN <= 4) ? 2 : 1) k_rns_gem
         const int8_t* sA = smem8 + (s % NST) * SBYTES;
         const int8_t* sB = sA + 2 * PA;
 #pragma unroll
-        for (int ks = 0; ks < 2; ks++) {
+        for (int ks = 0; ks < BKB / 32; ks++) {
             const int c = 2 * ks + hsel;
             v4i ah[BLM], al[BLM], as[BLM], bh[BLN], bl[BLN], bs[BLN];  // as/bs unused when P4
 #pragma unroll
@@ -409,7 +410,7 @@ __global__ void __launch_bounds__(WM* WN * 64, (WM * WN <= 4) ? 2 : 1) k_rns_gem
         }
     };
 
-    const int nst = (int)(kp / 64);
+    const int nst = (int)(kp / BKB);
 #pragma unroll
     for (int s = 0; s < NST - 1; s++)
         if (s < nst) issue(s);
@@ -460,7 +461,7 @@ __global__ void __launch_bounds__(WM* WN * 64, (WM * WN <= 4) ? 2 : 1) k_rns_gem
         }
 }
 
-template <int WM, int WN, int BLM, int BLN, int NST, int P4>
+template <int WM, int WN, int BLM, int BLN, int NST, int P4, int BKB = 64>
 int launch_gemm(const int8_t* Ah, const int8_t* Al, const int8_t* Bh, const int8_t* Bl, int G, int nb, i64 M, i64 N,
                 i64 kp, int16_t* acc, i64 ld_g, const i32* mods, const double* minv, const i32* kcoef, int g0,
                 int mode, hipStream_t st) {
@@ -469,14 +470,14 @@ int launch_gemm(const int8_t* Ah, const int8_t* Al, const int8_t* Bh, const int8
     const i64 total = tiles_m * tiles_n * (i64)G * nb;
     if (total == 0) return SDPV_OK;
     if (total > 0x7fffffffLL) return SDPV_ESIZE;
-    const size_t lds = (size_t)NST * 128 * (BM + BN);
+    const size_t lds = (size_t)NST * 2 * BKB * (BM + BN);
     static bool attr_set = false;
     if (!attr_set) {
-        (void)hipFuncSetAttribute((const void*)k_rns_gemm<WM, WN, BLM, BLN, NST, P4>,
+        (void)hipFuncSetAttribute((const void*)k_rns_gemm<WM, WN, BLM, BLN, NST, P4, BKB>,
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         attr_set = true;
     }
-    hipLaunchKernelGGL((k_rns_gemm<WM, WN, BLM, BLN, NST, P4>), dim3((unsigned)total), dim3(WM * WN * 64), lds, st, Ah,
+    hipLaunchKernelGGL((k_rns_gemm<WM, WN, BLM, BLN, NST, P4, BKB>), dim3((unsigned)total), dim3(WM * WN * 64), lds, st, Ah,
                        Al, Bh, Bl, G, nb, M, N, kp, acc, ld_g, mods, minv, kcoef, g0, mode, tiles_m, tiles_n);
     SDPV_TRY_LAUNCH();
     return SDPV_OK;
@@ -852,7 +853,7 @@ int sdpv_rns_gemm(const int8_t* Ah, const int8_t* Al, const int8_t* Bh, const in
                   int64_t M, int64_t N, int64_t kp, int16_t* acc, int64_t ld_g, const int32_t* mods,
                   const double* minv, const int32_t* kcoef, int64_t g0, int32_t mode, int32_t config, void* stream) {
     if (G < 0 || nb < 0 || M < 0 || N < 0 || kp < 0 || g0 < 0 || (mode & ~7)) return SDPV_EARG;
-    if (config < -1 || config > 11) return SDPV_EARG;
+    if (config < -1 || config > 12) return SDPV_EARG;
     if (kp % 64 != 0 || kp > 65536) return SDPV_ESIZE;
     if (nb * M > (1LL << 26) || nb * N > (1LL << 26)) return SDPV_ESIZE;   // 32-bit offsets in a stage
     if ((mode & SDPV_GEMM_SYRK_LOWER) && (M != N || config != -1)) return SDPV_EARG;
@@ -877,7 +878,8 @@ int sdpv_rns_gemm(const int8_t* Ah, const int8_t* Al, const int8_t* Bh, const in
         case 8: return launch_gemm<2, 4, 2, 2, 3, 1>(Ah, Al, Bh, Bl, g, b, M, N, kp, acc, ld_g, mods, minv, kcoef, q0, mode, st);
         case 9: return launch_gemm<2, 4, 2, 2, 3, 0>(Ah, Al, Bh, Bl, g, b, M, N, kp, acc, ld_g, mods, minv, kcoef, q0, mode, st);
         case 10: return launch_gemm<2, 2, 2, 1, 2, 1>(Ah, Al, Bh, Bl, g, b, M, N, kp, acc, ld_g, mods, minv, kcoef, q0, mode, st);
-        default: return launch_gemm<2, 2, 1, 2, 2, 1>(Ah, Al, Bh, Bl, g, b, M, N, kp, acc, ld_g, mods, minv, kcoef, q0, mode, st);
+        case 11: return launch_gemm<2, 2, 1, 2, 2, 1>(Ah, Al, Bh, Bl, g, b, M, N, kp, acc, ld_g, mods, minv, kcoef, q0, mode, st);
+        default: return launch_gemm<2, 2, 1, 2, 3, 1, 32>(Ah, Al, Bh, Bl, g, b, M, N, kp, acc, ld_g, mods, minv, kcoef, q0, mode, st);
     }
 }
 

@@ -247,7 +247,7 @@ def product_cases(configs, syrk):
 
 # the twelve tile configurations in two halves, so that the halves can run side by side
 CASES["product_a"] = lambda: product_cases(range(0, 6), True)
-CASES["product_b"] = lambda: product_cases(range(6, 12), False)
+CASES["product_b"] = lambda: product_cases(range(6, 13), False)
 
 
 def carry_cases():
