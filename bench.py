@@ -29,6 +29,24 @@ PEAK_HBM_GBS = 8000.0         # MI355X HBM3E, GB/s
 METRIC = "s/IPM iteration @768-bit N=3000 SDP (batched vs plain); mp-GEMM % INT8 MFMA peak"
 
 
+def _host_threads():
+    """host threads this process may use: its CPU affinity set (the machine's CPU share for this
+    GPU), capped by OMP_NUM_THREADS when that is set"""
+    n = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    omp = os.environ.get("OMP_NUM_THREADS", "")
+    return max(1, min(n, int(omp))) if omp.isdigit() and int(omp) > 0 else n
+
+
+def _cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -37,7 +55,7 @@ def parse():
     ap.add_argument("--n", type=int, default=512)
     ap.add_argument("--L", type=int, default=24)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
+    ap.add_argument("--cpu-threads", type=int, default=_host_threads())
     ap.add_argument("--no-batched", action="store_true", help="skip the uneven-block Cholesky comparison")
     ap.add_argument("--blocks", type=int, default=400)
     ap.add_argument("--no-gram", action="store_true", help="skip the Gram accumulation measurement")
@@ -83,30 +101,15 @@ def gram_acc(L, N=3000, rows=1024, chunks=4):
 
 
 def batched_chol(L, nblocks, seed=0, rhs=4):
-    """Uneven-block Cholesky + solve at L limbs: nblocks SPD blocks A = D (B B^T + n I) D, n uniform
-    in 8..96, B Gaussian MPF, D = diag(10^u) with u uniform in [-20, 20], `rhs` Gaussian right-hand
-    sides per block. Times the plain per-block form (one Chol and one solve per block) against the
-    bucketed form (one batched factorisation and solve per size class); results are bitwise equal."""
-    import numpy as np
+    """Uneven-block Cholesky + solve at L limbs (synthetic.uneven_spd_blocks: n uniform in 8..96,
+    A = D (B B^T + n I) D, D = diag(10^u), u uniform in [-20, 20], `rhs` right-hand sides per block).
+    Times the plain per-block form (one Chol and one solve per block) against the bucketed form
+    (one batched factorisation and solve per size class); results are bitwise equal."""
     import torch
     from sdpvibe_amd import mpf as F
-    from sdpvibe_amd.synthetic import gaussian_mpf
+    from sdpvibe_amd import synthetic as SY
 
-    rng = np.random.default_rng(seed)
-    dims = [int(x) for x in rng.integers(8, 97, nblocks)]
-    put = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()
-    blocks, rhs_l = [], []
-    p10 = F.from_strings([f"1e{k}" for k in range(-40, 41)], L)      # 10^k rounded to NB bits
-    for i, n in enumerate(dims):
-        bm, be = gaussian_mpf((n, n), L, seed=10_000 + i)
-        B = F.MPF(put(bm), put(be))
-        G = F.add(F.gemm(B, B, transB=True), F.eye(n, L, val=n))
-        u = rng.integers(-20, 21, n)
-        idx = put((u[:, None] + u[None, :] + 40).reshape(-1))
-        D = F.MPF(p10.m[:, idx].reshape(L, n, n).contiguous(), p10.e[idx].reshape(n, n).contiguous())
-        blocks.append(F.mul(G, D))
-        rm, re_ = gaussian_mpf((n, rhs), L, seed=20_000 + i)
-        rhs_l.append(F.MPF(put(rm), put(re_)))
+    dims, blocks, rhs_l = SY.uneven_spd_blocks(nblocks, L, seed=seed, rhs=rhs)
     torch.cuda.synchronize()
 
     def plain():
@@ -154,24 +157,60 @@ def measured_traffic(kernel, config, n, L):
 
 
 def cpu_baseline(n, threads, L):
-    """The reference's CPU path on the same shape: mpf_baseline <prec> <n> <nrhs> <threads>
-    (GMP mpf at 32L bits; its GEMM line is an n x n x n product)."""
+    """The reference's CPU path on the same shape: mpf_baseline <prec> <n> <nrhs> <threads> (GMP mpf at
+    32L bits): Crout Cholesky of an n x n SPD matrix, TRSM with n right-hand sides and an n x n x n
+    GEMM, each timed by the program itself. `value` is its GEMM rate; the Cholesky and TRSM rates
+    of the same run are reported beside it."""
     exe = os.path.join(ROOT, "oracle", "_ref", "mpf_baseline")
     if not os.path.exists(exe):
         return None
     prec = 32 * L
     t0 = time.time()
     r = subprocess.run([exe, str(prec), str(n), str(n), str(threads)], capture_output=True, text=True,
-                       timeout=600)
+                       timeout=900, env=dict(os.environ, OMP_NUM_THREADS=str(threads)))
     wall = time.time() - t0
-    m = re.search(r"gemm\s+([\d.]+) s\s+([\d.]+) M mp-MAC/s", r.stdout)
-    if r.returncode != 0 or not m:
+    rates = {k: re.search(k + r"\s+([\d.]+) s\s+([\d.]+) M mp-MAC/s", r.stdout) for k in ("cholesky", "trsm", "gemm")}
+    if r.returncode != 0 or not all(rates.values()):
         return None
-    return {"value": float(m.group(2)) * 1e6, "unit": "mp-MAC/s (768-bit GEMM)", "cores": threads,
-            "kind": "reference",
-            "sample": f"cpu_baseline/mpf_baseline.c {prec} {n} {n} {threads}: GMP mpf naive GEMM "
-                      f"{n}x{n}x{n} ({float(m.group(1)):.2f} s; Cholesky+TRSM of the same run not counted), "
+    sec = {k: float(m.group(1)) for k, m in rates.items()}
+    rate = {k: float(m.group(2)) * 1e6 for k, m in rates.items()}
+    return {"value": rate["gemm"], "unit": "mp-MAC/s (768-bit GEMM)", "cores": threads, "kind": "reference",
+            "nproc": os.cpu_count(), "cpu_model": _cpu_model(),
+            "cholesky_mp_mac_per_s": rate["cholesky"], "trsm_mp_mac_per_s": rate["trsm"],
+            "seconds": sec,
+            "sample": f"cpu_baseline/mpf_baseline.c {prec} {n} {n} {threads} (GMP mpf naive loops, OpenMP "
+                      f"{threads} threads = this process's CPU share): Cholesky {n}x{n} {sec['cholesky']:.2f} s, "
+                      f"TRSM {n} RHS {sec['trsm']:.2f} s, GEMM {n}x{n}x{n} {sec['gemm']:.2f} s; "
                       f"{wall:.1f} s wall in total"}
+
+
+def plain_chol(n, L, reps=3):
+    """GPU Cholesky of the plain-kernel configuration's SPD matrix A = B B^T + n I and its two solves
+    with n right-hand sides (L^-1 R, then L^-T of that), averaged over `reps` runs (ms), with the
+    rates in mp-MAC/s counted as the CPU baseline counts them (n^3/6 for the Cholesky, n^3/2 per
+    triangular solve of n right-hand sides)."""
+    import torch
+    from sdpvibe_amd import mpf as F
+    from sdpvibe_amd import synthetic as SY
+    _, A = SY.plain_config(n, L, seed=0)
+    Rm, Re = SY.gaussian_mpf((n, n), L, seed=1)
+    R = F.MPF(torch.from_numpy(Rm).cuda(), torch.from_numpy(Re).cuda())
+    F.Chol(A).solve(R)                                    # warm (autotuner)
+    torch.cuda.synchronize()
+    tc = ts = 0.0
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        C = F.Chol(A)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        C.solve_upper(C.solve_lower(R))
+        torch.cuda.synchronize()
+        tc += t1 - t0
+        ts += time.perf_counter() - t1
+    tc, ts = tc / reps, ts / reps
+    return {"workload": f"Chol of A = B B^T + {n} I ({n}x{n}, L={L}) and L^-1, L^-T solves with {n} RHS",
+            "chol_ms": round(tc * 1e3, 2), "solves_ms": round(ts * 1e3, 2),
+            "chol_mp_mac_per_s": n ** 3 / 6 / tc, "trsm_mp_mac_per_s": n ** 3 / ts}
 
 
 def main():
@@ -264,6 +303,10 @@ def main():
     ref_ops = 2 * (4 * 112 * M * N * K + 2 * 112 * 4 * L * (M + N) * K + 2 * 216 * 112 * M * N)
     value = 100.0 * ref_ops / dt / (PEAK_INT8_TOPS * 1e12)
 
+    # algorithmic HBM bytes per launch of the bandwidth-bound phases (limbs + exponents in, residue
+    # bytes out for a split; residue coefficients in, limbs + exponent out for the reconstruction)
+    hbm = {"split_A": (4 * L + 8 + 2 * nP) * M * K, "split_B": (4 * L + 8 + 2 * nP) * N * K,
+           "crt": (2 * nP + 4 * L + 8) * M * N}
     dom = max(phases, key=lambda p: ph[p])
     t_dom = ph[dom] * 1e-3
     if dom == "product":
@@ -292,6 +335,11 @@ def main():
             gram = gram_acc(L)
         except Exception as exc:                          # noqa: BLE001
             gram = {"error": f"{type(exc).__name__}: {exc}"}
+    pchol = None
+    try:                                                  # auxiliary measurement: never loses the line
+        pchol = plain_chol(n, L)
+    except Exception as exc:                              # noqa: BLE001
+        pchol = {"error": f"{type(exc).__name__}: {exc}"}
     bchol = None
     if not args.no_batched:
         try:                                              # auxiliary measurement: never loses the line
@@ -299,7 +347,8 @@ def main():
         except Exception as exc:                          # noqa: BLE001
             bchol = {"error": f"{type(exc).__name__}: {exc}"}
     line = {
-        "metric": METRIC, "value": round(value, 3), "unit": "% of dense INT8 MFMA peak (mp-GEMM)",
+        "metric": METRIC, "value": round(value, 3),
+        "unit": f"% of dense INT8 MFMA peak (mp-GEMM {M}x{K}x{N} at L={L}; s/IPM iteration not measured here)",
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": dt * 1e3,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "int8 (exact 768-bit)",
         "data": "synthetic (host PCG64 Gaussian mantissas, exponents in [-16, 16], seed 0/1)",
@@ -311,7 +360,8 @@ def main():
         "executed_int8_ops_per_step": ops_total,
         "pct_peak_executed_ops": round(100.0 * ops_total / dt / (PEAK_INT8_TOPS * 1e12), 3),
         "phase_ms": {p: round(v, 4) for p, v in ph.items()},
-        "roofline": roof, "cpu_baseline": cpu, "batched_chol": bchol, "gram_acc": gram,
+        "phase_hbm_frac": {p: round(hbm[p] / (ph[p] * 1e-3) / 1e9 / PEAK_HBM_GBS, 4) for p in hbm},
+        "roofline": roof, "cpu_baseline": cpu, "plain_chol": pchol, "batched_chol": bchol, "gram_acc": gram,
     }
     print(json.dumps(line))
 

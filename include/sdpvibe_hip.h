@@ -216,12 +216,16 @@ int sdpv_mpf_to_double(const int32_t* m, const int64_t* e, int64_t L, int64_t n,
 int sdpv_mpf_from_double(const double* v, int64_t n, int64_t L, int32_t* m, int64_t* e, void* stream);
 
 /* Eigenvalues (ascending, w (nb, n)) and optionally eigenvectors (columns of V (nb, n, n)) of the
- * symmetrised (A + A^T)/2 of nb float64 n x n matrices (n <= 255), and their Frobenius norms
- * (fro (nb,), may be null). work: 2 * nb * m * m doubles, m = n rounded up to even.
+ * symmetrised (A + A^T)/2 of nb float64 n x n matrices (n <= SDPV_SYMEIG_NMAX), their Frobenius
+ * norms (fro (nb,), may be null) and the Frobenius norm of what is left off the diagonal when the
+ * iteration stops (off (nb,), may be null): each returned eigenvalue is within off of an exact
+ * eigenvalue of the float64 matrix (Weyl). work: 2 * nb * m * m doubles, m = n rounded up to even.
  * Parallel cyclic Jacobi. Replaces numpy.linalg.eigvalsh / eigh in min_eig and refine_min_eig
  * (sdpvibe/solver.py:1039, :1077). */
+#define SDPV_SYMEIG_NMAX 4096
 int sdpv_symeig(const double* A, int64_t nb, int64_t n, double* work, double* w, double* V, double* fro,
-                void* stream);
+                double* off, void* stream);
+
 
 /* ---------------------------------------------------------------- diagnostics */
 

@@ -35,7 +35,7 @@ SIGNATURES = {
     "sdpv_tile_trsv": ([_P, _P, _P, _I64, _I64, _I64, _I64, _P], _I32),
     "sdpv_mpf_to_double": ([_P, _P, _I64, _I64, _P, _I64, _P, _P], _I32),
     "sdpv_mpf_from_double": ([_P, _I64, _I64, _P, _P, _P], _I32),
-    "sdpv_symeig": ([_P, _I64, _I64, _P, _P, _P, _P, _P], _I32),
+    "sdpv_symeig": ([_P, _I64, _I64, _P, _P, _P, _P, _P, _P], _I32),
     "sdpv_mfma_selftest": ([_P, _P, _P, _P], _I32),
 }
 ERRORS = {-1: "bad argument", -2: "unsupported size", -3: "kernel launch failed"}

@@ -66,40 +66,43 @@ def chol_inv_tile(A, LI=LI, threads=256):
     return Lout, X
 
 
-def apply_tile_inverse(t, B, refine=0):
-    """P = L_tile^-1 B through the explicit tile inverse (+ refinement steps); B: MP (b x m) with
-    per-column exponents (basecase.py:298-310)"""
-    from .mpmat import add, gemm_nt
+def _rows_of(M):
+    """M^T as the row operand of gemm_nt (its column exponents are passed as row exponents)"""
+    return MP(M.limbs.transpose(1, 2).contiguous(), 0)
 
-    def tr(M):
-        return MP(M.limbs.transpose(1, 2).contiguous(), 0)
-    P = gemm_nt(t.X, tr(B), out_exp="col", bt_rowexp=B.exp)
+
+def apply_tile_inverse(t, B, refine=0):
+    """L_tile^-1 B through the tile's explicit inverse, then `refine` residual corrections
+    (B - L P, multiplied by the inverse, added back aligned); B: MP (b x m) with per-column
+    exponents, and so is the result"""
+    from .mpmat import add, gemm_nt
+    P = gemm_nt(t.X, _rows_of(B), out_exp="col", bt_rowexp=B.exp)
     for _ in range(refine):
-        LP = gemm_nt(t.Lmp, tr(P), out_exp="col", bt_rowexp=P.exp)
-        Rr = add(B, LP, -1, out_exp="col")
-        D = gemm_nt(t.X, tr(Rr), out_exp="col", bt_rowexp=Rr.exp)
-        P = add(P, D, 1, out_exp="align")
+        resid = add(B, gemm_nt(t.Lmp, _rows_of(P), out_exp="col", bt_rowexp=P.exp), -1, out_exp="col")
+        P = add(P, gemm_nt(t.X, _rows_of(resid), out_exp="col", bt_rowexp=resid.exp), 1, out_exp="align")
     return P
 
 
 def trsv_tile(t, Vlimbs, vexp, out_exp=None):
-    """Solve L y = v for many vectors with a factored tile (basecase.py:313-332).
-    Vlimbs: (L, nvec, b) limb planes, vexp: (nvec,) exponents (each vector is fixed point at
-    32L-40 fractional bits of its own scale). out_exp None: dynamic per vector; int: fixed.
+    """y = L^-1 v for many vectors with a factored tile (forward substitution on the GPU).
+    Vlimbs: (L, nvec, b) limb planes, vexp: (nvec,) exponents, each vector fixed point with 32L - 40
+    fractional bits of its own scale. The vectors are widened exactly to the tile's wide format
+    (40 more fractional bits, entry-major words), solved, and rounded back: to headroom_bits(L)
+    below each vector's largest entry (out_exp None) or to the int exponent out_exp.
     Returns (limbs (L, nvec, b), exps (nvec,))."""
     L, nvec, b = Vlimbs.shape
-    F, FX = 32 * t.LF, 32 * L - 40
+    widen = 32 * t.LF - (32 * L - 40)                          # extra fractional bits of the wide format
     WL = t.WL
     n = nvec * b
-    # exact widening by F - FX = 40 bits, then entry-major (nvec, b, WL) words
-    Ws = shift_round(Vlimbs.reshape(L, n), torch.full((n,), FX - F, dtype=torch.int64, device=Vlimbs.device), WL)
-    W = Ws.view(WL, nvec, b).permute(1, 2, 0).contiguous()
-    call("sdpv_tile_trsv", ptr(t.Lw), ptr(t.RINV), ptr(W), nvec, b, WL, t.LF, stream())
-    Ws = W.permute(2, 0, 1).contiguous().view(WL, n)
-    base = vexp + FX - F
+    dv = Vlimbs.device
+    wide = shift_round(Vlimbs.reshape(L, n), torch.full((n,), -widen, dtype=torch.int64, device=dv), WL)
+    entry_major = wide.view(WL, nvec, b).permute(1, 2, 0).contiguous()
+    call("sdpv_tile_trsv", ptr(t.Lw), ptr(t.RINV), ptr(entry_major), nvec, b, WL, t.LF, stream())
+    planes = entry_major.permute(2, 0, 1).reshape(WL, n)
+    wide_exp = vexp - widen                                    # exponent of the wide words, per vector
     if out_exp is None:
-        ne = bitlen_words(Ws).view(nvec, b).amax(1) + base - headroom_bits(L)
+        target = bitlen_words(planes.contiguous()).view(nvec, b).amax(1) + wide_exp - headroom_bits(L)
     else:
-        ne = torch.full((nvec,), int(out_exp), dtype=torch.int64, device=Vlimbs.device)
-    s = (ne - base)[:, None].expand(nvec, b).reshape(-1).contiguous()
-    return shift_round(Ws, s, L).view(L, nvec, b), ne
+        target = torch.full((nvec,), int(out_exp), dtype=torch.int64, device=dv)
+    shift = (target - wide_exp).repeat_interleave(b)
+    return shift_round(planes.contiguous(), shift, L).view(L, nvec, b), target

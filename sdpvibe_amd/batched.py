@@ -1,8 +1,13 @@
-"""Batched multiprecision linear algebra: many independent equal-sized blocks per launch (mirror of
-sdpvibe/batched.py).
+"""Batched multiprecision linear algebra: nb independent blocks of one shape per launch (the
+reference's sdpvibe/batched.py interface).
 
-A BMP holds a stack of nb matrices:
-    limbs: int32 (L, nb, R, C) limb planes;  exp: Python int (uniform) or (nb, C) int64 tensor.
+A BMP is a stack of nb fixed-point matrices: limbs (L, nb, R, C) and an exponent that is either one
+Python int for the whole stack or one per (block, column), an (nb, C) int64 tensor.
+
+Recursive Cholesky and triangular solves walk a split tree of the dimension (halved at multiples of
+the tile size bt, leaves are bt x bt diagonal tiles), so the split points, and with them every
+rounding, are those of the reference's recursion; the factor is written in place into one output
+array instead of being concatenated level by level.
 """
 import torch
 
@@ -13,33 +18,36 @@ from .rns import dev
 
 
 class BMP:
-    """(batched.py:16-42)"""
+    """stack of nb fixed-point matrices (see the module docstring)"""
     __slots__ = ("limbs", "exp")
 
     def __init__(self, limbs, exp):
-        self.limbs, self.exp = limbs, exp            # exp: int or (nb, C) tensor
-
-    @property
-    def shape(self):
-        return tuple(self.limbs.shape[1:])
+        self.limbs, self.exp = limbs, exp
 
     @property
     def L(self):
         return self.limbs.shape[0]
 
+    @property
+    def shape(self):
+        return tuple(self.limbs.shape[1:])
+
     def expt(self):
-        nb, R, C = self.shape
+        """(nb, C) exponents, materialised when uniform"""
         if isinstance(self.exp, int):
+            nb, _, C = self.shape
             return torch.full((nb, C), self.exp, dtype=torch.int64, device=self.limbs.device)
         return self.exp
 
     def __getitem__(self, idx):
-        r, c = idx
-        e = self.exp if isinstance(self.exp, int) else self.exp[:, c].contiguous()
-        return BMP(self.limbs[:, :, r, c].contiguous(), e)
+        rows, cols = idx
+        e = self.exp if isinstance(self.exp, int) else self.exp[:, cols].contiguous()
+        return BMP(self.limbs[:, :, rows, cols].contiguous(), e)
 
     def t(self):
-        assert isinstance(self.exp, int)
+        """per-block transpose (uniform exponent only)"""
+        if not isinstance(self.exp, int):
+            raise ValueError("transpose needs a uniform exponent")
         return BMP(self.limbs.transpose(2, 3).contiguous(), self.exp)
 
     def flat(self):
@@ -47,26 +55,30 @@ class BMP:
 
 
 def brescale(A, new_exp, Lout=None):
-    """re-round every block to exponent(s) new_exp (int or (nb, C)) (batched.py:45-52)"""
+    """A re-rounded to new_exp (int, or (nb, C) per block and column), into Lout limbs"""
     nb, R, C = A.shape
     Lo = Lout or A.L
     if isinstance(new_exp, int) and isinstance(A.exp, int):
-        s = torch.full((nb * R * C,), new_exp - A.exp, dtype=torch.int64, device=A.limbs.device)
-        return BMP(shift_round(A.flat(), s, Lo).view(Lo, nb, R, C), new_exp)
-    ne = new_exp if torch.is_tensor(new_exp) else torch.full((nb, C), int(new_exp), dtype=torch.int64,
-                                                              device=A.limbs.device)
-    s = (ne - A.expt())[:, None, :].expand(nb, R, C).reshape(-1)
-    return BMP(shift_round(A.flat(), s, Lo).view(Lo, nb, R, C), ne.clone())
+        shift = torch.full((nb * R * C,), new_exp - A.exp, dtype=torch.int64, device=A.limbs.device)
+        return BMP(shift_round(A.flat(), shift, Lo).view(Lo, nb, R, C), new_exp)
+    if not torch.is_tensor(new_exp):
+        new_exp = torch.full((nb, C), int(new_exp), dtype=torch.int64, device=A.limbs.device)
+    shift = (new_exp - A.expt()).unsqueeze(1).expand(nb, R, C).reshape(-1)
+    return BMP(shift_round(A.flat(), shift, Lo).view(Lo, nb, R, C), new_exp.clone())
 
 
 def badd(A, B, alpha=1, out_exp=None):
-    """A + alpha*B: exact when both exponents equal out_exp (int), else 'align' (both rounded to
-    max(ea, eb) + 1 per column, then added) (batched.py:82-92)"""
+    """A + alpha B: an exact limb addition when both operands sit at the int exponent out_exp; with
+    out_exp 'align', both rounded to one more than the larger exponent of each (block, column) and
+    added (one fused kernel)"""
     nb, R, C = A.shape
     L = A.L
-    if isinstance(out_exp, int) and A.exp == out_exp and B.exp == out_exp:
+    if isinstance(out_exp, int):
+        if A.exp != out_exp or B.exp != out_exp:
+            raise ValueError("exact addition needs both operands at the output exponent")
         return BMP(add_same(A.flat(), B.flat(), alpha).view(L, nb, R, C), out_exp)
-    assert out_exp == "align"
+    if out_exp != "align":
+        raise ValueError("out_exp must be an int or 'align'")
     out = torch.empty((L, nb, R, C), dtype=torch.int32, device=A.limbs.device)
     e = torch.empty((nb, C), dtype=torch.int64, device=A.limbs.device)
     Al, Bl = A.flat().contiguous(), B.flat().contiguous()
@@ -163,127 +175,153 @@ def _bcrt_round_panel(R, acc, nb, M, N, j0, jw, e_in, out_exp, L, out, exps):
 
 # ------------------------------------------------------------------ batched Cholesky / TRSM
 class BTile:
-    """factor of a diagonal tile: Lmp (the factor, fixed point) and X (explicit inverse, one
+    """a factored diagonal tile: Lmp, the factor (fixed point), and X, its explicit inverse (one
     exponent per block)"""
     __slots__ = ("X", "Lmp")
 
 
 def bchol_tile(A, LI=24, threads=256):
-    """A: BMP (nb, b, b) fixed point at exponent -(32L-40) -> (L BMP, BTile with the inverse X)
-    (batched.py:243-262)"""
+    """factor and invert the bt x bt tiles A (nb, bt, bt) (fixed point at -(32L - 40)) on the GPU:
+    (L as a BMP, BTile) with X rounded to headroom_bits(L) below its largest entry, per block"""
     from .basecase import tile_factor
     from .limbops import bitlen_words
     nb, b, _ = A.shape
     L = A.L
-    e = A.exp
-    WL, F = L + LI, 32 * L
-    assert isinstance(e, int) and e == -(32 * L - 40), "tiles enter at the fixed-point exponent"
-    Lout_l, Xs, _ = tile_factor(A.limbs, b)
-    Lout = BMP(Lout_l, e)
-    top = bitlen_words(Xs.reshape(WL, -1)).view(nb, b * b).amax(1) - F
-    ex = (top - headroom_bits(L))[:, None].expand(nb, b).contiguous()
-    t = BTile()
-    t.Lmp = Lout
-    t.X = brescale(BMP(Xs, torch.full((nb, b), -F, dtype=torch.int64, device=Xs.device)), ex, L)
-    return Lout, t
+    if A.exp != -(32 * L - 40):
+        raise ValueError("tiles enter at the fixed-point exponent")
+    Fb = 32 * L
+    Lw, Xw, _ = tile_factor(A.limbs, b)
+    top = bitlen_words(Xw.reshape(L + LI, -1)).view(nb, b * b).amax(1) - Fb
+    tile = BTile()
+    tile.Lmp = BMP(Lw, A.exp)
+    wide = BMP(Xw, torch.full((nb, b), -Fb, dtype=torch.int64, device=Xw.device))
+    tile.X = brescale(wide, (top - headroom_bits(L)).unsqueeze(1).expand(nb, b).contiguous(), L)
+    return tile.Lmp, tile
 
 
-def _split(n, bt):
-    return (n // 2 + bt - 1) // bt * bt
+class _Node:
+    """a diagonal range [lo, lo + n) of the split tree; `head`/`tail` its two halves (None for a
+    leaf tile); `first` the index of its first leaf tile, `count` its number of tiles"""
+    __slots__ = ("lo", "n", "head", "tail", "first", "count")
+
+    def __init__(self, lo, n, bt, first):
+        self.lo, self.n, self.first = lo, n, first
+        if n <= bt:
+            self.head = self.tail = None
+            self.count = 1
+            return
+        k = (n // 2 + bt - 1) // bt * bt                      # leading part: half, up to a tile multiple
+        self.head = _Node(lo, k, bt, first)
+        self.tail = _Node(lo + k, n - k, bt, first + self.head.count)
+        self.count = self.head.count + self.tail.count
+
+    @property
+    def k(self):
+        return self.head.n
 
 
-def _ntiles(n, bt):
-    if n <= bt:
-        return 1
-    n1 = _split(n, bt)
-    return _ntiles(n1, bt) + _ntiles(n - n1, bt)
+def _tree(n, bt):
+    return _Node(0, n, bt, 0)
 
 
 def bchol(A, bt=32, refine=1):
-    """batched Cholesky of nb SPD equilibrated fixed-point blocks (nb, n, n) -> (L, tiles)
-    (batched.py:274-288)"""
+    """Cholesky factors of nb SPD equilibrated fixed-point blocks (nb, n, n), n a multiple of bt:
+    (L, list of factored diagonal tiles in tree order). Each internal node factors its head, solves
+    for the panel below it (btrsm_right), subtracts the panel's Gram matrix exactly and factors
+    the tail; the factor is written into one array as it is produced."""
     nb, n, _ = A.shape
-    L = A.L
     e = A.exp
-    if n <= bt:
-        assert n == bt
-        Lt, t = bchol_tile(A)
-        return Lt, [t]
-    n1 = _split(n, bt)
-    L11, t1 = bchol(A[:n1, :n1], bt, refine)
-    L21 = btrsm_right(A[n1:, :n1], L11, t1, bt, refine)
-    S = badd(A[n1:, n1:], bgemm_nt(L21, L21, out_exp=e), -1, out_exp=e)
-    L22, t2 = bchol(S, bt, refine)
-    out = torch.zeros((L, nb, n, n), dtype=torch.int32, device=A.limbs.device)
-    out[:, :, :n1, :n1] = L11.limbs
-    out[:, :, n1:, :n1] = L21.limbs
-    out[:, :, n1:, n1:] = L22.limbs
-    return BMP(out, e), t1 + t2
+    out = torch.zeros((A.L, nb, n, n), dtype=torch.int32, device=A.limbs.device)
+    tiles = []
+
+    def factor(node, M):
+        lo = node.lo
+        if node.head is None:
+            if node.n != bt:
+                raise ValueError("the dimension must be a multiple of the tile size")
+            Lt, t = bchol_tile(M)
+            tiles.append(t)
+            out[:, :, lo:lo + bt, lo:lo + bt] = Lt.limbs
+            return
+        k, hi = node.k, lo + node.n
+        factor(node.head, M[:k, :k])
+        L11 = BMP(out[:, :, lo:lo + k, lo:lo + k].contiguous(), e)
+        panel = _right_solve(node.head, M[k:, :k], L11, tiles, refine, lo)
+        out[:, :, lo + k:hi, lo:lo + k] = panel.limbs
+        factor(node.tail, badd(M[k:, k:], bgemm_nt(panel, panel, out_exp=e), -1, out_exp=e))
+
+    factor(_tree(n, bt), A)
+    return BMP(out, e), tiles
 
 
-def btrsm_right(B, Lf, tiles, bt=32, refine=1):
-    """X with X L^T = B, batched; fixed point (batched.py:291-305)"""
-    nb, m, n = B.shape
+def _right_solve(node, B, Lf, tiles, refine, base):
+    """X with X Lf^T = B for the node's diagonal block Lf (node offsets relative to `base`)"""
     e = Lf.exp
-    if n <= bt:
-        t = tiles[0]
+    if node.head is None:
+        t = tiles[node.first]
         X = bgemm_nt(B, t.X, out_exp=e, bt_rowexp=t.X.exp)
-        for _ in range(refine):
-            Rr = badd(B, bgemm_nt(X, t.Lmp, out_exp=e), -1, out_exp=e)
-            X = badd(X, bgemm_nt(Rr, t.X, out_exp=e, bt_rowexp=t.X.exp), 1, out_exp=e)
+        for _ in range(refine):                              # one residual correction per step
+            resid = badd(B, bgemm_nt(X, t.Lmp, out_exp=e), -1, out_exp=e)
+            X = badd(X, bgemm_nt(resid, t.X, out_exp=e, bt_rowexp=t.X.exp), 1, out_exp=e)
         return X
-    n1 = _split(n, bt)
-    k1 = _ntiles(n1, bt)
-    X1 = btrsm_right(B[:, :n1], Lf[:n1, :n1], tiles[:k1], bt, refine)
-    B2 = badd(B[:, n1:], bgemm_nt(X1, Lf[n1:, :n1], out_exp=e), -1, out_exp=e)
-    X2 = btrsm_right(B2, Lf[n1:, n1:], tiles[k1:], bt, refine)
+    k = node.k
+    X1 = _right_solve(node.head, B[:, :k], Lf[:k, :k], tiles, refine, base)
+    rest = badd(B[:, k:], bgemm_nt(X1, Lf[k:, :k], out_exp=e), -1, out_exp=e)
+    X2 = _right_solve(node.tail, rest, Lf[k:, k:], tiles, refine, base)
     return BMP(torch.cat([X1.limbs, X2.limbs], 3), e)
 
 
-def _tr(B):
-    """limbs of B^T per block, for use as the 'Bt' operand (row exponents passed separately)"""
+def btrsm_right(B, Lf, tiles, bt=32, refine=1):
+    """X with X L^T = B, batched, fixed point (B (nb, m, n), L from bchol)"""
+    return _right_solve(_tree(B.shape[2], bt), B, Lf, tiles, refine, 0)
+
+
+def _as_rows(B):
+    """B^T per block as the row-exponent operand of bgemm_nt (its column exponents go separately)"""
     return BMP(B.limbs.transpose(2, 3).contiguous(), 0)
 
 
-def _bleaf(X, Lm, B, refine):
-    """P = Lm^{-1} B with the explicit tile inverse X (+ refinement steps) (batched.py:313-319)"""
-    P = bgemm_nt(X, _tr(B), out_exp="col", bt_rowexp=B.expt())
+def _apply_tile(X, Lm, B, refine):
+    """Lm^-1 B through the explicit tile inverse X, then `refine` residual corrections, with
+    per-(block, column) exponents"""
+    P = bgemm_nt(X, _as_rows(B), out_exp="col", bt_rowexp=B.expt())
     for _ in range(refine):
-        Rr = badd(B, bgemm_nt(Lm, _tr(P), out_exp="col", bt_rowexp=P.expt()), -1, out_exp="align")
-        P = badd(P, bgemm_nt(X, _tr(Rr), out_exp="col", bt_rowexp=Rr.expt()), 1, out_exp="align")
+        resid = badd(B, bgemm_nt(Lm, _as_rows(P), out_exp="col", bt_rowexp=P.expt()), -1, out_exp="align")
+        P = badd(P, bgemm_nt(X, _as_rows(resid), out_exp="col", bt_rowexp=resid.expt()), 1, out_exp="align")
     return P
 
 
-def btrsm_left(Lf, B, tiles, bt=32, refine=0):
-    """P = L^{-1} B batched; B (nb, n, m) with per-(block, column) exponents (batched.py:322-335)"""
-    nb, n, m = B.shape
-    if n <= bt:
-        t = tiles[0]
-        return _bleaf(t.X, t.Lmp, B, refine)
-    n1 = _split(n, bt)
-    k1 = _ntiles(n1, bt)
-    P1 = btrsm_left(Lf[:n1, :n1], B[:n1, :], tiles[:k1], bt, refine)
-    U = bgemm_nt(Lf[n1:, :n1], _tr(P1), out_exp="col", bt_rowexp=P1.expt())
-    B2 = badd(B[n1:, :], U, -1, out_exp="align")
-    P2 = btrsm_left(Lf[n1:, n1:], B2, tiles[k1:], bt, refine)
+def _stack_common(P1, P2):
+    """[P1; P2] with one exponent per (block, column): both parts re-rounded to the larger"""
     ee = torch.maximum(P1.expt(), P2.expt())
-    P1, P2 = brescale(P1, ee), brescale(P2, ee)
-    return BMP(torch.cat([P1.limbs, P2.limbs], 2), ee)
+    return BMP(torch.cat([brescale(P1, ee).limbs, brescale(P2, ee).limbs], 2), ee)
+
+
+def _left_solve(node, Lf, B, tiles, refine, upper):
+    """L^-1 B (upper=False, forward: head first) or L^-T B (upper=True, backward: tail first)"""
+    if node.head is None:
+        t = tiles[node.first]
+        if upper:
+            XT = BMP(t.X.limbs.transpose(2, 3).contiguous(), t.X.exp)
+            return _apply_tile(XT, t.Lmp.t(), B, refine)
+        return _apply_tile(t.X, t.Lmp, B, refine)
+    k = node.k
+    if not upper:
+        P1 = _left_solve(node.head, Lf[:k, :k], B[:k, :], tiles, refine, upper)
+        coupled = bgemm_nt(Lf[k:, :k], _as_rows(P1), out_exp="col", bt_rowexp=P1.expt())
+        P2 = _left_solve(node.tail, Lf[k:, k:], badd(B[k:, :], coupled, -1, out_exp="align"), tiles, refine, upper)
+    else:
+        P2 = _left_solve(node.tail, Lf[k:, k:], B[k:, :], tiles, refine, upper)
+        coupled = bgemm_nt(Lf[k:, :k].t(), _as_rows(P2), out_exp="col", bt_rowexp=P2.expt())
+        P1 = _left_solve(node.head, Lf[:k, :k], badd(B[:k, :], coupled, -1, out_exp="align"), tiles, refine, upper)
+    return _stack_common(P1, P2)
+
+
+def btrsm_left(Lf, B, tiles, bt=32, refine=0):
+    """P = L^-1 B batched; B (nb, n, m) with per-(block, column) exponents, and so is P"""
+    return _left_solve(_tree(B.shape[1], bt), Lf, B, tiles, refine, False)
 
 
 def btrsm_left_T(Lf, B, tiles, bt=32, refine=1):
-    """P = L^{-T} B batched (L lower, fixed point from bchol) (batched.py:338-352)"""
-    nb, n, m = B.shape
-    if n <= bt:
-        t = tiles[0]
-        XT = BMP(t.X.limbs.transpose(2, 3).contiguous(), t.X.exp)
-        return _bleaf(XT, t.Lmp.t(), B, refine)
-    n1 = _split(n, bt)
-    k1 = _ntiles(n1, bt)
-    P2 = btrsm_left_T(Lf[n1:, n1:], B[n1:, :], tiles[k1:], bt, refine)
-    U = bgemm_nt(Lf[n1:, :n1].t(), _tr(P2), out_exp="col", bt_rowexp=P2.expt())
-    B1 = badd(B[:n1, :], U, -1, out_exp="align")
-    P1 = btrsm_left_T(Lf[:n1, :n1], B1, tiles[:k1], bt, refine)
-    ee = torch.maximum(P1.expt(), P2.expt())
-    P1, P2 = brescale(P1, ee), brescale(P2, ee)
-    return BMP(torch.cat([P1.limbs, P2.limbs], 2), ee)
+    """P = L^-T B batched (L lower, fixed point from bchol)"""
+    return _left_solve(_tree(B.shape[1], bt), Lf, B, tiles, refine, True)

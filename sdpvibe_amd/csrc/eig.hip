@@ -63,21 +63,28 @@ __global__ void __launch_bounds__(256) k_mpf_from_double(const double* __restric
 // Batched symmetric eigen-decomposition, one workgroup per n x n matrix: W = (A + A^T)/2 copied to
 // the work area, then Brent-Luk parallel cyclic Jacobi (n/2 disjoint rotations per step, n-1
 // steps per sweep) until the off-diagonal mass is below 2^-110 of ||W||_F^2 or 40 sweeps.
+// The final off-diagonal Frobenius norm is returned per matrix (off): by Weyl's inequality every
+// sorted diagonal entry is within ||offdiag(W)||_2 <= off of the matching exact eigenvalue of W,
+// so callers add it to their error bound (a matrix that did not converge gets a wide interval
+// instead of a silently wrong one). The rotation pairs live in dynamic LDS (24 bytes per pair).
 // Rotation of pair (p, q), Golub & Van Loan sym.schur2: theta = (w_qq - w_pp) / (2 w_pq),
 // t = sign(theta) / (|theta| + sqrt(1 + theta^2)), c = 1/sqrt(1 + t^2), s = t c; W <- J^T W J,
 // V <- V J. Outputs: eigenvalues ascending (w), eigenvectors as columns of V (same order) when
 // V is given, and ||W||_F (fro).
 __global__ void __launch_bounds__(ET) k_symeig(const double* __restrict__ A, i64 nb, int n, double* __restrict__ Wk,
                                               double* __restrict__ w, double* __restrict__ V,
-                                              double* __restrict__ fro) {
+                                              double* __restrict__ fro, double* __restrict__ offn) {
     __shared__ double red[ET];
-    __shared__ double rc[128], rs[128];
-    __shared__ int pp[128], qq[128];
     __shared__ int flag;
+    SDPV_DYNAMIC_LDS(double, pairs);
     const int t = threadIdx.x;
     const i64 b = blockIdx.x;
     const int m = n + (n & 1);                          // even player count (dummy index n if odd)
     const int h = m / 2;
+    double* rc = pairs;                                 // rotation cosines (h)
+    double* rs = pairs + h;                             // rotation sines (h)
+    int* pp = reinterpret_cast<int*>(pairs + 2 * h);    // pair rows p < q (h each)
+    int* qq = pp + h;
     const double* a = A + b * (i64)n * n;
     double* W = Wk + b * (i64)m * m;
     double* Vb = V ? V + b * (i64)n * n : nullptr;
@@ -121,12 +128,12 @@ __global__ void __launch_bounds__(ET) k_symeig(const double* __restrict__ A, i64
         if (flag) break;
         for (int step = 0; step < m - 1; step++) {
             // round-robin pairing: position 0 fixed, the others rotate by `step`
-            if (t < h) {
+            for (int k = t; k < h; k += ET) {
                 auto player = [&](int pos) { return pos == 0 ? 0 : 1 + (pos - 1 + step) % (m - 1); };
-                int p = player(t), q = player(m - 1 - t);
+                int p = player(k), q = player(m - 1 - k);
                 if (p > q) { const int x = p; p = q; q = x; }
-                pp[t] = p;
-                qq[t] = q;
+                pp[k] = p;
+                qq[k] = q;
                 const double apq = W[(i64)p * m + q];
                 double c = 1.0, s = 0.0;
                 if (apq != 0.0) {
@@ -136,8 +143,8 @@ __global__ void __launch_bounds__(ET) k_symeig(const double* __restrict__ A, i64
                     c = 1.0 / sqrt(1.0 + tt * tt);
                     s = tt * c;
                 }
-                rc[t] = c;
-                rs[t] = s;
+                rc[k] = c;
+                rs[k] = s;
             }
             __syncthreads();
             // rows p, q of every pair: W <- J^T W
@@ -168,12 +175,29 @@ __global__ void __launch_bounds__(ET) k_symeig(const double* __restrict__ A, i64
                 vi[q] = s * vx + c * vy;
             }
             __syncthreads();
-            if (t < h && rs[t] != 0.0) {
-                W[(i64)pp[t] * m + qq[t]] = 0.0;
-                W[(i64)qq[t] * m + pp[t]] = 0.0;
+            for (int k = t; k < h; k += ET) {
+                if (rs[k] == 0.0) continue;
+                W[(i64)pp[k] * m + qq[k]] = 0.0;
+                W[(i64)qq[k] * m + pp[k]] = 0.0;
             }
             __syncthreads();
         }
+    }
+    // remaining off-diagonal mass (bounds the distance of the diagonal to the spectrum)
+    {
+        double off = 0.0;
+        for (int idx = t; idx < m * m; idx += ET) {
+            const int i = idx / m, j = idx % m;
+            if (i != j) off += W[idx] * W[idx];
+        }
+        red[t] = off;
+        __syncthreads();
+        for (int st = ET / 2; st > 0; st >>= 1) {
+            if (t < st) red[t] += red[t + st];
+            __syncthreads();
+        }
+        if (t == 0 && offn) offn[b] = sqrt(red[0]);
+        __syncthreads();
     }
     // ascending order of the diagonal (ties by index), eigenvectors permuted alike
     for (int i = t; i < n; i += ET) {
@@ -215,12 +239,15 @@ int sdpv_mpf_from_double(const double* v, int64_t n, int64_t L, int32_t* m, int6
 }
 
 int sdpv_symeig(const double* A, int64_t nb, int64_t n, double* work, double* w, double* V, double* fro,
-                void* stream) {
-    if (nb < 0 || n < 1 || n > 255) return SDPV_EARG;
+                double* off, void* stream) {
+    if (nb < 0 || n < 1) return SDPV_EARG;
+    if (n > SDPV_SYMEIG_NMAX || nb > 0x7fffffffLL) return SDPV_ESIZE;
     if (nb == 0) return SDPV_OK;
     if (!A || !work || !w) return SDPV_EARG;
-    if (nb > 0x7fffffffLL) return SDPV_ESIZE;
-    hipLaunchKernelGGL(k_symeig, dim3((unsigned)nb), dim3(ET), 0, as_stream(stream), A, nb, (int)n, work, w, V, fro);
+    const i64 h = (n + (n & 1)) / 2;
+    const size_t lds = (size_t)h * (2 * sizeof(double) + 2 * sizeof(int));
+    hipLaunchKernelGGL(k_symeig, dim3((unsigned)nb), dim3(ET), lds, as_stream(stream), A, nb, (int)n, work, w, V, fro,
+                       off);
     SDPV_TRY_LAUNCH();
     return SDPV_OK;
 }

@@ -25,6 +25,7 @@ LMAX_WIDE = 1 << 20                                  # normalize() takes any inp
 
 
 class MPF:
+    """multiprecision floats: m (L, *shape) int32 limb planes, e (*shape) int64"""
     __slots__ = ("m", "e")
 
     def __init__(self, m, e):
@@ -39,17 +40,16 @@ class MPF:
         return tuple(self.e.shape)
 
     def __getitem__(self, idx):
-        if not isinstance(idx, tuple):
-            idx = (idx,)
-        return MPF(self.m[(slice(None),) + idx].contiguous(), self.e[idx].contiguous())
+        idx = idx if isinstance(idx, tuple) else (idx,)
+        return MPF(self.m[(slice(None), *idx)].contiguous(), self.e[idx].contiguous())
 
     @property
     def T(self):
-        """transpose of the last two dimensions"""
+        """transpose of the last two dimensions (a contiguous copy)"""
         return MPF(self.m.transpose(-1, -2).contiguous(), self.e.transpose(-1, -2).contiguous())
 
-    def reshape(self, *s):
-        return MPF(self.m.reshape(self.L, *s), self.e.reshape(*s))
+    def reshape(self, *shape):
+        return MPF(self.m.reshape(self.L, *shape), self.e.reshape(*shape))
 
     def clone(self):
         return MPF(self.m.clone(), self.e.clone())
@@ -57,36 +57,36 @@ class MPF:
     def __neg__(self):
         return add(zeros(self.shape, self.L), self, -1)
 
-    def __add__(self, o):
-        return add(self, o, 1)
+    def __add__(self, other):
+        return add(self, other, 1)
 
-    def __sub__(self, o):
-        return add(self, o, -1)
+    def __sub__(self, other):
+        return add(self, other, -1)
 
     def shift(self, k):
-        """multiply by 2^k (k: int or tensor broadcastable to shape) -- exact (mpf.py:194-197)"""
-        e = self.e + k
-        return MPF(self.m, torch.where(self.e == EZ, EZ, e))
+        """exact multiplication by 2^k (k an int or a tensor broadcastable to the shape); zero stays zero"""
+        return MPF(self.m, torch.where(self.e == EZ, self.e, self.e + k))
 
     def to_mp(self):
-        """-> numpy object array of mpmath mpf (host; tests / scalars)"""
+        """numpy object array of mpmath values (host; exact)"""
         L = self.L
-        a = self.m.reshape(L, -1).t().contiguous().cpu().numpy().astype(np.uint32)
-        ex = self.e.reshape(-1).cpu().numpy()
-        out = np.empty(a.shape[0], dtype=object)
-        for i in range(a.shape[0]):
-            v = int.from_bytes(a[i].tobytes(), "little", signed=True)
-            out[i] = mp.mpf(0) if v == 0 else mp.mpf((v, int(ex[i])))
+        words = self.m.reshape(L, -1).t().contiguous().cpu().numpy()
+        exps = self.e.reshape(-1).cpu().tolist()
+        vals = []
+        for row, ex in zip(words, exps):
+            v = int.from_bytes(row.tobytes(), "little", signed=True)
+            vals.append(mp.mpf((v, ex)) if v else mp.mpf(0))
+        out = np.empty(len(vals), dtype=object)
+        out[:] = vals
         return out.reshape(self.shape)
 
     def to_float(self):
-        """approximate float64 values: the top two limbs times 2^(e + 32(L-2)), the exponent
-        clamped to [-1100, 1100]; host numpy (mpf.py:210-215). The power of two is built exactly
-        from its bit pattern (a generic pow may round for subnormal results)."""
-        top = self.m[-1].double() * 2.0 ** 32 + (self.m[-2].double() % 2.0 ** 32) if self.L > 1 else self.m[-1].double()
-        sc = (self.e + 32 * (self.L - 2)).clamp(-1100, 1100)
-        v = torch.where(self.e == EZ, torch.zeros_like(top), top * _pow2(sc))
-        return v.cpu().numpy()
+        """float64 approximations (host numpy): the top two limbs times 2^(e + 32(L - 2)), the exponent
+        clamped to [-1100, 1100], each power of two built exactly from its bit pattern"""
+        hi = self.m[-1].double()
+        top = hi * 2.0 ** 32 + (self.m[-2].double() % 2.0 ** 32) if self.L > 1 else hi
+        scale = _pow2((self.e + 32 * (self.L - 2)).clamp(-1100, 1100))
+        return torch.where(self.e == EZ, torch.zeros_like(top), top * scale).cpu().numpy()
 
 
 def _pow2(k):
@@ -99,114 +99,108 @@ def _pow2(k):
     return torch.where(k > 1023, torch.full_like(v, float("inf")), v)
 
 
+def NBITS(L):
+    """mantissa bits of a normalised entry"""
+    return 32 * L - 2
+
+
 def zeros(shape, L):
+    shape = tuple(shape)
     return MPF(torch.zeros((L, *shape), dtype=torch.int32, device=dev),
                torch.full(shape, EZ, dtype=torch.int64, device=dev))
 
 
-def NBITS(L):
-    return 32 * L - 2
+# ------------------------------------------------------------------ host -> device constructors
+def _pack(ints, L):
+    """Python ints -> (L, n) int32 limb planes (two's complement), via one byte buffer"""
+    nbytes = 4 * L
+    raw = b"".join(v.to_bytes(nbytes, "little", signed=True) for v in ints)
+    return np.frombuffer(raw, dtype=np.int32).reshape(len(ints), L)
 
 
 def from_ints(ints, exps, L, shape):
-    """Python ints (mantissas) and exponents (flat lists) -> normalised MPF (mpf.py:226-235)"""
-    n = len(ints)
-    nb = 4 * L
-    buf = bytearray(nb * n)
-    for i, v in enumerate(ints):
-        buf[i * nb:(i + 1) * nb] = v.to_bytes(nb, "little", signed=True)
-    m = torch.from_numpy(np.frombuffer(bytes(buf), dtype=np.int32).reshape(n, L).T.copy()).to(dev)
-    e = torch.tensor(exps, dtype=torch.int64, device=dev)
-    return normalize(m, e, L).reshape(*shape)
+    """mantissas (Python ints) and exponents -> normalised MPF of the given shape"""
+    words = torch.from_numpy(np.ascontiguousarray(_pack(ints, L).T)).to(dev)
+    return normalize(words, torch.tensor(exps, dtype=torch.int64, device=dev), L).reshape(*shape)
+
+
+def _round_mag(v, sh):
+    """|v| / 2^sh rounded half up (sh > 0), sign kept"""
+    r = (abs(v) + (1 << (sh - 1))) >> sh
+    return -r if v < 0 else r
 
 
 def _dec_to_bin(t, NBb):
-    """decimal string -> (signed mantissa with <= NBb bits, binary exponent) (mpf.py:238-261)"""
+    """decimal string -> (mantissa of at most NBb bits, binary exponent), the reference's
+    conversion (sdpvibe/mpf.py:238-261): integers that fit are exact; larger integers are rounded
+    once (half up) to NBb bits; fractions are first rounded to the nearest multiple of 2^-sh with
+    sh large enough for NBb + 1 significant bits, then halved (half up) until they fit"""
     t = t.strip()
-    neg = t.startswith("-")
-    if t[0] in "+-":
-        t = t[1:]
-    e10 = 0
-    if "e" in t or "E" in t:
-        t, ex = t.replace("E", "e").split("e")
-        e10 = int(ex)
-    if "." in t:
-        ip, fp = t.split(".")
-        e10 -= len(fp)
-        t = ip + fp
-    man = int(t) if t else 0
+    neg = t[:1] == "-"
+    t = t.lstrip("+-")
+    mant_s, _, exp_s = t.replace("E", "e").partition("e")
+    e10 = int(exp_s) if exp_s else 0
+    ip, _, fp = mant_s.partition(".")
+    e10 -= len(fp)
+    man = int(ip + fp) if (ip + fp) else 0
     if man == 0:
         return 0, EZ
+    sgn = -1 if neg else 1
     if e10 >= 0:
         v = man * 10 ** e10
-        bl = v.bit_length()
-        if bl <= NBb:
-            return (-v if neg else v), 0
-        sh = bl - NBb
-        v = (v + (1 << (sh - 1))) >> sh
-        return (-v if neg else v), sh
-    den = 10 ** (-e10)
+        over = v.bit_length() - NBb
+        if over <= 0:
+            return sgn * v, 0
+        return sgn * _round_mag(v, over), over
+    den = 10 ** -e10
     sh = NBb + den.bit_length() - man.bit_length() + 1
     q = ((man << sh) + den // 2) // den
     ex = -sh
     while q.bit_length() > NBb:
         q = (q + 1) >> 1
         ex += 1
-    return (-q if neg else q), ex
+    return sgn * q, ex
 
 
 def from_strings(strs, L, shape=None):
-    """decimal strings -> MPF (mpf.py:264-270)"""
-    NBb = NBITS(L)
-    ints, exps = [], []
-    for t in strs:
-        m_, e_ = _dec_to_bin(t, NBb)
-        ints.append(m_)
-        exps.append(e_)
-    return from_ints(ints, exps, L, shape or (len(ints),))
+    """decimal strings -> normalised MPF"""
+    pairs = [_dec_to_bin(t, NBITS(L)) for t in strs]
+    return from_ints([m_ for m_, _ in pairs], [e_ for _, e_ in pairs], L, shape or (len(pairs),))
 
 
 def strings_to_host(strs, L):
-    """decimal strings -> (limbs int32 (n, L) entry-major numpy, exps int64 numpy) (mpf.py:273-281)"""
-    NBb = NBITS(L)
-    nb = 4 * L
-    n = len(strs)
-    buf = bytearray(nb * n)
-    exps = np.empty(n, dtype=np.int64)
-    for i, t in enumerate(strs):
-        m_, e_ = _dec_to_bin(t, NBb)
-        buf[i * nb:(i + 1) * nb] = m_.to_bytes(nb, "little", signed=True)
-        exps[i] = e_
-    return np.frombuffer(bytes(buf), dtype=np.int32).reshape(n, L), exps
+    """decimal strings -> (limbs (n, L) int32 entry-major, exponents (n,) int64) host arrays"""
+    pairs = [_dec_to_bin(t, NBITS(L)) for t in strs]
+    limbs = _pack([m_ for m_, _ in pairs], L)
+    return limbs, np.array([e_ for _, e_ in pairs], dtype=np.int64)
 
 
 def host_to_mpf(limbs, exps, L, shape):
-    """inverse of strings_to_host, on the GPU (normalised) (mpf.py:284-288)"""
-    m = torch.from_numpy(np.ascontiguousarray(limbs.T)).to(dev)
-    e = torch.from_numpy(exps).to(dev)
-    return normalize(m, e, L).reshape(*shape)
+    """host (limbs (n, L) entry-major, exponents (n,)) -> normalised MPF on the GPU"""
+    words = torch.from_numpy(np.ascontiguousarray(np.asarray(limbs).T)).to(dev)
+    return normalize(words, torch.from_numpy(np.asarray(exps, dtype=np.int64)).to(dev), L).reshape(*shape)
 
 
 def from_mp(vals, L, shape=None):
-    """iterable of mpmath numbers / strings -> MPF (rounded to NB bits) (mpf.py:291-304)"""
-    vals = list(vals)
+    """mpmath numbers (taken exactly) or anything mp.mpf accepts -> MPF rounded to NB bits (magnitude
+    half up)"""
     NBb = NBITS(L)
     ints, exps = [], []
     for v in vals:
-        v = mp.mpf(v)
-        if v == 0:
+        if not isinstance(v, mp.mpf):
+            v = mp.mpf(v)
+        sign, man, ex, bc = v._mpf_
+        if not man:
             ints.append(0)
             exps.append(EZ)
             continue
-        sign, man, ex, bc = v._mpf_
         man = int(man)
-        if bc > NBb:                                  # round to NB bits (half up on magnitude)
-            sh = bc - NBb
-            man = (man + (1 << (sh - 1))) >> sh
-            ex += sh
+        if bc > NBb:
+            man = _round_mag(man, bc - NBb)
+            ex += bc - NBb
         ints.append(-man if sign else man)
         exps.append(ex)
-    return from_ints(ints, exps, L, shape or (len(vals),))
+    return from_ints(ints, exps, L, shape or (len(ints),))
 
 
 def normalize(Xw, ein, L):
@@ -245,21 +239,23 @@ def add(A, B, sgn=1):
 
 
 def scalar(v, L):
+    """one-entry MPF of an mpmath-compatible value"""
     return from_mp([v], L, (1,))
 
 
 def eye(n, L, val=1):
+    """val times the n x n identity"""
+    d = scalar(val, L)
     A = zeros((n, n), L)
-    d = from_mp([val], L, (1,))
-    idx = torch.arange(n, device=dev)
-    A.m[:, idx, idx] = d.m[:, :1]
-    A.e[idx, idx] = d.e[0]
+    i = torch.arange(n, device=dev)
+    A.m[:, i, i] = d.m
+    A.e[i, i] = d.e
     return A
 
 
-# ------------------------------------------------------------------ reductions (host results)
+# ------------------------------------------------------------------ reductions (results on the host)
 def _top(A):
-    """per-entry e + bitlen (log2 of magnitude, roughly); EZ for zeros (mpf.py:348-351)"""
+    """per entry e + bitlen(m) (bitlen of ~m for m < 0); EZ for zeros"""
     m, e = A.m.contiguous(), A.e.contiguous()
     out = torch.empty_like(e)
     call("sdpv_mpf_top", ptr(m), ptr(e), A.L, e.numel(), ptr(out), stream())
@@ -267,22 +263,22 @@ def _top(A):
 
 
 def total(A):
-    """exact-ish sum of all entries -> mpmath mpf (error ~ 2^-NB * n * max) (mpf.py:354-366)"""
+    """sum of all entries as an mpmath number: every entry rounded (shift_round) to a common
+    exponent 32 bits below the largest entry exponent, in L + 2 limbs, and the limb columns summed
+    exactly (error below n 2^-NB of the largest entry)"""
+    if A.e.numel() == 0:
+        return mp.mpf(0)
+    base = int(A.e.max()) - 32
+    if base < EZ // 2:
+        return mp.mpf(0)
     L = A.L
-    n = A.e.numel()
-    if n == 0:
-        return mp.mpf(0)
-    t = int(A.e.max()) - 32
-    if t < EZ // 2:
-        return mp.mpf(0)
-    s = (t - A.e.reshape(-1))
-    W = shift_round(A.m.reshape(L, -1), s, L + 2)
-    S = (W.to(torch.int64) & 0xFFFFFFFF).sum(1)
-    top = W[-1].to(torch.int64).sum()
-    S = S.cpu().tolist()
-    top = int(top)
-    v = sum(int(S[k]) << (32 * k) for k in range(L + 1)) + (top << (32 * (L + 1)))
-    return mp.mpf((v, t)) if v else mp.mpf(0)
+    words = shift_round(A.m.reshape(L, -1), base - A.e.reshape(-1), L + 2).to(torch.int64)
+    low = (words[:-1] & 0xFFFFFFFF).sum(1).tolist()              # unsigned lower limbs
+    high = int(words[-1].sum())                                   # signed top limb
+    v = high << (32 * (L + 1))
+    for k, s in enumerate(low):
+        v += int(s) << (32 * k)
+    return mp.mpf((v, base)) if v else mp.mpf(0)
 
 
 def dot(A, B):
@@ -290,24 +286,24 @@ def dot(A, B):
 
 
 def max_abs(A):
-    """(mpf.py:372-381)"""
+    """largest |entry| as an exact mpmath number: the candidates within one binade of the largest
+    top, compared in float64, the chosen entry read exactly"""
     if A.e.numel() == 0:
         return mp.mpf(0)
-    tp = _top(A).reshape(-1)
-    t = int(tp.max())
-    if t <= EZ // 2:
+    tops = _top(A).reshape(-1)
+    tmax = int(tops.max())
+    if tmax <= EZ // 2:
         return mp.mpf(0)
-    cand = torch.nonzero(tp >= t - 1).reshape(-1)
-    Af = A.reshape(-1)
-    vals = MPF(Af.m[:, cand], Af.e[cand]).shift(-t).to_float()
-    i = int(cand[int(np.argmax(np.abs(vals)))])
-    return abs(Af[i:i + 1].to_mp()[0])
+    near = torch.nonzero(tops >= tmax - 1).reshape(-1)
+    flat = A.reshape(-1)
+    approx = np.abs(MPF(flat.m[:, near], flat.e[near]).shift(-tmax).to_float())
+    pick = int(near[int(np.argmax(approx))])
+    return abs(flat[pick:pick + 1].to_mp()[0])
 
 
 def trace(A):
-    n = A.shape[0]
-    idx = torch.arange(n, device=dev)
-    return total(MPF(A.m[:, idx, idx], A.e[idx, idx]))
+    i = torch.arange(A.shape[0], device=A.e.device)
+    return total(MPF(A.m[:, i, i], A.e[i, i]))
 
 
 # ------------------------------------------------------------------ GEMM
@@ -403,96 +399,85 @@ def fx_exp(L):
 
 
 class Chol:
-    """A = L L^T for one (n, n) or nb batched (nb, n, n) SPD matrices, with L = D^-1 Lt,
-    D = diag(2^-k): Lt is fixed point at fx_exp(L), k integer per row (mpf.py:461-532).
+    """A = L L^T for one (n, n) or a batch (nb, n, n) of SPD matrices, with L = D^-1 Lt, D = diag(2^-k):
+    Lt fixed point at fx_exp(L), k one integer per row (the reference's mpf.Chol, sdpvibe/mpf.py:461-532:
+    k_i = floor((top(A_ii) + 1) / 2), D A D rounded to the fixed-point exponent, identity padding to
+    a multiple of bt, then the batched recursive Cholesky). Attributes read by callers: Lt, k, n, L,
+    nb, npad, tiles.
 
-    The equilibrated D A D (unit-order diagonal) is padded with identity to a multiple of bt and
-    factored by the batched recursive Cholesky; solves go through the explicit tile inverses."""
+    dims: optional (nb,) true dimensions of blocks embedded in the top-left corner of A (zeros
+    elsewhere); rows beyond a block's dimension get the identity padding, so each block factors
+    exactly as it would alone. check=False leaves the positive-definiteness test on the device
+    (`not_pd`, a bool tensor) instead of raising, so that no host round trip is needed."""
 
     SOLVE_REFINE = int(os.environ.get("SDPB_GPU_SOLVE_REFINE", "0"))
 
-    def __init__(self, A, bt=32, refine=1, dims=None):
-        """dims: optional (nb,) true dimensions of batched blocks embedded in the top-left corner of
-        A (zeros elsewhere); the rows beyond a block's dimension get the identity padding, as rows
-        beyond n do, so each block factors exactly as it would alone."""
+    def __init__(self, A, bt=32, refine=1, dims=None, check=True):
         from .batched import BMP, bchol
         self.two_d = A.e.dim() == 2
         if self.two_d:
-            A = MPF(A.m[:, None], A.e[None])
+            A = A.reshape(1, *A.shape)
         nb, n, _ = A.shape
         L = A.L
+        dv = A.e.device
         self.nb, self.n, self.L, self.bt, self.refine = nb, n, L, bt, refine
-        d = torch.arange(n, device=A.e.device)
-        top_d = _top(A)[:, d, d]                                    # (nb, n)
-        neg_d = A.m[-1][:, d, d] < 0
-        live = None if dims is None else d[None, :] < torch.as_tensor(dims, device=A.e.device)[:, None]
-        bad = neg_d | (top_d <= EZ // 2)
-        if bool((bad if live is None else bad & live).any()):
-            raise ValueError("matrix is not positive definite (diagonal)")
-        k = torch.div(top_d + 1, 2, rounding_mode="floor")          # A_ii < 2^(2 k_i)
-        if live is not None:
-            k = torch.where(live, k, torch.zeros_like(k))
-        self.k = k
         self.npad = npad = max(bt, -(-n // bt) * bt)
+        r = torch.arange(n, device=dv)
+        dtop = _top(A)[:, r, r]                                       # (nb, n)
+        live = torch.ones((nb, n), dtype=torch.bool, device=dv) if dims is None else \
+            r[None, :] < torch.as_tensor(dims, device=dv)[:, None]
+        self.not_pd = (((A.m[-1][:, r, r] < 0) | (dtop <= EZ // 2)) & live).any()
+        if check and bool(self.not_pd):
+            raise ValueError("matrix is not positive definite (diagonal)")
+        self.k = k = torch.where(live, torch.div(dtop + 1, 2, rounding_mode="floor"), torch.zeros_like(dtop))
         efx = fx_exp(L)
-        # (D A D)_ij = m_ij 2^(e_ij - k_i - k_j), re-rounded to the fixed-point exponent
-        s = efx - (A.e - k[:, :, None] - k[:, None, :])
-        s = torch.where(A.e == EZ, torch.full_like(s, 1 << 30), s).reshape(-1)
-        Af = shift_round(A.m.reshape(L, -1), s, L).view(L, nb, n, n)
-        P = torch.zeros((L, nb, npad, npad), dtype=torch.int32, device=A.m.device)
-        P[:, :, :n, :n] = Af
-        one = 1 << (-efx)                                           # identity padding: 1 = 2^-efx units
-        words = [(one >> (32 * w)) & 0xFFFFFFFF for w in range(L)]
-        words = [v - (1 << 32) if v >> 31 else v for v in words]
-        if npad > n:
-            pad = torch.arange(n, npad, device=A.m.device)
-            for w in range(L):
-                P[w][:, pad, pad] = words[w]
-        if live is not None and not bool(live.all()):
-            bi, ri = (~live).nonzero(as_tuple=True)
-            for w in range(L):
-                P[w][bi, ri, ri] = words[w]
+        # (D A D)_ij = m_ij 2^(e_ij - k_i - k_j) at the fixed-point exponent
+        shift = efx - A.e + k[:, :, None] + k[:, None, :]
+        shift = torch.where(A.e == EZ, torch.full_like(shift, 1 << 30), shift)
+        P = torch.zeros((L, nb, npad, npad), dtype=torch.int32, device=dv)
+        P[:, :, :n, :n] = shift_round(A.m.reshape(L, -1), shift.reshape(-1), L).view(L, nb, n, n)
+        # identity on the padding rows: beyond n, and beyond each block's own dimension
+        ones = (1 << -efx).to_bytes(4 * L, "little", signed=True)
+        one = torch.frombuffer(bytearray(ones), dtype=torch.int32).to(dv)
+        pad = torch.ones((nb, npad), dtype=torch.bool, device=dv)
+        pad[:, :n] = ~live
+        diag = P.diagonal(dim1=-2, dim2=-1)                           # (L, nb, npad) view
+        diag.copy_(torch.where(pad[None], one[:, None, None], diag))
         self.Lt, self.tiles = bchol(BMP(P, efx), bt, refine)
 
     def _to_bmp(self, B):
-        """right-hand sides (nb, n, m) -> fixed point with one exponent per (block, column)"""
+        """right-hand sides (nb, n, m) -> fixed point with one exponent per (block, column), zero rows
+        below n"""
         from .batched import BMP
-        L = self.L
-        Bm, cE = _align_rows(B.T)                                   # (L, nb, m, n), (nb, m)
-        P = torch.zeros((L, self.nb, self.npad, B.shape[-1]), dtype=torch.int32, device=Bm.device)
-        P[:, :, :self.n] = Bm.transpose(2, 3)
-        return BMP(P, cE)
+        aligned, colexp = _align_rows(B.T)                            # (L, nb, m, n), (nb, m)
+        P = torch.zeros((self.L, self.nb, self.npad, B.shape[-1]), dtype=torch.int32, device=aligned.device)
+        P[:, :, :self.n] = aligned.transpose(2, 3)
+        return BMP(P, colexp)
 
     def _from_bmp(self, P):
+        """fixed-point solution (first n rows) -> normalised MPF (nb, n, m)"""
         L, m = self.L, P.shape[2]
-        X = P.limbs[:, :, :self.n].reshape(L, -1).contiguous()
-        ein = P.expt()[:, None, :].expand(self.nb, self.n, m).reshape(-1).contiguous()
-        return normalize(X, ein, L).reshape(self.nb, self.n, m)
+        words = P.limbs[:, :, :self.n].reshape(L, -1).contiguous()
+        ein = P.expt().unsqueeze(1).expand(self.nb, self.n, m).reshape(-1).contiguous()
+        return normalize(words, ein, L).reshape(self.nb, self.n, m)
 
-    def _io(self, B, f):
-        if self.two_d:
-            r = f(MPF(B.m[:, None], B.e[None]))
-            return MPF(r.m[:, 0], r.e[0])
-        return f(B)
+    def _batched(self, B, f):
+        if not self.two_d:
+            return f(B)
+        out = f(B.reshape(1, *B.shape))
+        return out.reshape(*out.shape[1:])
 
     def solve_lower(self, B):
         """L^-1 B = Lt^-1 (D B)"""
         from .batched import btrsm_left
-
-        def f(B):
-            P = btrsm_left(self.Lt, self._to_bmp(B.shift(-self.k[:, :, None])), self.tiles, self.bt,
-                           self.SOLVE_REFINE)
-            return self._from_bmp(P)
-        return self._io(B, f)
+        return self._batched(B, lambda X: self._from_bmp(btrsm_left(
+            self.Lt, self._to_bmp(X.shift(-self.k[:, :, None])), self.tiles, self.bt, self.SOLVE_REFINE)))
 
     def solve_upper(self, B):
         """L^-T B = D (Lt^-T B)"""
         from .batched import btrsm_left_T
-
-        def f(B):
-            P = btrsm_left_T(self.Lt, self._to_bmp(B), self.tiles, self.bt, self.SOLVE_REFINE)
-            return self._from_bmp(P).shift(-self.k[:, :, None])
-        return self._io(B, f)
+        return self._batched(B, lambda X: self._from_bmp(btrsm_left_T(
+            self.Lt, self._to_bmp(X), self.tiles, self.bt, self.SOLVE_REFINE)).shift(-self.k[:, :, None]))
 
     def solve(self, B):
         """A^-1 B"""
@@ -624,6 +609,7 @@ class GramAcc:
         return MPF(out_m, out_e)
 
     def _flush(self):
+        """reconstruct the residue sum into the MPF partial sum and start over"""
         if self.acc is None:
             return
         q = self._crt()
@@ -632,15 +618,20 @@ class GramAcc:
         self.flushes += 1
 
     def next_reference(self, margin=4):
-        """reference exponents for the next accumulation (columns never seen: none)"""
-        return torch.where(self.Tmax > EZ // 2, self.Tmax + margin, self.Tmax)
+        """reference exponents for the next accumulation: this one's column maxima plus margin (EZ
+        for columns never seen)"""
+        seen = self.Tmax > EZ // 2
+        return torch.where(seen, self.Tmax + margin, self.Tmax)
 
     def precision_loss(self):
-        """max over columns of (reference - actual column maximum), in bits"""
-        ok = (self.Tmax > EZ // 2) & (self.E > EZ // 2)
-        return int((self.E - self.Tmax)[ok].max()) if bool(ok.any()) else 0
+        """bits between a column's reference exponent and its actual maximum, largest over columns"""
+        both = (self.Tmax > EZ // 2) & (self.E > EZ // 2)
+        if not bool(both.any()):
+            return 0
+        return int((self.E - self.Tmax).masked_select(both).max())
 
     def result(self):
+        """Q (N, N), valid on and below the diagonal"""
         if self.acc is None:
             return self.part
         q = self._crt()

@@ -4,6 +4,7 @@ An MP matrix holds limbs: int32 tensor of shape (L, rows, cols) -- limb plane k 
 32-bit limb (little endian, two's complement) of every entry -- and exponents:
     value[i, j] = int(limbs[:, i, j]) * 2**exp[j]       (exp: (cols,) int64, or a Python int e0)
 """
+import numpy as np
 import torch
 
 from .limbops import add_same, bitlen_words, shift_round
@@ -15,161 +16,161 @@ RES_BUDGET = 4 << 30
 
 
 class MP:
-    """e0: Python int when the exponent is uniform (avoids device syncs), else None (mpmat.py:14-56)"""
-    __slots__ = ("limbs", "_exp", "e0")
+    """Fixed-point or block-floating matrix: limbs (L, R, C) and either one exponent for the whole
+    matrix (a Python int: no device round trip to read it) or one per column ((C,) int64 tensor).
+    `e0` is the uniform exponent when it is known on the host, else None."""
+    __slots__ = ("limbs", "e0", "_cols")
 
     def __init__(self, limbs, exp):
         self.limbs = limbs
+        self.e0, self._cols = None, None
         if isinstance(exp, int):
-            self.e0, self._exp = exp, None
-        elif torch.is_tensor(exp) and exp.dim() == 0:
-            self.e0, self._exp = None, exp.expand(limbs.shape[2]).clone()
+            self.e0 = exp
+        elif exp.dim() == 0:                           # one exponent held on the device
+            self._cols = exp.reshape(1).repeat(limbs.shape[2])
         else:
-            self.e0, self._exp = None, exp
+            self._cols = exp
 
     @property
-    def exp(self):
-        if self._exp is None:
-            self._exp = torch.full((self.limbs.shape[2],), self.e0, dtype=torch.int64, device=self.limbs.device)
-        return self._exp
+    def L(self):
+        return self.limbs.shape[0]
 
     @property
     def shape(self):
         return tuple(self.limbs.shape[1:])
 
     @property
-    def L(self):
-        return self.limbs.shape[0]
-
-    def t(self):
-        """transpose; requires a single exponent"""
-        return MP(self.limbs.transpose(1, 2).contiguous(), self.uniform_exp())
+    def exp(self):
+        """per-column exponents as a (C,) tensor (materialised from e0 when uniform)"""
+        if self._cols is None:
+            self._cols = torch.full((self.limbs.shape[2],), self.e0, dtype=torch.int64, device=self.limbs.device)
+        return self._cols
 
     def uniform_exp(self):
-        if self.e0 is not None:
-            return self.e0
-        e = int(self._exp[0])
-        assert bool((self._exp == e).all()), "matrix has per-column exponents"
-        self.e0 = e
-        return e
+        """the common exponent as an int (reads the device once when only the columns are known)"""
+        if self.e0 is None:
+            lo, hi = (int(v) for v in torch.aminmax(self._cols))
+            if lo != hi:
+                raise ValueError("matrix has per-column exponents")
+            self.e0 = lo
+        return self.e0
+
+    def t(self):
+        """transpose (needs a common exponent)"""
+        return MP(self.limbs.transpose(1, 2).contiguous(), self.uniform_exp())
 
     def __getitem__(self, idx):
-        r, c = idx
-        if self.e0 is not None:
-            return MP(self.limbs[:, r, c].contiguous(), self.e0)
-        return MP(self.limbs[:, r, c].contiguous(), self._exp[c].contiguous())
+        rows, cols = idx
+        part = self.limbs[:, rows, cols].contiguous()
+        return MP(part, self.e0 if self.e0 is not None else self._cols[cols].contiguous())
 
     def flat(self):
-        """(L, rows*cols) view"""
         return self.limbs.reshape(self.L, -1)
 
 
 def transpose_rows(B):
-    """B (n x m) with per-column exponents -> limbs of B^T (m x n) and the row exponents (m,)"""
+    """(limbs of B^T as an exponent-0 MP, B's column exponents): B^T as the row-scaled operand of
+    gemm_nt"""
     return MP(B.limbs.transpose(1, 2).contiguous(), 0), B.exp
 
 
 def headroom_bits(L):
-    return 32 * L - 32          # target mantissa size for dynamic rounding
+    """mantissa size that dynamic roundings aim at: 32 bits below the top of L limbs"""
+    return 32 * L - 32
 
 
 # ------------------------------------------------------------------ host conversions (tests)
 def from_ints(rows, exp, L):
-    """rows: list of lists of Python ints (mantissas) -> MP (mpmat.py:68-80)"""
+    """list of rows of Python ints (|v| < 2^(32L-1)) -> MP with exponent(s) exp"""
     R, C = len(rows), len(rows[0])
-    mask = (1 << 32) - 1
-    flat = []
-    for row in rows:
-        for v in row:
-            assert abs(v) < 1 << (32 * L - 1)
-            u = v % (1 << (32 * L))
-            flat.append([((u >> (32 * k)) & mask) for k in range(L)])
-    arr = torch.tensor(flat, dtype=torch.int64).view(R, C, L).permute(2, 0, 1).contiguous()
-    arr = torch.where(arr >= 1 << 31, arr - (1 << 32), arr).to(torch.int32)
-    return MP(arr.to(dev), exp)
+    vals = [v for row in rows for v in row]
+    if any(abs(v) >= 1 << (32 * L - 1) for v in vals):
+        raise OverflowError("value does not fit in L limbs")
+    raw = b"".join(v.to_bytes(4 * L, "little", signed=True) for v in vals)
+    words = np.frombuffer(raw, dtype=np.int32).reshape(R, C, L).transpose(2, 0, 1)
+    return MP(torch.from_numpy(np.ascontiguousarray(words)).to(dev), exp)
 
 
 def to_ints(A):
-    """MP -> list of lists of Python int mantissas, list of column exponents (mpmat.py:83-98)"""
-    L = A.L
-    a = (A.limbs.cpu().to(torch.int64) % (1 << 32)).permute(1, 2, 0)
+    """MP -> (rows of Python int mantissas, list of column exponents)"""
     R, C = A.shape
-    out = []
-    for i in range(R):
-        row = []
-        for j in range(C):
-            v = 0
-            for k in reversed(range(L)):
-                v = (v << 32) | int(a[i, j, k])
-            if v >= 1 << (32 * L - 1):
-                v -= 1 << (32 * L)
-            row.append(v)
-        out.append(row)
-    return out, A.exp.cpu().tolist()
+    words = A.limbs.permute(1, 2, 0).contiguous().cpu().numpy()
+    rows = [[int.from_bytes(words[i, j].tobytes(), "little", signed=True) for j in range(C)] for i in range(R)]
+    return rows, A.exp.cpu().tolist()
 
 
-# ------------------------------------------------------------------ re-rounding
-def _exp_vec(e, C, device=None):
-    if torch.is_tensor(e):
-        return e if e.dim() == 1 else e.expand(C)
-    return torch.full((C,), int(e), dtype=torch.int64, device=device or dev)
+# ------------------------------------------------------------------ re-rounding and additions
+def _col_exps(e, C, device):
+    """exponent argument (int, 0-d or (C,) tensor) as a (C,) tensor"""
+    if not torch.is_tensor(e):
+        return torch.full((C,), int(e), dtype=torch.int64, device=device)
+    return e.reshape(-1).expand(C) if e.numel() == 1 else e
 
 
 def rescale(A, new_exp, Lout=None):
-    """re-round A to exponent(s) new_exp (int, 0-d or (cols,) tensor), optionally to Lout limbs
-    (mpmat.py:108-117)"""
+    """A re-rounded (shift_round: half toward +inf) to the exponent(s) new_exp, into Lout limbs"""
     R, C = A.shape
     Lo = Lout or A.L
+    dev_ = A.limbs.device
     if isinstance(new_exp, int) and A.e0 is not None:
-        s = torch.full((R * C,), new_exp - A.e0, dtype=torch.int64, device=A.limbs.device)
-        return MP(shift_round(A.flat(), s, Lo).view(Lo, R, C), new_exp)
-    ne = _exp_vec(new_exp, C, A.limbs.device)
-    s = (ne - A.exp)[None, :].expand(R, C).reshape(-1)
-    return MP(shift_round(A.flat(), s, Lo).view(Lo, R, C), ne.clone())
+        shift = torch.full((R * C,), new_exp - A.e0, dtype=torch.int64, device=dev_)
+        return MP(shift_round(A.flat(), shift, Lo).view(Lo, R, C), new_exp)
+    target = _col_exps(new_exp, C, dev_)
+    shift = (target - A.exp).repeat(R)
+    return MP(shift_round(A.flat(), shift, Lo).view(Lo, R, C), target.clone())
 
 
 def bitlen_limbs(X):
     return bitlen_words(X.reshape(X.shape[0], -1))
 
 
-def normalize(A, mode="mat"):
+def _tops(A):
+    """(R, C) binary exponent of the top bit of every entry"""
     R, C = A.shape
-    top = bitlen_limbs(A.limbs).view(R, C) + A.exp[None, :]
-    t = top.max() if mode == "mat" else top.amax(0)
-    return rescale(A, t - headroom_bits(A.L))
+    return bitlen_limbs(A.limbs).view(R, C) + A.exp[None, :]
+
+
+def normalize(A, mode="mat"):
+    """re-round so the largest entry (mode 'mat') or each column's largest (mode 'col') has
+    headroom_bits(L) bits"""
+    t = _tops(A)
+    top = t.max() if mode == "mat" else t.amax(0)
+    return rescale(A, top - headroom_bits(A.L))
 
 
 def neg(A):
+    """-A, exact (two's complement wrap)"""
     R, C = A.shape
-    return MP(add_same(torch.zeros_like(A.flat()), A.flat(), -1).view(A.L, R, C),
-              A.e0 if A.e0 is not None else A.exp.clone())
+    zero = torch.zeros_like(A.flat())
+    return MP(add_same(zero, A.flat(), -1).view(A.L, R, C), A.e0 if A.e0 is not None else A.exp.clone())
 
 
 def add(A, B, alpha=1, out_exp=None):
-    """A + alpha*B (alpha = +-1). Exact when exponents agree and out_exp equals them.
-    out_exp='align': floating-point style (round both to the larger exponent per column)
-    (mpmat.py:135-156)."""
-    assert A.shape == B.shape and A.L == B.L
+    """A + alpha B (alpha = +-1).
+    out_exp int equal to both operands' common exponent: exact limb addition.
+    out_exp 'align': both operands rounded to one more than the larger exponent of each column, then
+    added (the floating-point style sum).
+    otherwise: exact sum at the smaller exponent of each column in enough limbs, then one rounding to
+    out_exp (int or tensor), or to the dynamic exponent of the matrix (None, 'mat') or of each
+    column ('col')."""
+    if A.shape != B.shape or A.L != B.L:
+        raise ValueError("operands differ in shape or limb count")
     R, C = A.shape
     L = A.L
     if isinstance(out_exp, int) and A.e0 == out_exp and B.e0 == out_exp:
         return MP(add_same(A.flat(), B.flat(), alpha).view(L, R, C), out_exp)
     if out_exp == "align":
         e = torch.maximum(A.exp, B.exp) + 1
-        a = rescale(A, e)
-        b = rescale(B, e)
-        return MP(add_same(a.flat(), b.flat(), alpha).view(L, R, C), e)
-    e = torch.minimum(A.exp, B.exp)
-    d = int(max((A.exp - e).max(), (B.exp - e).max()))
-    Lw = L + (d + 31) // 32 + 2
-    a = rescale(A, e, Lw)
-    b = rescale(B, e, Lw)
-    W = MP(add_same(a.flat(), b.flat(), alpha).view(Lw, R, C), e.clone())
-    if out_exp is None or out_exp in ("mat", "col"):
-        top = bitlen_limbs(W.limbs).view(R, C) + W.exp[None, :]
-        top = top.amax(0) if out_exp == "col" else top.max()
-        out_exp = top - headroom_bits(L)
-    return rescale(W, out_exp, L)
+        s = add_same(rescale(A, e).flat(), rescale(B, e).flat(), alpha)
+        return MP(s.view(L, R, C), e)
+    low = torch.minimum(A.exp, B.exp)
+    spread = int(torch.maximum(A.exp - low, B.exp - low).max())
+    Lw = L + 2 + (spread + 31) // 32                     # room for the larger operand and the carry
+    wide = MP(add_same(rescale(A, low, Lw).flat(), rescale(B, low, Lw).flat(), alpha).view(Lw, R, C), low.clone())
+    if out_exp is None or isinstance(out_exp, str):
+        t = _tops(wide)
+        out_exp = (t.amax(0) if out_exp == "col" else t.max()) - headroom_bits(L)
+    return rescale(wide, out_exp, L)
 
 
 # ------------------------------------------------------------------ GEMM
@@ -256,7 +257,7 @@ def gemm_nt(A, Bt, out_exp="col", kc=4096, colblock=None, bt_rowexp=None):
     fixed = isinstance(e_in, int) and isinstance(out_exp, int)
     exps = None if fixed else torch.empty((N,), dtype=torch.int64, device=dev)
     if not fixed:
-        e_in = _exp_vec(e_in, N)
+        e_in = _col_exps(e_in, N, dev)
     for j0, jw, acc in residue_gemm(R, fa0, fb0, 1, M, N, K):
         _crt_round_panel(R, acc, M, jw, j0, N, e_in, out_exp, L, out, exps, fixed)
     if fixed:
@@ -280,7 +281,7 @@ def _crt_round_panel(R, acc, M, jw, j0, N, e_in, out_exp, L, out, exps, fixed):
         sj = bl.view(M, jw).amax(0) - headroom_bits(L)
         ej = e_in[j0:j0 + jw] + sj
     else:
-        ej = _exp_vec(out_exp, N)[j0:j0 + jw]
+        ej = _col_exps(out_exp, N, dev)[j0:j0 + jw]
         sj = ej - e_in[j0:j0 + jw]
     s = sj[None, :].expand(M, jw).reshape(-1)
     exps[j0:j0 + jw] = ej

@@ -33,7 +33,7 @@ def build():
 
 def install():
     import torch
-    from sdpvibe_amd import _lib, basecase, batched, limbops, mpf, mpmat, rns, solver
+    from sdpvibe_amd import _lib, basecase, batched, limbops, mpf, mpmat, rns, sdp, solver, steplen
 
     h = ctypes.CDLL(EMU)
     for name, (args, res) in _lib.SIGNATURES.items():
@@ -48,7 +48,7 @@ def install():
         return ctypes.c_void_p(t.data_ptr())
 
     torch.cuda.current_device = lambda: 0
-    for mod in (limbops, mpf, mpmat, rns, batched, basecase, solver):
+    for mod in (limbops, mpf, mpmat, rns, batched, basecase, solver, steplen, sdp):
         if hasattr(mod, "ptr"):
             mod.ptr = emu_ptr
         if hasattr(mod, "stream"):

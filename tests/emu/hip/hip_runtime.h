@@ -29,6 +29,10 @@ typedef int hipError_t;
 #define hipSuccess 0
 #define hipFuncAttributeMaxDynamicSharedMemorySize 8
 inline hipError_t hipGetLastError() { return hipSuccess; }
+// one emulated device; a __device__ array is a host array, its "device address" its host address
+inline hipError_t hipGetDevice(int* d) { *d = 0; return hipSuccess; }
+#define HIP_SYMBOL(x) (&(x))
+inline hipError_t hipGetSymbolAddress(void** p, const void* sym) { *p = const_cast<void*>(sym); return hipSuccess; }
 inline hipError_t hipFuncSetAttribute(const void*, int, int) { return hipSuccess; }
 
 extern thread_local dim3 threadIdx, blockIdx, blockDim, gridDim;

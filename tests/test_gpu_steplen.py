@@ -88,20 +88,30 @@ def test_to_double_matches_to_float():
     assert np.array_equal(got, ref)
 
 
-@pytest.mark.parametrize("n", [1, 2, 7, 40, 97])
+@pytest.mark.parametrize("n", [1, 2, 7, 40, 97, 256, 300])
 def test_symeig_matches_numpy(n):
+    """eigenvalues within the stated bound of LAPACK's, orthonormal eigenvectors, Frobenius norm, and
+    the residual off-diagonal norm: zero-order small once converged, and part of the bound"""
     rng = np.random.default_rng(n)
     A = rng.standard_normal((3, n, n)) * np.array([1.0, 1e-5, 1e6])[:, None, None]
-    w, V, fro = S.symeig(torch.from_numpy(A).cuda(), vectors=True)
+    w, V, fro, off = S.symeig(torch.from_numpy(A).cuda(), vectors=True)
     As = (A + A.transpose(0, 2, 1)) / 2
     wr = np.linalg.eigvalsh(As)
-    w, V = w.cpu().numpy(), V.cpu().numpy()
+    w, V, off = w.cpu().numpy(), V.cpu().numpy(), off.cpu().numpy()
     for b in range(3):
-        bound = (n + 2) * 2.0 ** -52 * np.sqrt((As[b] ** 2).sum())
+        fb = np.sqrt((As[b] ** 2).sum())
+        assert off[b] <= 2.0 ** -50 * fb
+        bound = (n + 2) * 2.0 ** -52 * fb + off[b]
         assert np.abs(w[b] - wr[b]).max() <= bound
         assert np.abs(As[b] @ V[b] - V[b] * w[b][None, :]).max() <= 4 * bound
         assert np.abs(V[b].T @ V[b] - np.eye(n)).max() <= 1e-12 * n
     assert np.allclose(fro.cpu().numpy(), np.sqrt((As ** 2).sum(axis=(1, 2))), rtol=1e-14)
+
+
+def test_symeig_size_limit():
+    from sdpvibe_amd._lib import HipLibraryError
+    with pytest.raises(HipLibraryError, match="unsupported size"):
+        S.symeig(torch.zeros((1, 4097, 4097), dtype=torch.float64, device="cuda"))
 
 
 @pytest.mark.parametrize("n", [40, 144])
