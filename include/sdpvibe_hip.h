@@ -227,6 +227,23 @@ int sdpv_symeig(const double* A, int64_t nb, int64_t n, double* work, double* w,
                 double* off, void* stream);
 
 
+/* ---------------------------------------------------------------- SDPB binary BigFloat codec (host) */
+/* n consecutive El::BigFloat records of boost's binary archive (uint8 is_zero; int32 size, int64 exp,
+ * nl uint64 GMP limbs: value = sign(size) D 2^(64 (exp - |size|))) from byte pos of buf[len] into
+ * MPF entries: out (n, L) int32 entry-major limbs holding the magnitude rounded half up to at most
+ * 32L - 2 bits in two's complement, exps (n,) the binary exponents; zeros -> 0 and ez. Returns the
+ * position after the last record, or -1 for a short or corrupt buffer. Host code only.
+ * Replaces decode_bigfloats (sdpvibe/_bigfloat.c:16-59), with the same arguments. */
+int64_t sdpv_decode_bigfloats(const uint8_t* buf, int64_t len, int64_t pos, int64_t n, int32_t nl, int32_t L,
+                              int32_t* out, int64_t* exps, int64_t ez);
+
+/* Inverse of sdpv_decode_bigfloats: n MPF entries (L limbs entry-major, exponents; zero when the
+ * mantissa is 0 or the exponent is ez) -> records with nl limbs written to out[cap]; returns the bytes
+ * written, or -1 when out is too small or a value needs more than nl limbs. Host code only.
+ * Replaces encode_bigfloats (sdpvibe/_bigfloat.c:61-111), with the same arguments. */
+int64_t sdpv_encode_bigfloats(const int32_t* in, const int64_t* exps, int64_t n, int32_t L, int32_t nl,
+                              uint8_t* out, int64_t cap, int64_t ez);
+
 /* ---------------------------------------------------------------- diagnostics */
 
 /* D = A * B through one wave of v_mfma_i32_32x32x32_i8 using the same fragment mapping as the

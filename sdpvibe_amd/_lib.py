@@ -37,6 +37,8 @@ SIGNATURES = {
     "sdpv_mpf_from_double": ([_P, _I64, _I64, _P, _P, _P], _I32),
     "sdpv_symeig": ([_P, _I64, _I64, _P, _P, _P, _P, _P, _P], _I32),
     "sdpv_mfma_selftest": ([_P, _P, _P, _P], _I32),
+    "sdpv_decode_bigfloats": ([_P, _I64, _I64, _I64, _I32, _I32, _P, _P, _I64], _I64),
+    "sdpv_encode_bigfloats": ([_P, _P, _I64, _I32, _I32, _P, _I64, _I64], _I64),
 }
 ERRORS = {-1: "bad argument", -2: "unsupported size", -3: "kernel launch failed"}
 EXPECTED_VERSION = 1

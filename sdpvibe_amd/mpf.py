@@ -115,7 +115,7 @@ def _pack(ints, L):
     """Python ints -> (L, n) int32 limb planes (two's complement), via one byte buffer"""
     nbytes = 4 * L
     raw = b"".join(v.to_bytes(nbytes, "little", signed=True) for v in ints)
-    return np.frombuffer(raw, dtype=np.int32).reshape(len(ints), L)
+    return np.frombuffer(raw, dtype=np.int32).reshape(len(ints), L).copy()
 
 
 def from_ints(ints, exps, L, shape):

@@ -101,15 +101,40 @@ class SDP:
     Construct with SDP.from_blocks (host block data), SDP.synthetic (random data of given block
     shapes) or SDP.polynomial (the strictly feasible polynomial-matrix program of the benchmark)."""
 
-    def __init__(self):
-        raise TypeError("use SDP.from_blocks, SDP.synthetic or SDP.polynomial")
+    def __init__(self, path, L, workers=None, budget=None, chunk=None, stream=None, layout="batched"):
+        """the SDP stored at `path` (pmp2sdp output: directory or archive, binary or JSON blocks);
+        workers/budget/chunk/stream are accepted for the reference's signature (the whole SDP is
+        resident on the one GPU)"""
+        self.__dict__.update(SDP.from_path(path, L, layout).__dict__)
+
+    @classmethod
+    def from_path(cls, path, L, layout="batched"):
+        from .sdp_io import SdpSource, read_block, read_normalization, read_objectives
+        src = SdpSource(path)
+        nblocks = src.json("control.json")["num_blocks"]
+        const, b_strs = read_objectives(src)
+        N = len(b_strs)
+        blocks = []
+        for j in range(nblocks):
+            r = read_block(src, j, L)
+            d, K = r["dim"], r["K"]
+            P = K * d * (d + 1) // 2
+            V = [None if arr is None else F.host_to_mpf(*arr, L, (h, K))
+                 for h, arr in (r["bilinear_bases_even"], r["bilinear_bases_odd"])]
+            blocks.append(dict(dim=d, K=K, Ve=V[0], Vo=V[1], c=F.host_to_mpf(*r["c"], L, (P, 1)),
+                               B=F.host_to_mpf(*r["B"], L, (P, N))))
+        mp.mp.prec = F.NBITS(L) + 64
+        self = cls.from_blocks(blocks, F.from_strings(b_strs, L, (N, 1)), L, mp.mpf(const), layout)
+        self.normalization = read_normalization(src)
+        return self
 
     # ---------------------------------------------------------------- construction
     @classmethod
     def _empty(cls, L, N, const, layout):
         self = cls.__new__(cls)
+        mp.mp.prec = F.NBITS(L) + 64                        # the solver's scalar precision
         self.L, self.N, self.const, self.layout = L, N, mp.mpf(const), layout
-        mp.mp.prec = F.NBITS(L) + 64
+        self.normalization = None
         return self
 
     def _plan(self, shapes):

@@ -11,7 +11,7 @@ import pytest
 import torch
 
 from sdpvibe_amd import mpf as F
-from sdpvibe_amd import solver as S
+from sdpvibe_amd.blockops import shift_diagonal
 from tests.intutil import norm_closed_form
 
 pytestmark = pytest.mark.gpu
@@ -59,7 +59,7 @@ def test_mpf_baseline_check_values(case):
     G = F.from_ints(gm, ge, L, (n, n))
     bm, be = _to_mpf_ints(bcol, L)
     B = F.from_ints(bm, be, L, (n, 1))
-    A = S.add_diag(F.gemm(G, G, transB=True), F.scalar(n, L))
+    A = shift_diagonal(F.gemm(G, G, transB=True), F.scalar(n, L))
     ch = F.Chol(A)
     mp.mp.prec = 64
     lt = F.MPF(ch.Lt.limbs[:, 0, n - 1, n - 1].reshape(L, 1).contiguous(),

@@ -106,7 +106,7 @@ def test_polynomial_sdp_is_strictly_feasible():
         from sdpvibe_amd.sdp import segments
         for c, r, s in segments(g.dim):
             if c == r:
-                xs.m[:, :, s * g.K:(s + 1) * g.K] = one.m[:, :1, None]
+                xs.m[:, :, s * g.K:(s + 1) * g.K] = one.m[:, :, None, None]
                 xs.e[:, s * g.K:(s + 1) * g.K] = one.e[0]
         Xs = S.constraint_sum(g, xs)
         I = F.eye(g.n, L, 1)
