@@ -226,6 +226,15 @@ int sdpv_mpf_from_double(const double* v, int64_t n, int64_t L, int32_t* m, int6
 int sdpv_symeig(const double* A, int64_t nb, int64_t n, double* work, double* w, double* V, double* fro,
                 double* off, void* stream);
 
+/* sdpv_symeig without the work argument: the work area lives in a device array of the library,
+ * SDPV_SYMEIG_STATIC_WORK doubles (64 MiB) per GPU, held until the process ends; the call allocates
+ * nothing. No counterpart in the reference: for callers that cannot allocate device memory (code
+ * under HIP graph capture, serving loops). Sizes with 2 nb m^2 > SDPV_SYMEIG_STATIC_WORK (m = n
+ * rounded up to even) return SDPV_ESIZE. Every call on one GPU shares its array, so calls that
+ * overlap on one GPU (different streams, or host threads) race. */
+#define SDPV_SYMEIG_STATIC_WORK (8LL << 20)
+int sdpv_symeig_static(const double* A, int64_t nb, int64_t n, double* w, double* V, double* fro, double* off,
+                       void* stream);
 
 /* ---------------------------------------------------------------- SDPB binary BigFloat codec (host) */
 /* n consecutive El::BigFloat records of boost's binary archive (uint8 is_zero; int32 size, int64 exp,

@@ -36,6 +36,7 @@ SIGNATURES = {
     "sdpv_mpf_to_double": ([_P, _P, _I64, _I64, _P, _I64, _P, _P], _I32),
     "sdpv_mpf_from_double": ([_P, _I64, _I64, _P, _P, _P], _I32),
     "sdpv_symeig": ([_P, _I64, _I64, _P, _P, _P, _P, _P, _P], _I32),
+    "sdpv_symeig_static": ([_P, _I64, _I64, _P, _P, _P, _P, _P], _I32),
     "sdpv_mfma_selftest": ([_P, _P, _P, _P], _I32),
     "sdpv_decode_bigfloats": ([_P, _I64, _I64, _I64, _I32, _I32, _P, _P, _I64], _I64),
     "sdpv_encode_bigfloats": ([_P, _P, _I64, _I32, _I32, _P, _I64, _I64], _I64),

@@ -2,6 +2,8 @@
 // eigensolver (parallel cyclic Jacobi, one workgroup per matrix). The reference copies the blocks
 // to the host and calls LAPACK (sdpvibe/solver.py:996-1093); here the float64 copy never leaves
 // HBM, only the per-block minimum and the error-bound norm do.
+#include <mutex>
+
 #include "common.hpp"
 
 namespace {
@@ -213,6 +215,29 @@ __global__ void __launch_bounds__(ET) k_symeig(const double* __restrict__ A, i64
     }
 }
 
+// Device-resident workspace of sdpv_symeig_static: the Jacobi work area (2 nb m^2 doubles) of callers
+// that cannot allocate. One copy per GPU, held until the process ends; host code reaches it only
+// through the address hipGetSymbolAddress returns for the current device.
+constexpr int64_t SYMEIG_SCRATCH_DOUBLES = SDPV_SYMEIG_STATIC_WORK;
+__device__ __align__(256) double scratch[SYMEIG_SCRATCH_DOUBLES];
+
+constexpr int MAX_DEVICES = 64;
+double* scratch_of_device[MAX_DEVICES];
+std::once_flag scratch_once[MAX_DEVICES];
+hipError_t scratch_err[MAX_DEVICES];
+
+// address of the current device's scratch array (looked up once per device, with it current)
+double* device_scratch() {
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= MAX_DEVICES) return nullptr;
+    std::call_once(scratch_once[dev], [dev] {
+        void* p = nullptr;
+        scratch_err[dev] = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        scratch_of_device[dev] = scratch_err[dev] == hipSuccess ? static_cast<double*>(p) : nullptr;
+    });
+    return scratch_of_device[dev];
+}
+
 }  // namespace
 
 extern "C" {
@@ -246,6 +271,23 @@ int sdpv_symeig(const double* A, int64_t nb, int64_t n, double* work, double* w,
     if (!A || !work || !w) return SDPV_EARG;
     const i64 h = (n + (n & 1)) / 2;
     const size_t lds = (size_t)h * (2 * sizeof(double) + 2 * sizeof(int));
+    hipLaunchKernelGGL(k_symeig, dim3((unsigned)nb), dim3(ET), lds, as_stream(stream), A, nb, (int)n, work, w, V, fro,
+                       off);
+    SDPV_TRY_LAUNCH();
+    return SDPV_OK;
+}
+
+int sdpv_symeig_static(const double* A, int64_t nb, int64_t n, double* w, double* V, double* fro, double* off,
+                       void* stream) {
+    if (nb < 0 || n < 1) return SDPV_EARG;
+    if (n > SDPV_SYMEIG_NMAX || nb > 0x7fffffffLL) return SDPV_ESIZE;
+    const int64_t m = n + (n & 1);
+    if (nb > 0 && 2 * m * m > SYMEIG_SCRATCH_DOUBLES / nb) return SDPV_ESIZE;   // 2 nb m^2 doubles
+    if (nb == 0) return SDPV_OK;
+    if (!A || !w) return SDPV_EARG;
+    double* work = device_scratch();
+    if (!work) return SDPV_ELAUNCH;
+    const size_t lds = (size_t)(m / 2) * (2 * sizeof(double) + 2 * sizeof(int));
     hipLaunchKernelGGL(k_symeig, dim3((unsigned)nb), dim3(ET), lds, as_stream(stream), A, nb, (int)n, work, w, V, fro,
                        off);
     SDPV_TRY_LAUNCH();
