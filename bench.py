@@ -1,17 +1,22 @@
-"""Benchmark: exact multiprecision GEMM (mpf.gemm) 512 x 512 x 512 at 768 bits (L = 24 limbs) on one
-MI355X -- the product of the plain-kernel configuration (Gaussian mantissas, exponents in
-[-16, 16], seed 0). Prints one JSON line.
+"""Benchmark on one MI355X. Prints one JSON line.
 
     python bench.py [--steps K] [--warmup W] [--gpus 1]
 
-value      : effective INT8 operations of the whole mp-GEMM per second, as % of the MI355X dense
-             INT8 MFMA peak, counted with the reference algorithm's operation count for the same
-             product (four INT8 products per prime, n = 112 primes, E = 216 CRT digits at L = 24:
-             2 [4 n M N K + 2 n 4L (M + N) K + 2 E n M N]); the count of the work this build
-             actually executes (three products per prime, E = 200) is reported beside it
-roofline   : the dominant kernel of one step, timed with HIP events on its launch stream
+step       : one Mehrotra predictor-corrector iteration of the interior-point solver (sdpvibe_amd.ipm)
+             on the synthetic polynomial-matrix SDP of the IPM configuration: 300 dimension-1 and 60
+             dimension-3 positivity constraints of degree 26 (27 rescaled Laguerre points, orthonormal
+             bilinear bases), N = 3000 free variables, 768 bits (L = 24), strictly feasible by
+             construction (sdp.SDP.polynomial, seeded host PCG64 data). W iterations run untimed, then
+             K iterations are timed, each between two device synchronisations.
+value      : seconds per IPM iteration of the batched solver (lower is better)
+plain      : the same SDP in the per-block layout (one launch sequence per block), timed over the
+             same iterations; its records must be bitwise those of the batched run
+mp_gemm    : the exact multiprecision GEMM of the plain-kernel configuration (512 x 512 x 512 at
+             L = 24) as % of the dense INT8 MFMA peak, counted with the reference algorithm's
+             operation count (2 [4 n M N K + 2 n 4L (M + N) K + 2 E n M N], n = 112, E = 216)
+roofline   : the product kernel of that GEMM, timed with HIP events on its launch stream
 cpu_baseline: the reference's own CPU path (cpu_baseline/mpf_baseline.c, GMP mpf naive loops,
-             compiled from source into oracle/_ref) on the same 512 x 512 x 512 768-bit shape
+             compiled from source into oracle/_ref) at 512 / 768 bits on this process's CPU share
 """
 import argparse
 import json
@@ -50,8 +55,11 @@ def _cpu_model():
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=50)
-    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--steps", type=int, default=8, help="timed IPM iterations")
+    ap.add_argument("--warmup", type=int, default=2, help="untimed IPM iterations before them")
+    ap.add_argument("--gemm-steps", type=int, default=50)
+    ap.add_argument("--plain-iters", type=int, default=2, help="iterations timed in the per-block layout")
+    ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--n", type=int, default=512)
     ap.add_argument("--L", type=int, default=24)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -213,18 +221,55 @@ def plain_chol(n, L, reps=3):
             "chol_mp_mac_per_s": n ** 3 / 6 / tc, "trsm_mp_mac_per_s": n ** 3 / ts}
 
 
-def main():
-    args = parse()
-    if args.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (no multi-GPU path "
-              f"in this build)", file=sys.stderr)
-        sys.exit(2)
+def ipm_leg(args, L=24):
+    """seconds per IPM iteration, batched and per-block, on the polynomial SDP (see the docstring)"""
+    import torch
+    from sdpvibe_amd.ipm import Solver
+    from sdpvibe_amd.sdp import SDP
+
+    t0 = time.perf_counter()
+    sdp = SDP.polynomial(n1=300, n3=60, degree=26, N=3000, L=L, seed=0)
+    torch.cuda.synchronize()
+    setup = time.perf_counter() - t0
+
+    def timed(s, iters):
+        solver = Solver(s, L)
+        gen = solver.iterate(max_iter=iters)
+        times = []
+        for _ in range(iters):
+            torch.cuda.synchronize()
+            t = time.perf_counter()
+            next(gen)
+            torch.cuda.synchronize()
+            times.append(time.perf_counter() - t)
+        return solver, times
+
+    W, K = args.warmup, args.steps
+    solver, times = timed(sdp, W + K)
+    out = {"s_per_iteration": sum(times[W:]) / K, "iteration_s": [round(t, 4) for t in times],
+           "setup_s": round(setup, 2), "groups": len(sdp.groups), "total_psd_rows": sdp.total_psd_rows,
+           "last_record": {k: solver.log[-1][k][:12] for k in ("mu", "P-obj", "D-obj", "P-step", "D-step")}}
+    if not args.no_plain and args.plain_iters > 0:
+        P = args.plain_iters
+        plain = sdp.relayout("plain")
+        psolver, ptimes = timed(plain, 1 + P)                # iteration 1 untimed in both forms
+        same = all(ra[k] == rb[k] for ra, rb in zip(solver.log, psolver.log)
+                   for k in ("mu", "P-obj", "D-obj", "P-err", "p-err", "D-err", "R-err", "P-step", "D-step", "beta"))
+        bt = sum(times[1:1 + P]) / P
+        pt = sum(ptimes[1:]) / P
+        out["plain"] = {"groups": len(plain.groups), "s_per_iteration": round(pt, 4),
+                        "batched_s_per_iteration_same_iterations": round(bt, 4), "speedup": round(pt / bt, 2),
+                        "iterations": f"2..{1 + P}", "records_bitwise_equal": bool(same)}
+    return out
+
+
+def gemm_leg(args):
+    """the 512^3 exact GEMM at L = 24: % of INT8 peak, HIP-graph replay, per-phase times, roofline"""
     import torch
     from sdpvibe_amd import mpf as F
     from sdpvibe_amd import mpmat
     from sdpvibe_amd.synthetic import gaussian_mpf
 
-    torch.cuda.set_device(0)
     n, L = args.n, args.L
     M = N = K = n
     Am, Ae = gaussian_mpf((1, n, n), L, seed=0)
@@ -232,27 +277,28 @@ def main():
     A = F.MPF(torch.from_numpy(Am).cuda(), torch.from_numpy(Ae).cuda())
     B = F.MPF(torch.from_numpy(Bm).cuda(), torch.from_numpy(Be).cuda())
     R = mpmat.ctx(L)
-    for _ in range(args.warmup):
+    steps = args.gemm_steps
+    for _ in range(5):
         F.gemm(A, B)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    for _ in range(args.steps):
+    for _ in range(steps):
         C = F.gemm(A, B)
     torch.cuda.synchronize()
-    dt = (time.perf_counter() - t0) / args.steps
+    dt = (time.perf_counter() - t0) / steps
     del C
 
     # ---- the same step replayed from a HIP graph (one capture, no host work per step)
     from sdpvibe_amd.graph import Captured
     cap = Captured(F.gemm, A, B)
-    for _ in range(args.warmup):
+    for _ in range(5):
         cap.replay()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    for _ in range(args.steps):
+    for _ in range(steps):
         cap.replay()
     torch.cuda.synchronize()
-    dt_graph = (time.perf_counter() - t0) / args.steps
+    dt_graph = (time.perf_counter() - t0) / steps
     del cap
 
     # ---- per-phase times with HIP events on the launch stream (torch's current stream): every
@@ -280,7 +326,7 @@ def main():
         "crt": lambda: R.crt(acc, M, N, 0, N, 1, outm=om, oute=oe, L=L, plane=M * N, ldo=N, jo=0, Mb=M,
                              erow=EA, ecol=EB, scaled=True),
     }
-    reps = max(10, args.steps)
+    reps = max(10, steps)
     acc_ms = {}
     for p in phases:
         run[p]()
@@ -328,40 +374,48 @@ def main():
         roof["config"] = cfg
     roof["traffic"] = measured_traffic(roof["kernel"], cfg if cfg is not None else -1, n, L)
 
-    cpu = None if args.no_cpu_baseline else cpu_baseline(n, args.cpu_threads, L)
-    gram = None
-    if not args.no_gram:
-        try:                                              # auxiliary measurement: never loses the line
-            gram = gram_acc(L)
-        except Exception as exc:                          # noqa: BLE001
-            gram = {"error": f"{type(exc).__name__}: {exc}"}
-    pchol = None
-    try:                                                  # auxiliary measurement: never loses the line
-        pchol = plain_chol(n, L)
+    return {"workload": f"mpf.gemm {M}x{K}x{N} at {32 * L}-bit (L={L}), plain-kernel configuration",
+            "pct_int8_peak": round(value, 3), "ms_per_gemm": round(dt * 1e3, 4),
+            "ms_per_gemm_graph": round(dt_graph * 1e3, 4),
+            "pct_peak_graph": round(100.0 * ref_ops / dt_graph / (PEAK_INT8_TOPS * 1e12), 3),
+            "mp_mac_per_s": M * N * K / dt, "effective_int8_ops": ref_ops, "executed_int8_ops": ops_total,
+            "pct_peak_executed_ops": round(100.0 * ops_total / dt / (PEAK_INT8_TOPS * 1e12), 3),
+            "primes": nP, "crt_digits": E, "phase_ms": {p: round(v, 4) for p, v in ph.items()},
+            "phase_hbm_frac": {p: round(hbm[p] / (ph[p] * 1e-3) / 1e9 / PEAK_HBM_GBS, 4) for p in hbm}}, roof
+
+
+def _guard(f, *a):
+    """an auxiliary measurement never loses the line"""
+    try:
+        return f(*a)
     except Exception as exc:                              # noqa: BLE001
-        pchol = {"error": f"{type(exc).__name__}: {exc}"}
-    bchol = None
-    if not args.no_batched:
-        try:                                              # auxiliary measurement: never loses the line
-            bchol = batched_chol(L, args.blocks)
-        except Exception as exc:                          # noqa: BLE001
-            bchol = {"error": f"{type(exc).__name__}: {exc}"}
+        return {"error": f"{type(exc).__name__}: {exc}"}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (this build runs on one GPU; "
+              f"the path would shard by prime and by block, see the design notes)", file=sys.stderr)
+        sys.exit(2)
+    import torch
+    torch.cuda.set_device(0)
+    L = args.L
+    ipm = ipm_leg(args, L)
+    gemm, roof = gemm_leg(args)
+    cpu = None if args.no_cpu_baseline else cpu_baseline(args.n, args.cpu_threads, L)
     line = {
-        "metric": METRIC, "value": round(value, 3),
-        "unit": f"% of dense INT8 MFMA peak (mp-GEMM {M}x{K}x{N} at L={L}; s/IPM iteration not measured here)",
-        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": dt * 1e3,
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "int8 (exact 768-bit)",
-        "data": "synthetic (host PCG64 Gaussian mantissas, exponents in [-16, 16], seed 0/1)",
-        "config": {"workload": f"mpf.gemm {M}x{K}x{N} at {32 * L}-bit (L={L}), plain-kernel configuration",
-                   "L": L, "primes": nP, "crt_digits": E},
-        "ms_per_step_graph": dt_graph * 1e3,
-        "pct_peak_graph": round(100.0 * ref_ops / dt_graph / (PEAK_INT8_TOPS * 1e12), 3),
-        "mp_mac_per_s": M * N * K / dt, "effective_int8_ops_per_step": ref_ops,
-        "executed_int8_ops_per_step": ops_total,
-        "pct_peak_executed_ops": round(100.0 * ops_total / dt / (PEAK_INT8_TOPS * 1e12), 3),
-        "phase_ms": {p: round(v, 4) for p, v in ph.items()},
-        "phase_hbm_frac": {p: round(hbm[p] / (ph[p] * 1e-3) / 1e9 / PEAK_HBM_GBS, 4) for p in hbm},
-        "roofline": roof, "cpu_baseline": cpu, "plain_chol": pchol, "batched_chol": bchol, "gram_acc": gram,
+        "metric": METRIC, "value": round(ipm["s_per_iteration"], 4), "unit": "s/IPM iteration",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": ipm["s_per_iteration"] * 1e3,
+        "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "int8 (exact 768-bit)",
+        "data": "synthetic (strictly feasible polynomial-matrix SDP from host PCG64 data, seed 0)",
+        "config": {"workload": "Mehrotra predictor-corrector IPM iteration, polynomial-matrix SDP: 300 dim-1 + 60 "
+                               "dim-3 positivity constraints, degree 26, N=3000, 768-bit (L=24), batched layout",
+                   "L": L, "N": 3000, "blocks": 360},
+        "ipm": ipm, "mp_gemm": gemm, "roofline": roof, "cpu_baseline": cpu,
+        "plain_chol": _guard(plain_chol, args.n, L),
+        "batched_chol": None if args.no_batched else _guard(batched_chol, L, args.blocks),
+        "gram_acc": None if args.no_gram else _guard(gram_acc, L),
     }
     print(json.dumps(line))
 

@@ -140,6 +140,7 @@ class Solver:
         self._update = None                                   # (alphaP, alphaD, directions) pending
         self.log = []
         self.final_factors = None
+        self.timers = Timers()
 
     # ---------------------------------------------------------------- block operators
     def constraint_sum(self, g, a):
@@ -241,9 +242,12 @@ class Solver:
     def _schur_and_q(self, work):
         """Chol(S) per group, P = L_S^-1 B, Q = sum P^T P (exact, residues), Chol(Q)"""
         sdp, L = self.sdp, self.L
+        TM = self.timers
         best, cond = -1.0, (mp.mpf(0), "")
         for g, w in zip(sdp.groups, work):
+            TM("schur: assemble S")
             S = self.schur(g, w.pop("AXI"), w.pop("AY"))
+            TM("schur: chol(S)")
             cS = Chol(S)
             del S
             lg = chol_log2_cond(cS).tolist()
@@ -252,7 +256,9 @@ class Solver:
                 best = lg[q]
                 cond = (mp.mpf(2) ** mp.mpf(lg[q]), f"schur_complement_cholesky.block_{g.idx[q]}")
             w["cS"] = cS
+            TM("schur: P = L^-1 B")
             w["P"] = cS.solve_lower(g.B)
+        TM("Q: accumulate")
         tops = None
         for w in work:
             t = F._top(stack_rows(w["P"])).amax(0)
@@ -261,6 +267,7 @@ class Solver:
         for w in work:
             acc.add(stack_rows(w["P"]))
         Q = mirror_lower(acc.result())
+        TM("chol(Q)")
         cQ = Chol(Q)
         qc = mp.mpf(2) ** mp.mpf(float(chol_log2_cond(cQ)[0]))
         return cQ, qc, cond

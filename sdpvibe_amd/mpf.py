@@ -398,6 +398,19 @@ def fx_exp(L):
     return -(32 * L - 40)
 
 
+_ONES = {}
+
+
+def _fixed_one(L, device):
+    """limbs of 1 at the fixed-point exponent fx_exp(L), cached per (L, device) so that later calls
+    copy nothing from the host (and can be captured in a HIP graph)"""
+    key = (L, str(device))
+    if key not in _ONES:
+        words = (1 << (32 * L - 40)).to_bytes(4 * L, "little", signed=True)
+        _ONES[key] = torch.frombuffer(bytearray(words), dtype=torch.int32).to(device)
+    return _ONES[key]
+
+
 class Chol:
     """A = L L^T for one (n, n) or a batch (nb, n, n) of SPD matrices, with L = D^-1 Lt, D = diag(2^-k):
     Lt fixed point at fx_exp(L), k one integer per row (the reference's mpf.Chol, sdpvibe/mpf.py:461-532:
@@ -437,8 +450,7 @@ class Chol:
         P = torch.zeros((L, nb, npad, npad), dtype=torch.int32, device=dv)
         P[:, :, :n, :n] = shift_round(A.m.reshape(L, -1), shift.reshape(-1), L).view(L, nb, n, n)
         # identity on the padding rows: beyond n, and beyond each block's own dimension
-        ones = (1 << -efx).to_bytes(4 * L, "little", signed=True)
-        one = torch.frombuffer(bytearray(ones), dtype=torch.int32).to(dv)
+        one = _fixed_one(L, dv)
         pad = torch.ones((nb, npad), dtype=torch.bool, device=dv)
         pad[:, :n] = ~live
         diag = P.diagonal(dim1=-2, dim2=-1)                           # (L, nb, npad) view

@@ -88,7 +88,7 @@ def from_ints(rows, exp, L):
         raise OverflowError("value does not fit in L limbs")
     raw = b"".join(v.to_bytes(4 * L, "little", signed=True) for v in vals)
     words = np.frombuffer(raw, dtype=np.int32).reshape(R, C, L).transpose(2, 0, 1)
-    return MP(torch.from_numpy(np.ascontiguousarray(words)).to(dev), exp)
+    return MP(torch.from_numpy(np.array(words)).to(dev), exp)
 
 
 def to_ints(A):
