@@ -235,6 +235,162 @@ __device__ void wave_rsqrt(const Pivot& P) {
     }
 }
 
+// ---------------------------------------------------------------- lane-parallel pivot arithmetic
+// For WL <= 64 the pivot chain runs on wave 0 with one word per lane (word k in lane k, lanes >= WL
+// hold 0): magnitudes and additions use a carry lookahead over ballots, products sum their
+// columns on the lanes and normalise the column sums with shuffles and one more lookahead. The
+// results are word for word those of wave_mul / wave_rsqrt above (the same truncated product).
+__device__ __forceinline__ int lane_id() { return threadIdx.x & 63; }
+__device__ __forceinline__ u32 lane_get(u32 v, int src) { return (u32)__shfl((int)v, src, 64); }
+__device__ __forceinline__ u64 lane_get64(u64 v, int src) {
+    return ((u64)lane_get((u32)(v >> 32), src) << 32) | lane_get((u32)v, src);
+}
+// value of the previous lane (0 in lane 0)
+__device__ __forceinline__ u64 from_below64(u64 v) {
+    const int l = lane_id();
+    const u64 r = lane_get64(v, l > 0 ? l - 1 : 0);
+    return l > 0 ? r : 0;
+}
+
+// top word index of the number (>= 1), else 0 (topl_slot semantics)
+__device__ __forceinline__ int lane_top(u32 w, int WL) {
+    const u64 nz = __ballot(lane_id() < WL && w != 0);
+    return nz ? 63 - __builtin_clzll(nz) : 0;
+}
+
+// two's complement negation over WL words: ~w + 1, the +1 absorbed by the lowest nonzero word
+__device__ u32 lane_neg(u32 w, int WL) {
+    const int l = lane_id();
+    const u64 nz = __ballot(l < WL && w != 0);
+    const int first = nz ? __builtin_ctzll(nz) : 64;
+    if (l >= WL) return 0u;
+    return l < first ? 0u : (l == first ? ~w + 1u : ~w);
+}
+
+// a + b + cin mod 2^(32 WL): generate / propagate masks from two ballots, the carry into every word
+// from one 64-bit addition ((G|P) + G + cin) ^ (G|P) ^ G
+__device__ u32 lane_add(u32 a, u32 b, unsigned cin, int WL) {
+    const int l = lane_id();
+    const u64 s = (u64)a + b;
+    const u64 G = __ballot(l < WL && (s >> 32) != 0);
+    const u64 P = __ballot(l < WL && (u32)s == 0xffffffffu);
+    const u64 X = G | P;
+    const u64 carries = (X + G + cin) ^ X ^ G;
+    return l < WL ? (u32)s + (u32)((carries >> l) & 1u) : 0u;
+}
+
+// mulfx(a, b, t) of two's complement lane numbers; xs, ys: WL words of LDS for the magnitudes
+__device__ u32 lane_mulfx(u32 a, u32 b, int t, int WL, int LF, u32* xs, u32* ys) {
+    const int l = lane_id();
+    const bool sa = (lane_get(a, WL - 1) >> 31) != 0, sb = (lane_get(b, WL - 1) >> 31) != 0;
+    const u32 ma = sa ? lane_neg(a, WL) : a, mb = sb ? lane_neg(b, WL) : b;
+    const int ta = lane_top(ma, WL), tb = lane_top(mb, WL);
+    __builtin_amdgcn_wave_barrier();                     // earlier readers of xs / ys are done
+    if (l < WL) {
+        xs[l] = ma;
+        ys[l] = mb;
+    }
+    __builtin_amdgcn_wave_barrier();
+    const int la = t ? max(0, ta - t + 1) : 0, lb = t ? max(0, tb - t + 1) : 0;
+    const int kmin = max(LF - 2, la + lb), kmax = ta + tb;
+    // column sums of columns kmin + l (h = 0) and kmin + 64 + l (h = 1) as lo / hi 32-bit halves
+    u64 lo[2], hi[2];
+    for (int h = 0; h < 2; h++) {
+        const int k = kmin + l + 64 * h;
+        u64 cl = 0, ch = 0;
+        if (k <= kmax) {
+            const int i0 = max(la, k - tb), i1 = min(ta, k - lb);
+            for (int i = i0; i <= i1; i++) {
+                const u64 p = (u64)xs[i] * ys[k - i];
+                cl += (u32)p;
+                ch += p >> 32;
+            }
+        }
+        lo[h] = cl;
+        hi[h] = ch;
+    }
+    // V = sum_q (lo_q + hi_{q-1}) 2^(32 q), q = k - kmin over 128 positions (two per lane)
+    // (every lane takes part in every shuffle; lane 0 of the upper half takes lane 63 of the lower)
+    auto below = [&](u64 v0, u64 v1, u64& b0, u64& b1) {
+        const u64 top = lane_get64(v0, 63), p0 = from_below64(v0), p1 = from_below64(v1);
+        b0 = p0;
+        b1 = l == 0 ? top : p1;
+    };
+    u64 b0, b1;
+    below(hi[0], hi[1], b0, b1);
+    const u64 s0 = lo[0] + b0, s1 = lo[1] + b1;
+    // digits below 2^32 + 2^7: d_q = (s_q mod 2^32) + (s_{q-1} >> 32)
+    below(s0 >> 32, s1 >> 32, b0, b1);
+    const u64 d0 = (s0 & 0xffffffffull) + b0, d1 = (s1 & 0xffffffffull) + b1;
+    // final digits: (d_q mod 2^32) + (d_{q-1} >> 32) + carry, the carries from a 128-bit lookahead
+    below(d0 >> 32, d1 >> 32, b0, b1);
+    const u64 e0 = (d0 & 0xffffffffull) + b0, e1 = (d1 & 0xffffffffull) + b1;
+    const u64 G0 = __ballot((e0 >> 32) != 0), P0 = __ballot((u32)e0 == 0xffffffffu);
+    const u64 G1 = __ballot((e1 >> 32) != 0), P1 = __ballot((u32)e1 == 0xffffffffu);
+    const u64 X0 = G0 | P0, X1 = G1 | P1;
+    const u64 sum0 = X0 + G0;
+    const u64 C0 = sum0 ^ X0 ^ G0;
+    const u64 C1 = (X1 + G1 + (sum0 < X0 ? 1u : 0u)) ^ X1 ^ G1;
+    const u32 dig0 = (u32)e0 + (u32)((C0 >> l) & 1u), dig1 = (u32)e1 + (u32)((C1 >> l) & 1u);
+    // output word j = digit of column LF + j
+    const int q = LF + l - kmin;
+    const u32 w0 = lane_get(dig0, q & 63), w1 = lane_get(dig1, q & 63);
+    u32 w = (l >= WL || q < 0 || q > kmax + 2 - kmin) ? 0u : (q < 64 ? w0 : w1);
+    __builtin_amdgcn_wave_barrier();
+    return (sa ^ sb) ? lane_neg(w, WL) : w;
+}
+
+// basecase.py:59-68 on a lane number, the same value in every lane
+__device__ double lane_to_double(u32 a, int WL, int LF) {
+    const bool s = (lane_get(a, WL - 1) >> 31) != 0;
+    const u32 m = s ? lane_neg(a, WL) : a;
+    const int t = lane_top(m, WL);
+    const u32 m0 = lane_get(m, t), m1 = lane_get(m, t >= 1 ? t - 1 : 0), m2 = lane_get(m, t >= 2 ? t - 2 : 0);
+    double v = (double)m0;
+    if (t >= 1) v = v * 4294967296.0 + (double)m1;
+    if (t >= 2) v = v * 4294967296.0 + (double)m2;
+    v = ldexp(v, 32 * (t - 2 > 0 ? t - 2 : 0) - 32 * LF);
+    return s ? -v : v;
+}
+
+// basecase.py:69-78: the lane's word of the fixed-point value of v (0 for non-positive v)
+__device__ u32 lane_from_double(double v, int WL, int LF) {
+    const int l = lane_id();
+    if (!(v > 0.0) || !isfinite(v) || l >= WL) return 0u;
+    int e;
+    const double m = frexp(v, &e);
+    u64 mi = (u64)ldexp(m, 53);
+    int pos = e - 53 + 32 * LF;
+    if (pos < 0) {
+        mi = (-pos < 64) ? (mi >> (-pos)) : 0;
+        pos = 0;
+    }
+    const int li = pos / 32, sh = pos % 32;
+    const u64 lo = mi << sh;
+    const u64 hi = sh ? (mi >> (64 - sh)) : 0;
+    const int k = l - li;
+    return k == 0 ? (u32)lo : k == 1 ? (u32)(lo >> 32) : k == 2 ? (u32)hi : k == 3 ? (u32)(hi >> 32) : 0u;
+}
+
+// d^-1/2 (basecase.py:145-156): double seed, Newton r <- r + r (1 - d r^2) / 2, precision doubling
+__device__ u32 lane_rsqrt(u32 d, int WL, int LF, u32* xs, u32* ys) {
+    const int l = lane_id();
+    u32 r = lane_from_double(1.0 / sqrt(lane_to_double(d, WL, LF)), WL, LF);
+    for (int bits = 100;; bits *= 2) {
+        int t = bits / 32 + 2;
+        if (t >= WL) t = 0;
+        u32 t1 = lane_mulfx(r, r, t, WL, LF, xs, ys);
+        u32 t2 = lane_mulfx(t1, d, t, WL, LF, xs, ys);
+        t1 = lane_add(l == LF ? 1u : 0u, ~t2, 1u, WL);            // one - t2
+        t2 = lane_mulfx(r, t1, t, WL, LF, xs, ys);
+        const u32 above = lane_get(t2, l < 63 ? l + 1 : 63);      // word l + 1
+        const u32 half = (l < WL - 1) ? ((t2 >> 1) | (above << 31)) : (u32)(((int)t2) >> 1);
+        r = lane_add(r, half, 0u, WL);                            // r += t2 >> 1 (arithmetic)
+        if (t == 0) break;
+    }
+    return r;
+}
+
 // ---------------------------------------------------------------- the tile kernel
 // W, X, S: AoS (nb, BT, BT, WL) words in global memory (W: working copy, factored in place;
 // X: L^-1; S: products of the inverse). LDS: 2 operand slots per thread + pivot area + RINV.
@@ -263,16 +419,30 @@ __global__ void __launch_bounds__(TT) k_tile_factor(u32* __restrict__ Wg, u32* _
 
     for (int k = 0; k < BT; k++) {
         // pivot: d = A_kk, r = d^-1/2, A_kk = d r, RINV_k = r
-        for (int q = tid; q < WL; q += TT) P.d[q] = ent(W, k, k)[q];
-        __syncthreads();
-        wave_rsqrt(P);
-        wave_mul(P, P.t1, P.d, P.r, 0);
-        for (int q = tid; q < WL; q += TT) {
-            ent(W, k, k)[q] = P.t1[q];
-            rinv[k * WL + q] = P.r[q];
-            Rg[(tile * BT + k) * WL + q] = P.r[q];
+        if (WL <= 64) {                                   // one word per lane on wave 0
+            if (tid < 64) {
+                const u32 d = tid < WL ? ent(W, k, k)[tid] : 0u;
+                const u32 r = lane_rsqrt(d, WL, LF, P.sx, P.sy);
+                const u32 dr = lane_mulfx(d, r, 0, WL, LF, P.sx, P.sy);
+                if (tid < WL) {
+                    ent(W, k, k)[tid] = dr;
+                    rinv[k * WL + tid] = r;
+                    Rg[(tile * BT + k) * WL + tid] = r;
+                }
+            }
+            __syncthreads();
+        } else {
+            for (int q = tid; q < WL; q += TT) P.d[q] = ent(W, k, k)[q];
+            __syncthreads();
+            wave_rsqrt(P);
+            wave_mul(P, P.t1, P.d, P.r, 0);
+            for (int q = tid; q < WL; q += TT) {
+                ent(W, k, k)[q] = P.t1[q];
+                rinv[k * WL + q] = P.r[q];
+                Rg[(tile * BT + k) * WL + q] = P.r[q];
+            }
+            __syncthreads();
         }
-        __syncthreads();
         // column scale: A_ik = mulfx(A_ik, r)
         for (int i = k + 1 + tid; i < BT; i += TT) {
             u32* aik = ent(W, i, k);

@@ -47,6 +47,16 @@ long long emu_shfl_xor64(long long v, int mask) {
     return r;
 }
 
+long long emu_shfl_idx64(long long v, int src) {
+    Wave& w = my_wave();
+    int lane = threadIdx.x & 63;
+    w.x[lane] = v;
+    w.bar->arrive_and_wait();
+    long long r = w.x[src & 63];
+    w.bar->arrive_and_wait();
+    return r;
+}
+
 // D = A B + C with the fragment mapping the kernels assume (verified on the GPU by
 // sdpv_mfma_selftest): A row = lane%32 over bytes of lanes (row, row+32); B column = lane%32.
 emu_v16i emu_mfma_i32_32x32x32_i8(emu_v4i a, emu_v4i b, emu_v16i c, int, int, int) {

@@ -54,6 +54,8 @@ inline long long atomicMax(long long* p, long long v) {
     return cur;
 }
 long long emu_shfl_xor64(long long v, int mask);
+long long emu_shfl_idx64(long long v, int src);
+inline int __shfl(int v, int src, int width = 64) { (void)width; return (int)emu_shfl_idx64(v, src); }
 inline long long __shfl_xor(long long v, int mask, int width = 64) { (void)width; return emu_shfl_xor64(v, mask); }
 inline long __shfl_xor(long v, int mask, int width = 64) { (void)width; return (long)emu_shfl_xor64(v, mask); }
 inline int __shfl_xor(int v, int mask, int width = 64) { (void)width; return (int)emu_shfl_xor64(v, mask); }

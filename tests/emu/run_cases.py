@@ -125,6 +125,9 @@ def tile_case(L, nb, BT, logc, seed):
 
 
 CASES["tile"] = lambda: tile_case(2, 2, 32, 6, 11) and tile_case(4, 1, 20, 12, 3)
+# the lane-parallel pivot path up to its limit (WL = L + 24 = 64) and the wave path just beyond it
+CASES["tile_wide"] = lambda: (tile_case(24, 1, 32, 20, 5) and tile_case(40, 1, 12, 8, 7)
+                              and tile_case(41, 1, 8, 4, 9))
 
 
 def chol_cases():
