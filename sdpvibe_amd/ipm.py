@@ -141,6 +141,7 @@ class Solver:
         self.log = []
         self.final_factors = None
         self.timers = Timers()
+        self.iterations_done, self.alphaP, self.alphaD = 0, mp.mpf(0), mp.mpf(0)
 
     # ---------------------------------------------------------------- block operators
     def constraint_sum(self, g, a):
@@ -334,10 +335,10 @@ class Solver:
         p, sdp, L = self.p, self.sdp, self.L
         mp.mp.prec = F.NBITS(L) + 64
         fmt = lambda v: fmt_big(v, self.digits)
-        alphaP = alphaD = mp.mpf(0)
+        alphaP, alphaD = self.alphaP, self.alphaD              # the last steps (0 before the first)
         t0 = time.time()
         last_ck = t0
-        it = 0
+        it = self.iterations_done                                # > 0 when resumed from a checkpoint
         pobj = dobj = gap = primal_err = dual_err = mp.mpf(0)
         self.timers = TM = Timers()
 
@@ -411,6 +412,7 @@ class Solver:
             TM("update")
             aP, aD = self.S(alphaP), self.S(alphaD)
             self._update = (aP, aD, dirs)
+            self.iterations_done, self.alphaP, self.alphaD = it, alphaP, alphaD
             self.y = add(self.y, mul(dy, aD))
             self.final_factors = dict(X=[w["cX"] for w in work], Y=[w["cY"] for w in work],
                                       S=[w["cS"] for w in work], Q=cQ)

@@ -25,12 +25,12 @@ def _host(X):
     return X.m.reshape(L, -1).t().contiguous().cpu().numpy(), X.e.reshape(-1).cpu().numpy()
 
 
-def test_binary_sdp_round_trip(tmp_path):
+def write_golden_sdp(path, normalization=None):
+    """the golden polynomial SDP written in pmp2sdp's binary format at `path`; -> (L, blocks, b)"""
     g = C.golden()
     L = g["L"]
     with mp.workprec(1700):
         blocks, b = C.golden_blocks(g)
-    mem = gpu_sdp(blocks, b, L)
     host_blocks = []
     for bl in blocks:
         K = bl["K"]
@@ -43,7 +43,13 @@ def test_binary_sdp_round_trip(tmp_path):
         host_blocks.append(ent)
     with mp.workprec(32 * L + 64):
         b_strs = [mp.nstr(v, 240, strip_zeros=False) for v in b]
-    IO.write_sdp(str(tmp_path / "sdp"), host_blocks, b_strs, 32 * L, L)
+    IO.write_sdp(str(path), host_blocks, b_strs, 32 * L, L, normalization=normalization)
+    return L, blocks, b
+
+
+def test_binary_sdp_round_trip(tmp_path):
+    L, blocks, b = write_golden_sdp(tmp_path / "sdp")
+    mem = gpu_sdp(blocks, b, L)
     disk = SDP(str(tmp_path / "sdp"), L)
     assert torch.equal(disk.b.m, mem.b.m) and torch.equal(disk.b.e, mem.b.e)
     A, B = Solver(mem, L), Solver(disk, L)
