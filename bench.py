@@ -182,10 +182,18 @@ def cpu_baseline(n, threads, L):
         return None
     sec = {k: float(m.group(1)) for k, m in rates.items()}
     rate = {k: float(m.group(2)) * 1e6 for k, m in rates.items()}
+    # one IPM iteration of the N = 3000 polynomial SDP at these rates, counting only its three dominant
+    # contractions (a lower bound): Q += P^T P over sum P_j = 17820 rows (lower triangle), P = L_S^-1 B
+    # per block (P_j^2 N / 2; 300 blocks of P = 27, 60 of P = 162) and chol(Q) (N^3 / 6)
+    N, rows = 3000, 300 * 27 + 60 * 162
+    work = {"gemm": rows * N * (N + 1) / 2, "trsm": (300 * 27 ** 2 + 60 * 162 ** 2) * N / 2, "cholesky": N ** 3 / 6}
+    ipm_est = sum(work[k] / rate[k] for k in work)
     return {"value": rate["gemm"], "unit": "mp-MAC/s (768-bit GEMM)", "cores": threads, "kind": "reference",
             "nproc": os.cpu_count(), "cpu_model": _cpu_model(),
             "cholesky_mp_mac_per_s": rate["cholesky"], "trsm_mp_mac_per_s": rate["trsm"],
             "seconds": sec,
+            "ipm_s_per_iteration_lower_bound": round(ipm_est, 1),
+            "ipm_lower_bound_mp_mac": {k: float(f"{v:.4g}") for k, v in work.items()},
             "sample": f"cpu_baseline/mpf_baseline.c {prec} {n} {n} {threads} (GMP mpf naive loops, OpenMP "
                       f"{threads} threads = this process's CPU share): Cholesky {n}x{n} {sec['cholesky']:.2f} s, "
                       f"TRSM {n} RHS {sec['trsm']:.2f} s, GEMM {n}x{n}x{n} {sec['gemm']:.2f} s; "

@@ -35,30 +35,51 @@ __device__ __forceinline__ bool load_mag(Slot s, const Src& v, int WL) {
     return neg;
 }
 
-__device__ __forceinline__ int topl_slot(Slot s, int WL) {
+template <class S>
+__device__ __forceinline__ int topl_slot(const S& s, int WL) {
     for (int i = WL - 1; i > 0; i--)
         if (s[i]) return i;
     return 0;
 }
 
-// Streams the words j = 0..WL-1 of mulfx(a, b) (sdpvibe/basecase.py:31-53) given magnitudes in x, y
-// and the product sign; out(j, word). Columns kmin..LF-1 only feed the carry chain.
-template <class Out>
-__device__ __forceinline__ void mulfx_stream(Slot x, Slot y, bool neg, int t, int WL, int LF, Out out) {
+// column sum sum_{i=i0}^{i1} x[i] y[k - i] as 32-bit halves (lo: low words, hi: high words), eight
+// terms at a time so that sixteen LDS reads are in flight before the multiplies use them
+template <class X, class Y>
+__device__ __forceinline__ void col_sum(const X& x, const Y& y, int k, int i0, int i1, u64& lo, u64& hi) {
+    int i = i0;
+    for (; i + 7 <= i1; i += 8) {
+        u32 a[8], b[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+            a[u] = x[i + u];
+            b[u] = y[k - i - u];
+        }
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+            const u64 p = (u64)a[u] * b[u];
+            lo += (u32)p;
+            hi += p >> 32;
+        }
+    }
+    for (; i <= i1; i++) {
+        const u64 p = (u64)x[i] * y[k - i];
+        lo += (u32)p;
+        hi += p >> 32;
+    }
+}
+
+// Streams the magnitude words j = 0..WL-1 of mulfx(a, b, t) (sdpvibe/basecase.py:31-53) from the
+// magnitudes x = |a|, y = |b| (word accessors x[i], y[i]); out(j, word). Columns kmin..LF-1 only
+// feed the carry chain.
+template <class X, class Y, class Out>
+__device__ __forceinline__ void mulfx_mag(const X& x, const Y& y, int t, int WL, int LF, Out out) {
     const int ta = topl_slot(x, WL), tb = topl_slot(y, WL);
     const int la = t ? max(0, ta - t + 1) : 0, lb = t ? max(0, tb - t + 1) : 0;
     const int kmin = max(LF - 2, la + lb), kmax = ta + tb;
-    u64 carry = 0, hprev = 0, nc = 1;
+    u64 carry = 0, hprev = 0;
     auto column = [&](int k) -> u64 {
         u64 lo = 0, hi = 0;
-        if (k <= kmax) {
-            const int i0 = max(la, k - tb), i1 = min(ta, k - lb);
-            for (int i = i0; i <= i1; i++) {
-                const u64 p = (u64)x[i] * y[k - i];
-                lo += (u32)p;
-                hi += p >> 32;
-            }
-        }
+        if (k <= kmax) col_sum(x, y, k, max(la, k - tb), min(ta, k - lb), lo, hi);
         const u64 sm = lo + hprev + carry;
         carry = sm >> 32;
         hprev = hi;
@@ -67,15 +88,22 @@ __device__ __forceinline__ void mulfx_stream(Slot x, Slot y, bool neg, int t, in
     for (int k = kmin; k < LF && k <= kmax + 2; k++) column(k);
     for (int j = 0; j < WL; j++) {
         const int k = LF + j;
-        u32 w = 0;
-        if (k >= kmin && k <= kmax + 2) w = (u32)column(k);
+        out(j, (k >= kmin && k <= kmax + 2) ? (u32)column(k) : 0u);
+    }
+}
+
+// the same product with the sign applied: out(j, word j of the two's complement result)
+template <class Out>
+__device__ __forceinline__ void mulfx_stream(Slot x, Slot y, bool neg, int t, int WL, int LF, Out out) {
+    u64 nc = 1;
+    mulfx_mag(x, y, t, WL, LF, [&](int j, u32 w) {
         if (neg) {
             const u64 tt = (u64)(~w) + nc;
             w = (u32)tt;
             nc = tt >> 32;
         }
         out(j, w);
-    }
+    });
 }
 
 struct GlobalEntry {  // WL consecutive words in global memory
@@ -298,14 +326,7 @@ __device__ u32 lane_mulfx(u32 a, u32 b, int t, int WL, int LF, u32* xs, u32* ys)
     for (int h = 0; h < 2; h++) {
         const int k = kmin + l + 64 * h;
         u64 cl = 0, ch = 0;
-        if (k <= kmax) {
-            const int i0 = max(la, k - tb), i1 = min(ta, k - lb);
-            for (int i = i0; i <= i1; i++) {
-                const u64 p = (u64)xs[i] * ys[k - i];
-                cl += (u32)p;
-                ch += p >> 32;
-            }
-        }
+        if (k <= kmax) col_sum(xs, ys, k, max(la, k - tb), min(ta, k - lb), cl, ch);
         lo[h] = cl;
         hi[h] = ch;
     }
@@ -524,6 +545,248 @@ __global__ void __launch_bounds__(TT) k_tile_factor(u32* __restrict__ Wg, u32* _
     }
 }
 
+// ---------------------------------------------------------------- LDS-resident factorisation
+// For tiles whose lower triangle fits in LDS (WL <= 64 at BT = 32): the same recurrences as
+// k_tile_factor, with the working tile, the pivot column's magnitudes and the pivot in LDS, so that
+// every product reads its operands from LDS and updates its target there. The inverse is built by
+// k_tile_inverse, one wave per (tile, column). Products and sums are exactly those of
+// k_tile_factor: a product enters as the truncated product of the magnitudes with its sign, and
+// A - (-|p|) is A + |p| modulo 2^(32 WL).
+struct Row {  // consecutive words in LDS
+    u32* p;
+    __device__ __forceinline__ u32& operator[](int k) const { return p[k]; }
+};
+
+// words of the LDS-resident factorisation for BT, WL (entries padded to an odd stride WS)
+__host__ __device__ inline i64 lds_factor_words(int BT, int WL) {
+    const i64 WS = WL | 1;
+    return (i64)BT * (BT + 1) / 2 * WS + 2 * (i64)BT * WS + WL + BT + 1 + 128;
+}
+
+// |v| of WL words from src into dst, returns the sign
+__device__ __forceinline__ bool mag_copy(u32* dst, const u32* src, int WL) {
+    const bool neg = (src[WL - 1] >> 31) != 0;
+    u64 c = 1;
+    for (int q = 0; q < WL; q++) {
+        u32 w = src[q];
+        if (neg) {
+            const u64 t = (u64)(~w) + c;
+            w = (u32)t;
+            c = t >> 32;
+        }
+        dst[q] = w;
+    }
+    return neg;
+}
+
+// row i of the packed lower triangle starts at entry i (i + 1) / 2
+__device__ __forceinline__ int tri_base(int i) { return i * (i + 1) / 2; }
+
+// index p of the pairs (i, j), 0 <= j <= i, in row order -> (i, j)
+__device__ __forceinline__ void tri_pair(int p, int& i, int& j) {
+    int r = (int)((sqrtf(8.0f * p + 1.0f) - 1.0f) * 0.5f);
+    while ((r + 1) * (r + 2) / 2 <= p) r++;
+    while (r * (r + 1) / 2 > p) r--;
+    i = r;
+    j = p - r * (r + 1) / 2;
+}
+
+__global__ void __launch_bounds__(TT) k_tile_chol_lds(u32* __restrict__ Wg, u32* __restrict__ Rg, int BT, int WL,
+                                                     int LF) {
+    SDPV_DYNAMIC_LDS(u32, lds);
+    const int tid = threadIdx.x;
+    const i64 tile = blockIdx.x;
+    const int WS = WL | 1;
+    const int ne = BT * (BT + 1) / 2;
+    u32* T = lds;                  // ne x WS: the lower triangle, row by row
+    u32* CA = T + ne * WS;         // BT x WS: |A_ik| of the pivot column
+    u32* CL = CA + BT * WS;        // BT x WS: |L_ik| of the pivot column
+    u32* RM = CL + BT * WS;        // WL: |r| of the pivot
+    u32* SG = RM + WL;             // BT + 1: sign of L_ik (slot i), sign of r (slot BT)
+    u32* PX = SG + BT + 1;         // 64 + 64: lane_mulfx operands
+    u32* PY = PX + 64;
+    u32* W = Wg + tile * BT * BT * WL;
+    auto te = [&](int i, int j) { return T + (i64)(tri_base(i) + j) * WS; };
+    const int total = BT * BT * WL;
+    // the lower triangle into LDS, 16 independent loads per thread in flight
+    for (int base = tid; base < total; base += 16 * TT) {
+        u32 v[16];
+#pragma unroll
+        for (int u = 0; u < 16; u++) {
+            const int idx = base + u * TT;
+            v[u] = idx < total ? W[idx] : 0u;
+        }
+#pragma unroll
+        for (int u = 0; u < 16; u++) {
+            const int idx = base + u * TT;
+            if (idx < total) {
+                const int e = idx / WL, q = idx - e * WL, i = e / BT, j = e - i * BT;
+                if (j <= i) te(i, j)[q] = v[u];
+            }
+        }
+    }
+    __syncthreads();
+    for (int k = 0; k < BT; k++) {
+        // pivot on wave 0: d = A_kk, r = d^-1/2, A_kk = mulfx(d, r), RINV_k = r
+        if (tid < 64) {
+            u32* dkk = te(k, k);
+            const u32 d = tid < WL ? dkk[tid] : 0u;
+            const u32 r = lane_rsqrt(d, WL, LF, PX, PY);
+            const u32 dr = lane_mulfx(d, r, 0, WL, LF, PX, PY);
+            const bool nr = (lane_get(r, WL - 1) >> 31) != 0;
+            const u32 rm = nr ? lane_neg(r, WL) : r;
+            if (tid < WL) {
+                dkk[tid] = dr;
+                RM[tid] = rm;
+                Rg[(tile * BT + k) * WL + tid] = r;
+            }
+            if (tid == 0) SG[BT] = nr;
+        }
+        __syncthreads();
+        // column scale, thread i - k - 1: L_ik = mulfx(A_ik, r)
+        const int i = k + 1 + tid;
+        if (i < BT) {
+            u32* a = te(i, k);
+            u32* ca = CA + (i64)i * WS;
+            u32* cl = CL + (i64)i * WS;
+            const bool neg = mag_copy(ca, a, WL) ^ (SG[BT] != 0);
+            u64 c = 1;
+            mulfx_mag(Row{ca}, Row{RM}, 0, WL, LF, [&](int j, u32 w) {
+                if (neg) {
+                    const u64 t = (u64)(~w) + c;
+                    w = (u32)t;
+                    c = t >> 32;
+                }
+                a[j] = w;
+            });
+            // the trailing products take the magnitude of the stored value, as k_tile_factor does
+            SG[i] = mag_copy(cl, a, WL);
+        }
+        __syncthreads();
+        // trailing update over the lower pairs k < j <= i: A_ij -= L_ik L_jk
+        const int m = BT - k - 1, npairs = m * (m + 1) / 2;
+        for (int p = tid; p < npairs; p += TT) {
+            int ii, jj;
+            tri_pair(p, ii, jj);
+            const int r = k + 1 + ii, c = k + 1 + jj;
+            u32* aij = te(r, c);
+            if (SG[r] ^ SG[c]) {  // the product is negative: add its magnitude
+                u64 cy = 0;
+                mulfx_mag(Row{CL + (i64)r * WS}, Row{CL + (i64)c * WS}, 0, WL, LF, [&](int jw, u32 w) {
+                    const u64 q = (u64)aij[jw] + w + cy;
+                    aij[jw] = (u32)q;
+                    cy = q >> 32;
+                });
+            } else {
+                i64 br = 0;
+                mulfx_mag(Row{CL + (i64)r * WS}, Row{CL + (i64)c * WS}, 0, WL, LF, [&](int jw, u32 w) {
+                    const i64 q = (i64)aij[jw] - (i64)w + br;
+                    aij[jw] = (u32)q;
+                    br = q >> 32;
+                });
+            }
+        }
+        __syncthreads();
+    }
+    // the factor back, zeros above the diagonal
+    for (int idx = tid; idx < total; idx += TT) {
+        const int e = idx / WL, q = idx - e * WL, i = e / BT, j = e - i * BT;
+        W[idx] = j <= i ? te(i, j)[q] : 0u;
+    }
+}
+
+// words of k_tile_inverse's LDS for BT, WL
+__host__ __device__ inline i64 lds_inverse_words(int BT, int WL) {
+    const i64 WS = WL | 1;
+    return 3 * (i64)BT * WS + BT + 128;
+}
+
+// Column k of X = L^-1 for one tile, on one wave (the recurrences of k_tile_factor's inverse):
+// x_kk = rinv_k; for i > k, x_ik = -mulfx(sum_{j=k}^{i-1} mulfx(l_ij, x_jk), rinv_i), every product
+// truncated, the sum exact. Step i: lane j - k forms the product for j; the lanes then add the
+// products word-parallel (lane q holds word q) and one lane-parallel product finishes x_ik.
+// Grid (nb, BT), 64 threads. X entries above the diagonal are written as zeros.
+__global__ void __launch_bounds__(64) k_tile_inverse(const u32* __restrict__ Wg, const u32* __restrict__ Rg,
+                                                    u32* __restrict__ Xg, int BT, int WL, int LF) {
+    SDPV_DYNAMIC_LDS(u32, lds);
+    const int l = threadIdx.x;
+    const i64 tile = blockIdx.x;
+    const int k = blockIdx.y;
+    const int WS = WL | 1;
+    u32* XC = lds;                      // BT x WS: |x_jk|
+    u32* LR = XC + BT * WS;             // BT x WS: |l_ij|, j = k..i-1 (slot j)
+    u32* PR = LR + BT * WS;             // BT x WS: the products (two's complement), slot j
+    u32* XS = PR + BT * WS;             // BT: sign of x_jk
+    u32* PX = XS + BT;                  // 64 + 64: lane_mulfx operands
+    u32* PY = PX + 64;
+    const u32* W = Wg + tile * BT * BT * WL;
+    const u32* R = Rg + tile * BT * WL;
+    u32* X = Xg + tile * BT * BT * WL;
+    for (int i = 0; i < k; i++)
+        for (int q = l; q < WL; q += 64) X[((i64)i * BT + k) * WL + q] = 0u;
+    // x_kk = r_k
+    {
+        const u32 r = l < WL ? R[(i64)k * WL + l] : 0u;
+        const bool nr = (lane_get(r, WL - 1) >> 31) != 0;
+        const u32 rm = nr ? lane_neg(r, WL) : r;
+        if (l < WL) {
+            X[((i64)k * BT + k) * WL + l] = r;
+            XC[(i64)k * WS + l] = rm;
+        }
+        if (l == 0) XS[k] = nr;
+    }
+    wave_sync();
+    for (int i = k + 1; i < BT; i++) {
+        // l_ij, j = k..i-1: (i - k) WL consecutive words of row i, read by all lanes
+        const int nw = (i - k) * WL;
+        const u32* src = W + ((i64)i * BT + k) * WL;
+        for (int idx = l; idx < nw; idx += 64) {
+            const int e = idx / WL, q = idx - e * WL;
+            LR[(i64)(k + e) * WS + q] = src[idx];
+        }
+        const u32 ri = l < WL ? R[(i64)i * WL + l] : 0u;
+        wave_sync();
+        const int j = k + l;
+        if (j < i) {  // product of slot j
+            u32* lr = LR + (i64)j * WS;
+            u32* pr = PR + (i64)j * WS;
+            // |l_ij| in place: the magnitude is formed word by word ahead of its first read
+            const bool neg = mag_copy(lr, lr, WL) ^ (XS[j] != 0);
+            u64 c = 1;
+            mulfx_mag(Row{lr}, Row{XC + (i64)j * WS}, 0, WL, LF, [&](int q, u32 w) {
+                if (neg) {
+                    const u64 t = (u64)(~w) + c;
+                    w = (u32)t;
+                    c = t >> 32;
+                }
+                pr[q] = w;
+            });
+        }
+        wave_sync();
+        // acc = sum of the products (mod 2^(32 WL)), word q on lane q
+        u64 s = 0;
+        if (l < WL)
+            for (int jj = k; jj < i; jj++) s += PR[(i64)jj * WS + l];
+        const u64 hb = from_below64(s >> 32);                 // the high part of word q - 1
+        const u64 d = (s & 0xffffffffull) + hb;               // < 2^33
+        const u64 G = __ballot(l < WL && (d >> 32) != 0);
+        const u64 P = __ballot(l < WL && (u32)d == 0xffffffffu);
+        const u64 Xm = G | P;
+        const u64 carries = (Xm + G) ^ Xm ^ G;
+        const u32 acc = l < WL ? (u32)d + (u32)((carries >> l) & 1u) : 0u;
+        // x_ik = -mulfx(acc, r_i)
+        const u32 x = lane_neg(lane_mulfx(acc, ri, 0, WL, LF, PX, PY), WL);
+        const bool nx = (lane_get(x, WL - 1) >> 31) != 0;
+        const u32 xm = nx ? lane_neg(x, WL) : x;
+        if (l < WL) {
+            X[((i64)i * BT + k) * WL + l] = x;
+            XC[(i64)i * WS + l] = xm;
+        }
+        if (l == 0) XS[i] = nx;
+        wave_sync();
+    }
+}
+
 // Forward substitution L y = v for many vectors with one factored tile (the reference's trsv_warp,
 // sdpvibe/basecase.py:228-245): 32 lanes per vector, lane i owns v_i (BT <= 32), kept in place in
 // V (nvec, BT, WL). Step j: lane j scales y_j = mulfx(v_j, rinv_j); after the barrier every lane
@@ -613,10 +876,22 @@ int sdpv_tile_chol(const int32_t* A, int64_t L, int64_t nb, int64_t BT, int32_t*
     static bool attr = false;
     if (!attr) {
         (void)hipFuncSetAttribute((const void*)k_tile_factor, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+        (void)hipFuncSetAttribute((const void*)k_tile_chol_lds, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  160 * 1024);
         attr = true;
     }
-    hipLaunchKernelGGL(k_tile_factor, dim3((unsigned)nb), dim3(TT), lds, st, W, X, S, RI, (int)BT, WL, (int)L);
-    SDPV_TRY_LAUNCH();
+    const size_t lds_f = (size_t)lds_factor_words((int)BT, WL) * 4;
+    if (WL <= 64 && lds_f <= 160 * 1024) {  // the tile stays in LDS; the inverse one wave per column
+        hipLaunchKernelGGL(k_tile_chol_lds, dim3((unsigned)nb), dim3(TT), lds_f, st, W, RI, (int)BT, WL, (int)L);
+        SDPV_TRY_LAUNCH();
+        const size_t lds_i = (size_t)lds_inverse_words((int)BT, WL) * 4;
+        hipLaunchKernelGGL(k_tile_inverse, dim3((unsigned)nb, (unsigned)BT), dim3(64), lds_i, st, W, RI, X, (int)BT,
+                           WL, (int)L);
+        SDPV_TRY_LAUNCH();
+    } else {
+        hipLaunchKernelGGL(k_tile_factor, dim3((unsigned)nb), dim3(TT), lds, st, W, X, S, RI, (int)BT, WL, (int)L);
+        SDPV_TRY_LAUNCH();
+    }
     hipLaunchKernelGGL(k_tile_narrow, dim3(nblk(n, 256)), dim3(256), 0, st, W, X, (int)L, WL, Lout, Xw, n, n, flag);
     SDPV_TRY_LAUNCH();
     return SDPV_OK;

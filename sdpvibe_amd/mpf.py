@@ -286,8 +286,11 @@ def dot(A, B):
 
 
 def max_abs(A):
-    """largest |entry| as an exact mpmath number: the candidates within one binade of the largest
-    top, compared in float64, the chosen entry read exactly"""
+    """largest |entry| as an exact mpmath number, the same for any order or grouping of the entries:
+    the candidates within one binade of the largest top are compared in float64; the ones float64
+    cannot separate from the largest (within 2^-40) are deduplicated by their exact words and
+    compared exactly. (The reference, sdpvibe/mpf.py:370-380, takes the float64 argmax, which picks
+    among such near-ties by position.)"""
     if A.e.numel() == 0:
         return mp.mpf(0)
     tops = _top(A).reshape(-1)
@@ -297,8 +300,14 @@ def max_abs(A):
     near = torch.nonzero(tops >= tmax - 1).reshape(-1)
     flat = A.reshape(-1)
     approx = np.abs(MPF(flat.m[:, near], flat.e[near]).shift(-tmax).to_float())
-    pick = int(near[int(np.argmax(approx))])
-    return abs(flat[pick:pick + 1].to_mp()[0])
+    close = torch.from_numpy(np.nonzero(approx >= approx.max() * (1 - 2.0 ** -40))[0]).to(near.device)
+    idx = near[close]
+    if idx.numel() > 1:
+        key = torch.cat([flat.m[:, idx].t().to(torch.int64), flat.e[idx][:, None]], 1)
+        _, first = np.unique(key.cpu().numpy(), axis=0, return_index=True)
+        idx = idx[torch.from_numpy(np.sort(first)).to(idx.device)]
+    vals = MPF(flat.m[:, idx], flat.e[idx]).to_mp()
+    return max(abs(v) for v in vals)
 
 
 def trace(A):

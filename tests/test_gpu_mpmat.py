@@ -120,3 +120,21 @@ def test_mpf_reductions(L):
     d = F.dot(A, A)
     exact = sum((mp.mpf(v) * mp.mpf(2) ** e) ** 2 for v, e in zip(ints, exps) if v)
     assert abs(d - exact) <= exact * mp.mpf(2) ** (4 - NB)
+
+
+@pytest.mark.parametrize("L", [3, 24])
+def test_max_abs_near_ties_any_order(L):
+    """max_abs is the exact maximum whatever the order of the entries, also among values that
+    float64 cannot tell apart (equal to 2^-60 relative), duplicates and opposite signs"""
+    NB = 32 * L - 2
+    base = (1 << (NB - 1)) + 12345
+    cands = [(base, 0), (-(base + 1), 0), (base - 1, 0), (base + 1, 0), (base, 0), (-(base + 1), 0), (base >> 3, 3)]
+    rng = random.Random(5)
+    fill = [(rng.randrange(1, 1 << (NB - 8)), rng.randrange(-40, -1)) for _ in range(50)]
+    mp.mp.prec = 64 * L + 100
+    want = mp.mpf(base + 1)
+    for seed in range(4):
+        entries = cands + fill
+        random.Random(seed).shuffle(entries)
+        A = F.from_ints([v for v, _ in entries], [e for _, e in entries], L, (len(entries),))
+        assert F.max_abs(A) == want
