@@ -18,8 +18,10 @@ dr1 = -dc + dB y, dr2 = db - dB^T x, so for any linear functional l = g_x . x + 
 
 i.e. one solve with the stored factor of Q and two triangular solves per block: the gradients are
 dl/dc = -lam_x, dl/db = lam_y, dl/dB = lam_x y^T - x lam_y^T. All products are exact multiprecision
-GEMMs. `directional` evaluates (1) for a batch of directions with one exact GEMM per term;
-`SDPValue` exposes v(theta) = v(data + sum_k theta_k D_k) to torch.autograd.
+GEMMs. With l = c.x (or b.y) this gives the gradient of the primal (dual) objective
+(`objective_gradients`); `tangent` is the forward solve of the same system. `directional` evaluates
+(1) for a batch of directions with one exact GEMM per term; `SDPValue` exposes
+v(theta) = v(data + sum_k theta_k D_k) to torch.autograd, its backward through the adjoint solve.
 """
 import mpmath as mp
 import torch
@@ -74,20 +76,27 @@ def directional(solver, dc, dB, db):
     return out
 
 
+def _group_slices(solver, v):
+    """a vector over all blocks (sum P, C) -> per group (nb, P, C) MPF"""
+    out, o = [], 0
+    for g in solver.sdp.groups:
+        n = g.nb * g.P
+        out.append(MPF(v.m[:, o:o + n].contiguous(), v.e[o:o + n].contiguous()).reshape(g.nb, g.P, v.shape[-1]))
+        o += n
+    return out
+
+
+def _factors(solver):
+    ff = solver.final_factors
+    return ff if ff is not None and "P" in ff else refactor(solver)
+
+
 def adjoint(solver, gx, gy):
     """gradients of l = gx . x + gy . y (gx (sum P, 1), gy (N, 1) MPF) with respect to (c, b, B) at the
     factored iterate, through the adjoint Schur-complement solve (see the module docstring)"""
-    ff = solver.final_factors
-    if ff is None or "P" not in ff:
-        ff = refactor(solver)
+    ff = _factors(solver)
     groups = solver.sdp.groups
-    # per group slices of gx, in the group's batched (nb, P, 1) shape
-    gxs, o = [], 0
-    for g in groups:
-        n = g.nb * g.P
-        gxs.append(MPF(gx.m[:, o:o + n].contiguous(), gx.e[o:o + n].contiguous()).reshape(g.nb, g.P, 1))
-        o += n
-    u = [cS.solve_lower(v) for cS, v in zip(ff["S"], gxs)]          # L_S^-1 g_x
+    u = [cS.solve_lower(v) for cS, v in zip(ff["S"], _group_slices(solver, gx))]     # L_S^-1 g_x
     Pall = _cat_rows([stack_rows(P) for P in ff["P"]])
     uall = _cat_rows([stack_rows(v) for v in u])
     lam_y = ff["Q"].solve(add(gy, gemm(Pall, uall, transA=True)))
@@ -100,23 +109,85 @@ def adjoint(solver, gx, gy):
     return -lam_x, lam_y, dB
 
 
+def tangent(solver, dc, dB, db):
+    """(dx, dy) solving the Newton system K [dx; dy] = [-dc + dB y; db - dB^T x] with the stored factors
+    (dc (sum P, 1), dB (sum P, N), db (N, 1)): dy = Q^-1 (r2 - P^T L_S^-1 r1), dx = L_S^-T (L_S^-1 r1 + P dy).
+    adjoint() is its transpose: gx . dx + gy . dy equals the contraction of adjoint(gx, gy) with the
+    direction."""
+    ff = _factors(solver)
+    x = primal_vector(solver)
+    r1 = add(gemm(dB, solver.y), dc, -1)
+    r2 = add(db, gemm(dB, x, transA=True), -1)
+    u = [cS.solve_lower(v) for cS, v in zip(ff["S"], _group_slices(solver, r1))]
+    Pall = _cat_rows([stack_rows(P) for P in ff["P"]])
+    dy = ff["Q"].solve(add(r2, gemm(Pall, _cat_rows([stack_rows(v) for v in u]), transA=True), -1))
+    dx = [stack_rows(cS.solve_upper(add(v, gemm(P, repeat_batch(dy, g.nb)))))
+          for g, cS, P, v in zip(solver.sdp.groups, ff["S"], ff["P"], u)]
+    return _cat_rows(dx), dy
+
+
+def c_vector(sdp):
+    """c over all blocks (sum P, 1), in the order of primal_vector"""
+    L = sdp.L
+    return MPF(torch.cat([g.c.m.reshape(L, -1) for g in sdp.groups], 1).reshape(L, -1, 1),
+               torch.cat([g.c.e.reshape(-1) for g in sdp.groups]).reshape(-1, 1))
+
+
+def objective_gradients(solver, which="primal"):
+    """(dv/dc (sum P, 1), dv/db (N, 1), dv/dB (sum P, N)) of the primal objective c.x (which="primal") or
+    the dual objective b.y ("dual") at the solver's iterate, through the adjoint solve:
+    d(c.x) = dc.x + c.dx, d(b.y) = db.y + b.dy, with c.dx and b.dy from adjoint()"""
+    x, y = primal_vector(solver), solver.y
+    L = solver.L
+    if which == "primal":
+        gc, gb, gB = adjoint(solver, c_vector(solver.sdp), F.zeros(tuple(y.shape), L))
+        return add(gc, x), gb, gB
+    if which == "dual":
+        gc, gb, gB = adjoint(solver, F.zeros(tuple(x.shape), L), solver.sdp.b)
+        return gc, add(gb, y), gB
+    raise ValueError(f"which must be 'primal' or 'dual', not {which!r}")
+
+
+def contract(grads, dc, dB, db):
+    """sum of grads . direction for each of K directions (dc (K, sum P, 1), dB (K, sum P, N),
+    db (K, N, 1)) -> list of K mpmath numbers, each product exact and the sum one `total`"""
+    gc, gb, gB = grads
+    K = dc.shape[0]
+    t1 = mul(dc, repeat_batch(gc, K))
+    t2 = mul(db, repeat_batch(gb, K))
+    t3 = mul(dB, repeat_batch(gB, K))
+    out = []
+    for k in range(K):
+        parts = [t1[k], t2[k], t3[k]]
+        out.append(total(MPF(torch.cat([q.m.reshape(q.L, -1) for q in parts], 1),
+                             torch.cat([q.e.reshape(-1) for q in parts]))))
+    return out
+
+
 class SDPValue(torch.autograd.Function):
     """v(theta) = optimal value of the SDP with data (c, B, b) + sum_k theta_k (dc_k, dB_k, db_k), for a
-    solver already run to optimality at theta = 0. forward returns v as a float64 scalar (its exact
-    mpmath value is kept in ctx.value); backward returns dv/dtheta_k from (1), evaluated exactly.
-    Use: v = SDPValue.apply(theta, solver, (dc, dB, db))."""
+    solver already run to optimality at theta = 0. forward returns v (the primal or the dual objective)
+    as a float64 scalar (its exact mpmath value is kept in ctx.value); backward returns dv/dtheta_k,
+    by default through the adjoint Schur-complement solve (objective_gradients), or from the
+    first-order estimate (1) with method="first_order".
+    Use: v = SDPValue.apply(theta, solver, (dc, dB, db), "primal", "adjoint")."""
 
     @staticmethod
-    def forward(ctx, theta, solver, directions):
-        ctx.solver, ctx.directions = solver, directions
-        ctx.value = solver.result["primalObjective"]
+    def forward(ctx, theta, solver, directions, which="primal", method="adjoint"):
+        if method not in ("adjoint", "first_order"):
+            raise ValueError(f"method must be 'adjoint' or 'first_order', not {method!r}")
+        ctx.solver, ctx.directions, ctx.which, ctx.method = solver, directions, which, method
+        ctx.value = solver.result["primalObjective" if which == "primal" else "dualObjective"]
         return torch.tensor(float(ctx.value), dtype=torch.float64)
 
     @staticmethod
     def backward(ctx, grad):
-        d = directional(ctx.solver, *ctx.directions)
+        if ctx.method == "adjoint":
+            d = contract(objective_gradients(ctx.solver, ctx.which), *ctx.directions)
+        else:
+            d = directional(ctx.solver, *ctx.directions)
         g = torch.tensor([float(v) for v in d], dtype=torch.float64) * grad
-        return g, None, None
+        return g, None, None, None, None
 
 
 def perturbed(sdp, dc, dB, db, t):
