@@ -1,12 +1,16 @@
 // Tile base case on gfx950: batched multiprecision Cholesky and triangular inverse of BT x BT
 // tiles in wide fixed point (WL = L + 24 words, F = 32 L fractional bits), the arithmetic of the
-// reference's chol_tiles / trinv_tiles (sdpvibe/basecase.py:13-246). One workgroup per tile.
+// reference's chol_tiles / trinv_tiles (sdpvibe/basecase.py:13-246).
 //
 // Every product is the reference's truncated product mulfx (columns below LF - 2 dropped, signs
-// applied to magnitudes), so results are bit-identical. Operands are staged as magnitudes into
-// per-thread LDS slots (thread-strided, conflict-free) and the product is streamed out word by
-// word into its consumer (store, subtract, accumulate). The pivot reciprocal square root (double
-// seed, precision-doubling Newton) runs on one wave, its column sums spread over the lanes.
+// applied to magnitudes), so results are bit-identical. Two forms of the product: one thread
+// streams it word by word from LDS operands into its consumer (store, subtract, accumulate), or
+// one wave holds a word per lane, sums the columns on the lanes and normalises them with shuffles
+// and ballot carry-lookahead (lane_mulfx). When the tile's lower triangle fits in LDS
+// (k_tile_chol_lds) the factorisation runs there and the inverse is built column-parallel
+// (k_tile_inverse); otherwise k_tile_factor works on the tile in HBM.
+#include <cstdlib>
+
 #include "common.hpp"
 
 namespace {
@@ -547,36 +551,23 @@ __global__ void __launch_bounds__(TT) k_tile_factor(u32* __restrict__ Wg, u32* _
 
 // ---------------------------------------------------------------- LDS-resident factorisation
 // For tiles whose lower triangle fits in LDS (WL <= 64 at BT = 32): the same recurrences as
-// k_tile_factor, with the working tile, the pivot column's magnitudes and the pivot in LDS, so that
-// every product reads its operands from LDS and updates its target there. The inverse is built by
-// k_tile_inverse, one wave per (tile, column). Products and sums are exactly those of
-// k_tile_factor: a product enters as the truncated product of the magnitudes with its sign, and
-// A - (-|p|) is A + |p| modulo 2^(32 WL).
+// k_tile_factor, with the working tile in LDS. Products are formed either by one thread (the
+// column sums in a loop, operands read from LDS) or by one wave with a word per lane (lane_mulfx);
+// both are the truncated product of the magnitudes with the sign applied, so the results are those
+// of k_tile_factor bit for bit (A - (-|p|) is A + |p| modulo 2^(32 WL)). The wave form has far
+// less latency per product; the thread form does 256 products at once. The inverse is built by
+// k_tile_inverse, one workgroup per (tile, column).
 struct Row {  // consecutive words in LDS
     u32* p;
     __device__ __forceinline__ u32& operator[](int k) const { return p[k]; }
 };
 
+constexpr int TWAVES = TT / 64;
+
 // words of the LDS-resident factorisation for BT, WL (entries padded to an odd stride WS)
 __host__ __device__ inline i64 lds_factor_words(int BT, int WL) {
     const i64 WS = WL | 1;
-    return (i64)BT * (BT + 1) / 2 * WS + 2 * (i64)BT * WS + WL + BT + 1 + 128;
-}
-
-// |v| of WL words from src into dst, returns the sign
-__device__ __forceinline__ bool mag_copy(u32* dst, const u32* src, int WL) {
-    const bool neg = (src[WL - 1] >> 31) != 0;
-    u64 c = 1;
-    for (int q = 0; q < WL; q++) {
-        u32 w = src[q];
-        if (neg) {
-            const u64 t = (u64)(~w) + c;
-            w = (u32)t;
-            c = t >> 32;
-        }
-        dst[q] = w;
-    }
-    return neg;
+    return (i64)BT * (BT + 1) / 2 * WS + (i64)BT * WS + WL + 2 * BT + 128 * TWAVES;
 }
 
 // row i of the packed lower triangle starts at entry i (i + 1) / 2
@@ -591,19 +582,100 @@ __device__ __forceinline__ void tri_pair(int p, int& i, int& j) {
     j = p - r * (r + 1) / 2;
 }
 
+// A -= s * mulfx(x, y, 0) on LDS words (s = +1, or -1 when `add`): x, y magnitudes with top words
+// ta, tb, A a WL-word two's complement target. The column sums are formed four columns at a time:
+// x[i] is read once per block and the four operand words y[kb + c - i] sit in a window of registers
+// that shifts by one word per i, so a block costs two LDS reads per i instead of eight; the target
+// words of a block are read before its columns are summed. The column sums, the carry chain and the
+// target words equal those of mulfx_mag and its consumer (terms outside the tops are zero).
+__device__ __forceinline__ void trail_update(const u32* x, const u32* y, int ta, int tb, int WL, int LF, u32* A,
+                                             bool add) {
+    const int kmin = max(LF - 2, 0), kmax = ta + tb, kend = LF + WL;
+    u64 carry = 0, hprev = 0, cy = 0;  // product chain; carry of the addition
+    i64 br = 0;                          // borrow of the subtraction
+    for (int kb = kmin; kb < kend; kb += 4) {
+        u32 tw[4];
+#pragma unroll
+        for (int c = 0; c < 4; c++) {
+            const int j = kb + c - LF;
+            tw[c] = (j >= 0 && j < WL) ? A[j] : 0u;
+        }
+        u64 lo[4] = {0, 0, 0, 0}, hi[4] = {0, 0, 0, 0};
+        if (kb <= kmax) {
+            const int I0 = max(0, kb - tb), I1 = min(ta, kb + 3);
+            u32 w[4];
+#pragma unroll
+            for (int c = 0; c < 4; c++) {
+                const int j = kb + c - I0;
+                w[c] = j <= tb ? y[j] : 0u;
+            }
+            for (int i = I0; i <= I1; i += 4) {
+                u32 xa[4], yn[4];
+#pragma unroll
+                for (int u = 0; u < 4; u++) {
+                    const int ii = i + u, jn = kb - ii - 1;
+                    xa[u] = ii <= I1 ? x[ii] : 0u;
+                    yn[u] = (jn >= 0 && jn <= tb) ? y[jn] : 0u;
+                }
+#pragma unroll
+                for (int u = 0; u < 4; u++) {
+#pragma unroll
+                    for (int c = 0; c < 4; c++) {
+                        const u64 p = (u64)xa[u] * w[c];
+                        lo[c] += (u32)p;
+                        hi[c] += p >> 32;
+                    }
+                    w[3] = w[2];
+                    w[2] = w[1];
+                    w[1] = w[0];
+                    w[0] = yn[u];
+                }
+            }
+        }
+#pragma unroll
+        for (int c = 0; c < 4; c++) {
+            const int k = kb + c;
+            if (k >= kend) break;
+            u32 wv = 0u;
+            if (k <= kmax + 2) {
+                const u64 sm = lo[c] + hprev + carry;
+                carry = sm >> 32;
+                hprev = hi[c];
+                wv = (u32)sm;
+            }
+            if (k >= LF) {
+                if (add) {
+                    const u64 q = (u64)tw[c] + wv + cy;
+                    A[k - LF] = (u32)q;
+                    cy = q >> 32;
+                } else {
+                    const i64 q = (i64)tw[c] - (i64)wv + br;
+                    A[k - LF] = (u32)q;
+                    br = q >> 32;
+                }
+            }
+        }
+    }
+}
+
+// Pivot k: wave 0 computes r = d^-1/2 and d r (lane_rsqrt, lane_mulfx). Column scale: the waves
+// take the entries i > k in turn, L_ik = lane_mulfx(A_ik, r), and keep |L_ik|, its top word and its
+// sign for the trailing products. Trailing update over the pairs k < j <= i: wave 0 updates the
+// next diagonal entry and goes on to pivot k + 1 (look-ahead), the other three waves update the
+// remaining pairs, one pair per thread while there are more than `lane_pairs`, else one per wave.
 __global__ void __launch_bounds__(TT) k_tile_chol_lds(u32* __restrict__ Wg, u32* __restrict__ Rg, int BT, int WL,
-                                                     int LF) {
+                                                     int LF, int lane_pairs) {
     SDPV_DYNAMIC_LDS(u32, lds);
-    const int tid = threadIdx.x;
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const i64 tile = blockIdx.x;
     const int WS = WL | 1;
     const int ne = BT * (BT + 1) / 2;
     u32* T = lds;                  // ne x WS: the lower triangle, row by row
-    u32* CA = T + ne * WS;         // BT x WS: |A_ik| of the pivot column
-    u32* CL = CA + BT * WS;        // BT x WS: |L_ik| of the pivot column
-    u32* RM = CL + BT * WS;        // WL: |r| of the pivot
-    u32* SG = RM + WL;             // BT + 1: sign of L_ik (slot i), sign of r (slot BT)
-    u32* PX = SG + BT + 1;         // 64 + 64: lane_mulfx operands
+    u32* CL = T + ne * WS;         // BT x WS: |L_ik| of the pivot column
+    u32* RK = CL + BT * WS;        // WL: r of the pivot
+    u32* SG = RK + WL;             // BT: sign of L_ik
+    u32* TP = SG + BT;             // BT: top word of |L_ik|
+    u32* PX = TP + BT + 128 * wave;  // 64 + 64 per wave: lane_mulfx operands
     u32* PY = PX + 64;
     u32* W = Wg + tile * BT * BT * WL;
     auto te = [&](int i, int j) { return T + (i64)(tri_base(i) + j) * WS; };
@@ -626,64 +698,69 @@ __global__ void __launch_bounds__(TT) k_tile_chol_lds(u32* __restrict__ Wg, u32*
         }
     }
     __syncthreads();
+    // pivot k on wave 0, from the (updated) diagonal entry d held one word per lane:
+    // r = d^-1/2, A_kk = mulfx(d, r), RINV_k = r
+    auto pivot = [&](int k, u32 d) {
+        const u32 r = lane_rsqrt(d, WL, LF, PX, PY);
+        const u32 dr = lane_mulfx(d, r, 0, WL, LF, PX, PY);
+        if (lane < WL) {
+            te(k, k)[lane] = dr;
+            RK[lane] = r;
+            Rg[(tile * BT + k) * WL + lane] = r;
+        }
+    };
+    if (wave == 0) pivot(0, lane < WL ? te(0, 0)[lane] : 0u);
+    __syncthreads();
     for (int k = 0; k < BT; k++) {
-        // pivot on wave 0: d = A_kk, r = d^-1/2, A_kk = mulfx(d, r), RINV_k = r
-        if (tid < 64) {
-            u32* dkk = te(k, k);
-            const u32 d = tid < WL ? dkk[tid] : 0u;
-            const u32 r = lane_rsqrt(d, WL, LF, PX, PY);
-            const u32 dr = lane_mulfx(d, r, 0, WL, LF, PX, PY);
-            const bool nr = (lane_get(r, WL - 1) >> 31) != 0;
-            const u32 rm = nr ? lane_neg(r, WL) : r;
-            if (tid < WL) {
-                dkk[tid] = dr;
-                RM[tid] = rm;
-                Rg[(tile * BT + k) * WL + tid] = r;
-            }
-            if (tid == 0) SG[BT] = nr;
-        }
-        __syncthreads();
-        // column scale, thread i - k - 1: L_ik = mulfx(A_ik, r)
-        const int i = k + 1 + tid;
-        if (i < BT) {
-            u32* a = te(i, k);
-            u32* ca = CA + (i64)i * WS;
-            u32* cl = CL + (i64)i * WS;
-            const bool neg = mag_copy(ca, a, WL) ^ (SG[BT] != 0);
-            u64 c = 1;
-            mulfx_mag(Row{ca}, Row{RM}, 0, WL, LF, [&](int j, u32 w) {
-                if (neg) {
-                    const u64 t = (u64)(~w) + c;
-                    w = (u32)t;
-                    c = t >> 32;
+        // column scale, entry i on wave (i - k - 1) mod 4: L_ik = mulfx(A_ik, r)
+        {
+            const u32 r = lane < WL ? RK[lane] : 0u;
+            for (int i = k + 1 + wave; i < BT; i += TWAVES) {
+                u32* a = te(i, k);
+                const u32 lv = lane_mulfx(lane < WL ? a[lane] : 0u, r, 0, WL, LF, PX, PY);
+                const bool neg = (lane_get(lv, WL - 1) >> 31) != 0;
+                const u32 mag = neg ? lane_neg(lv, WL) : lv;
+                const int top = lane_top(mag, WL);
+                if (lane < WL) {
+                    a[lane] = lv;
+                    CL[(i64)i * WS + lane] = mag;
                 }
-                a[j] = w;
-            });
-            // the trailing products take the magnitude of the stored value, as k_tile_factor does
-            SG[i] = mag_copy(cl, a, WL);
+                if (lane == 0) {
+                    SG[i] = neg;
+                    TP[i] = top;
+                }
+            }
         }
         __syncthreads();
-        // trailing update over the lower pairs k < j <= i: A_ij -= L_ik L_jk
+        if (k == BT - 1) break;
+        // trailing update over the lower pairs k < j <= i: A_ij -= L_ik L_jk. Pair 0, the next
+        // diagonal entry, on wave 0, which then computes pivot k + 1 while waves 1..3 update the
+        // other pairs (one per wave while there are at most `lane_pairs`, else one per thread)
         const int m = BT - k - 1, npairs = m * (m + 1) / 2;
-        for (int p = tid; p < npairs; p += TT) {
-            int ii, jj;
-            tri_pair(p, ii, jj);
-            const int r = k + 1 + ii, c = k + 1 + jj;
-            u32* aij = te(r, c);
-            if (SG[r] ^ SG[c]) {  // the product is negative: add its magnitude
-                u64 cy = 0;
-                mulfx_mag(Row{CL + (i64)r * WS}, Row{CL + (i64)c * WS}, 0, WL, LF, [&](int jw, u32 w) {
-                    const u64 q = (u64)aij[jw] + w + cy;
-                    aij[jw] = (u32)q;
-                    cy = q >> 32;
-                });
-            } else {
-                i64 br = 0;
-                mulfx_mag(Row{CL + (i64)r * WS}, Row{CL + (i64)c * WS}, 0, WL, LF, [&](int jw, u32 w) {
-                    const i64 q = (i64)aij[jw] - (i64)w + br;
-                    aij[jw] = (u32)q;
-                    br = q >> 32;
-                });
+        if (wave == 0) {
+            const int r = k + 1;
+            const u32 x = lane < WL ? te(r, k)[lane] : 0u;
+            const u32 prod = lane_mulfx(x, x, 0, WL, LF, PX, PY);
+            const u32 d = lane_add(lane < WL ? te(r, r)[lane] : 0u, ~prod, 1u, WL);
+            if (lane < WL) te(r, r)[lane] = d;
+            pivot(r, d);
+        } else if (npairs - 1 <= lane_pairs) {
+            for (int p = wave; p < npairs; p += TWAVES - 1) {
+                int ii, jj;
+                tri_pair(p, ii, jj);
+                const int r = k + 1 + ii, c = k + 1 + jj;
+                const u32 x = lane < WL ? te(r, k)[lane] : 0u, y = lane < WL ? te(c, k)[lane] : 0u;
+                const u32 prod = lane_mulfx(x, y, 0, WL, LF, PX, PY);
+                u32* aij = te(r, c);
+                const u32 nv = lane_add(lane < WL ? aij[lane] : 0u, ~prod, 1u, WL);
+                if (lane < WL) aij[lane] = nv;
+            }
+        } else {
+            for (int p = 1 + tid - 64; p < npairs; p += TT - 64) {
+                int ii, jj;
+                tri_pair(p, ii, jj);
+                const int r = k + 1 + ii, c = k + 1 + jj;
+                trail_update(CL + (i64)r * WS, CL + (i64)c * WS, TP[r], TP[c], WL, LF, te(r, c), SG[r] ^ SG[c]);
             }
         }
         __syncthreads();
@@ -695,95 +772,82 @@ __global__ void __launch_bounds__(TT) k_tile_chol_lds(u32* __restrict__ Wg, u32*
     }
 }
 
-// words of k_tile_inverse's LDS for BT, WL
+// words of k_tile_inverse's LDS for BT, WL; the row prefetch takes BT WL <= 8 TT
 __host__ __device__ inline i64 lds_inverse_words(int BT, int WL) {
-    const i64 WS = WL | 1;
-    return 3 * (i64)BT * WS + BT + 128;
+    return 4 * (i64)BT * WL + (i64)TWAVES * WL + 128 * TWAVES;
 }
+constexpr int INV_PRE = 8;  // words of the next row each thread prefetches
 
-// Column k of X = L^-1 for one tile, on one wave (the recurrences of k_tile_factor's inverse):
+// Column k of X = L^-1 for one tile (the recurrences of k_tile_factor's inverse):
 // x_kk = rinv_k; for i > k, x_ik = -mulfx(sum_{j=k}^{i-1} mulfx(l_ij, x_jk), rinv_i), every product
-// truncated, the sum exact. Step i: lane j - k forms the product for j; the lanes then add the
-// products word-parallel (lane q holds word q) and one lane-parallel product finishes x_ik.
-// Grid (nb, BT), 64 threads. X entries above the diagonal are written as zeros.
-__global__ void __launch_bounds__(64) k_tile_inverse(const u32* __restrict__ Wg, const u32* __restrict__ Rg,
+// truncated, the sum exact (modulo 2^(32 WL)). Step i: the four waves take the products j = k..i-1
+// in turn, one lane_mulfx each, and add them into a running lane sum; wave 0 adds the four sums
+// and finishes x_ik with one lane_mulfx. Row i + 1 of L is loaded while step i computes.
+// Grid (nb, BT), TT threads. X entries above the diagonal are written as zeros.
+__global__ void __launch_bounds__(TT) k_tile_inverse(const u32* __restrict__ Wg, const u32* __restrict__ Rg,
                                                     u32* __restrict__ Xg, int BT, int WL, int LF) {
     SDPV_DYNAMIC_LDS(u32, lds);
-    const int l = threadIdx.x;
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const i64 tile = blockIdx.x;
     const int k = blockIdx.y;
-    const int WS = WL | 1;
-    u32* XC = lds;                      // BT x WS: |x_jk|
-    u32* LR = XC + BT * WS;             // BT x WS: |l_ij|, j = k..i-1 (slot j)
-    u32* PR = LR + BT * WS;             // BT x WS: the products (two's complement), slot j
-    u32* XS = PR + BT * WS;             // BT: sign of x_jk
-    u32* PX = XS + BT;                  // 64 + 64: lane_mulfx operands
+    u32* XT = lds;                      // BT x WL: x_jk
+    u32* RI = XT + BT * WL;             // BT x WL: rinv
+    u32* LR = RI + BT * WL;             // 2 x BT x WL: rows of L (slot j holds l_ij), double buffered
+    u32* PART = LR + 2 * BT * WL;       // TWAVES x WL: the waves' partial sums
+    u32* PX = PART + TWAVES * WL + 128 * wave;
     u32* PY = PX + 64;
     const u32* W = Wg + tile * BT * BT * WL;
     const u32* R = Rg + tile * BT * WL;
     u32* X = Xg + tile * BT * BT * WL;
+    for (int idx = tid; idx < BT * WL; idx += TT) RI[idx] = R[idx];
     for (int i = 0; i < k; i++)
-        for (int q = l; q < WL; q += 64) X[((i64)i * BT + k) * WL + q] = 0u;
-    // x_kk = r_k
-    {
-        const u32 r = l < WL ? R[(i64)k * WL + l] : 0u;
-        const bool nr = (lane_get(r, WL - 1) >> 31) != 0;
-        const u32 rm = nr ? lane_neg(r, WL) : r;
-        if (l < WL) {
-            X[((i64)k * BT + k) * WL + l] = r;
-            XC[(i64)k * WS + l] = rm;
-        }
-        if (l == 0) XS[k] = nr;
+        for (int q = tid; q < WL; q += TT) X[((i64)i * BT + k) * WL + q] = 0u;
+    // row k + 1 into buffer 0 (entries j = k..k, at slot j)
+    if (k + 1 < BT)
+        for (int idx = tid; idx < WL; idx += TT) LR[(i64)k * WL + idx] = W[((i64)(k + 1) * BT + k) * WL + idx];
+    __syncthreads();
+    if (wave == 0 && lane < WL) {       // x_kk = r_k
+        const u32 r = RI[k * WL + lane];
+        X[((i64)k * BT + k) * WL + lane] = r;
+        XT[k * WL + lane] = r;
     }
-    wave_sync();
+    __syncthreads();
     for (int i = k + 1; i < BT; i++) {
-        // l_ij, j = k..i-1: (i - k) WL consecutive words of row i, read by all lanes
-        const int nw = (i - k) * WL;
-        const u32* src = W + ((i64)i * BT + k) * WL;
-        for (int idx = l; idx < nw; idx += 64) {
-            const int e = idx / WL, q = idx - e * WL;
-            LR[(i64)(k + e) * WS + q] = src[idx];
+        u32* cur = LR + ((i - k - 1) & 1) * BT * WL;
+        u32* nxt = LR + ((i - k) & 1) * BT * WL;
+        // prefetch row i + 1: (i + 1 - k) WL words from entry (i + 1, k)
+        u32 pre[INV_PRE];
+        const int nw = i + 1 < BT ? (i + 1 - k) * WL : 0;
+        const u32* src = W + ((i64)(i + 1) * BT + k) * WL;
+#pragma unroll
+        for (int u = 0; u < INV_PRE; u++) {
+            const int idx = tid + u * TT;
+            pre[u] = idx < nw ? src[idx] : 0u;
         }
-        const u32 ri = l < WL ? R[(i64)i * WL + l] : 0u;
-        wave_sync();
-        const int j = k + l;
-        if (j < i) {  // product of slot j
-            u32* lr = LR + (i64)j * WS;
-            u32* pr = PR + (i64)j * WS;
-            // |l_ij| in place: the magnitude is formed word by word ahead of its first read
-            const bool neg = mag_copy(lr, lr, WL) ^ (XS[j] != 0);
-            u64 c = 1;
-            mulfx_mag(Row{lr}, Row{XC + (i64)j * WS}, 0, WL, LF, [&](int q, u32 w) {
-                if (neg) {
-                    const u64 t = (u64)(~w) + c;
-                    w = (u32)t;
-                    c = t >> 32;
-                }
-                pr[q] = w;
-            });
+        // this wave's products, summed exactly
+        u32 acc = 0u;
+        for (int j = k + wave; j < i; j += TWAVES) {
+            const u32 lij = lane < WL ? cur[j * WL + lane] : 0u;
+            const u32 xjk = lane < WL ? XT[j * WL + lane] : 0u;
+            acc = lane_add(acc, lane_mulfx(lij, xjk, 0, WL, LF, PX, PY), 0u, WL);
         }
-        wave_sync();
-        // acc = sum of the products (mod 2^(32 WL)), word q on lane q
-        u64 s = 0;
-        if (l < WL)
-            for (int jj = k; jj < i; jj++) s += PR[(i64)jj * WS + l];
-        const u64 hb = from_below64(s >> 32);                 // the high part of word q - 1
-        const u64 d = (s & 0xffffffffull) + hb;               // < 2^33
-        const u64 G = __ballot(l < WL && (d >> 32) != 0);
-        const u64 P = __ballot(l < WL && (u32)d == 0xffffffffu);
-        const u64 Xm = G | P;
-        const u64 carries = (Xm + G) ^ Xm ^ G;
-        const u32 acc = l < WL ? (u32)d + (u32)((carries >> l) & 1u) : 0u;
-        // x_ik = -mulfx(acc, r_i)
-        const u32 x = lane_neg(lane_mulfx(acc, ri, 0, WL, LF, PX, PY), WL);
-        const bool nx = (lane_get(x, WL - 1) >> 31) != 0;
-        const u32 xm = nx ? lane_neg(x, WL) : x;
-        if (l < WL) {
-            X[((i64)i * BT + k) * WL + l] = x;
-            XC[(i64)i * WS + l] = xm;
+        if (lane < WL) PART[wave * WL + lane] = acc;
+#pragma unroll
+        for (int u = 0; u < INV_PRE; u++) {
+            const int idx = tid + u * TT;
+            if (idx < nw) nxt[k * WL + idx] = pre[u];
         }
-        if (l == 0) XS[i] = nx;
-        wave_sync();
+        __syncthreads();
+        if (wave == 0) {
+            u32 sum = lane < WL ? PART[lane] : 0u;
+            for (int w = 1; w < TWAVES; w++) sum = lane_add(sum, lane < WL ? PART[w * WL + lane] : 0u, 0u, WL);
+            const u32 x = lane_neg(lane_mulfx(sum, lane < WL ? RI[i * WL + lane] : 0u, 0, WL, LF, PX, PY), WL);
+            if (lane < WL) {
+                X[((i64)i * BT + k) * WL + lane] = x;
+                XT[i * WL + lane] = x;
+            }
+        }
+        __syncthreads();
     }
 }
 
@@ -853,6 +917,15 @@ __global__ void __launch_bounds__(256) k_tile_narrow(const u32* __restrict__ W, 
 
 inline unsigned nblk(i64 n, int tpb) { return (unsigned)((n + tpb - 1) / tpb); }
 
+// trailing updates with at most this many pairs run one pair per wave (SDPV_TILE_LANE_PAIRS)
+inline int tile_lane_pairs() {
+    static const int v = [] {
+        const char* e = getenv("SDPV_TILE_LANE_PAIRS");
+        return e ? atoi(e) : 32;
+    }();
+    return v;
+}
+
 }  // namespace
 
 extern "C" {
@@ -881,11 +954,13 @@ int sdpv_tile_chol(const int32_t* A, int64_t L, int64_t nb, int64_t BT, int32_t*
         attr = true;
     }
     const size_t lds_f = (size_t)lds_factor_words((int)BT, WL) * 4;
-    if (WL <= 64 && lds_f <= 160 * 1024) {  // the tile stays in LDS; the inverse one wave per column
-        hipLaunchKernelGGL(k_tile_chol_lds, dim3((unsigned)nb), dim3(TT), lds_f, st, W, RI, (int)BT, WL, (int)L);
+    const size_t lds_i = (size_t)lds_inverse_words((int)BT, WL) * 4;
+    if (WL <= 64 && lds_f <= 160 * 1024 && BT * WL <= INV_PRE * TT) {
+        // the tile stays in LDS; the inverse one workgroup per column
+        hipLaunchKernelGGL(k_tile_chol_lds, dim3((unsigned)nb), dim3(TT), lds_f, st, W, RI, (int)BT, WL, (int)L,
+                           tile_lane_pairs());
         SDPV_TRY_LAUNCH();
-        const size_t lds_i = (size_t)lds_inverse_words((int)BT, WL) * 4;
-        hipLaunchKernelGGL(k_tile_inverse, dim3((unsigned)nb, (unsigned)BT), dim3(64), lds_i, st, W, RI, X, (int)BT,
+        hipLaunchKernelGGL(k_tile_inverse, dim3((unsigned)nb, (unsigned)BT), dim3(TT), lds_i, st, W, RI, X, (int)BT,
                            WL, (int)L);
         SDPV_TRY_LAUNCH();
     } else {
