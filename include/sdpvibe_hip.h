@@ -66,6 +66,22 @@ int sdpv_badd_align(const int32_t* A, const int32_t* Bm, const int64_t* EA, cons
                     int32_t* out, int64_t* EO, int64_t L, int64_t nb, int64_t rows, int64_t cols,
                     int32_t sgn, void* stream);
 
+/* Exact products of a batch of fixed-point matrices with a few vectors, without residues: for every
+ * entry e = (b * M + i) * N + j, S = sum_k A[b, i, k] * B[b, j, k] over L-limb two's-complement
+ * operands (limb u of A[b, i, k] at A[u * planeA + b * ldAb + i * ldAr + k * ldAk], likewise B),
+ * written exactly as NW >= 2L + 2 words W[w * E + e] (E = nb * M * N), with its bit length (of ~S
+ * when S < 0) in bitlen[e]: the words and bit lengths that sdpv_rns_crt's word mode returns for the
+ * same product. K beyond 256 is split over waves, whose column sums go through `work`
+ * (sdpv_exact_dot_work(...) int64 entries; 0 when K <= 256). No counterpart in the reference, which
+ * always goes through residues: the direct path for products with one or two right-hand-side
+ * columns (triangular solves with one vector), where the residue path's fixed costs dominate.
+ * L <= 63. */
+int64_t sdpv_exact_dot_work(int64_t L, int64_t nb, int64_t M, int64_t N, int64_t K);
+int sdpv_exact_dot(const int32_t* A, int64_t planeA, int64_t ldAb, int64_t ldAr, int64_t ldAk,
+                   const int32_t* B, int64_t planeB, int64_t ldBb, int64_t ldBr, int64_t ldBk,
+                   int64_t L, int64_t nb, int64_t M, int64_t N, int64_t K, int32_t* W, int64_t NW,
+                   int64_t* bitlen, int64_t* work, int64_t nwork, void* stream);
+
 /* ---------------------------------------------------------------- per-entry floats (MPF) */
 
 /* top[e] = e + bitlen(m) (bitlen of ~m for m < 0), or SDPV_EZ when e == SDPV_EZ.

@@ -20,6 +20,9 @@ SIGNATURES = {
     "sdpv_add_same": ([_P, _P, _P, _I64, _I64, _I32, _P], _I32),
     "sdpv_bitlen": ([_P, _I64, _I64, _P, _P], _I32),
     "sdpv_badd_align": ([_P, _P, _P, _P, _P, _P, _I64, _I64, _I64, _I64, _I32, _P], _I32),
+    "sdpv_exact_dot_work": ([_I64, _I64, _I64, _I64, _I64], _I64),
+    "sdpv_exact_dot": ([_P, _I64, _I64, _I64, _I64, _P, _I64, _I64, _I64, _I64, _I64, _I64, _I64, _I64, _I64, _P,
+                        _I64, _P, _P, _I64, _P], _I32),
     "sdpv_mpf_top": ([_P, _P, _I64, _I64, _P, _P], _I32),
     "sdpv_mpf_mul": ([_P, _P, _P, _P, _P, _P, _I64, _I64, _I32, _P], _I32),
     "sdpv_mpf_add": ([_P, _P, _P, _P, _P, _P, _I64, _I64, _I32, _P], _I32),
@@ -85,6 +88,16 @@ def ptr(t):
         raise ValueError("the HIP library takes GPU tensors only")
     if not t.is_contiguous():
         raise ValueError("the HIP library takes contiguous tensors only")
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def ptr_strided(t):
+    """Device pointer of a GPU tensor view whose strides the caller passes to the entry point
+    (a 4-d limb view: non-negative strides, unit stride along the last dimension)"""
+    if not t.is_cuda:
+        raise ValueError("the HIP library takes GPU tensors only")
+    if t.dim() != 4 or t.stride(-1) != 1 or min(t.stride()) < 0:
+        raise ValueError("strided operands are 4-d views with unit stride along the last dimension")
     return ctypes.c_void_p(t.data_ptr())
 
 

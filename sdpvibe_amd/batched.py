@@ -11,7 +11,7 @@ array instead of being concatenated level by level.
 """
 import torch
 
-from ._lib import call, ptr, stream
+from ._lib import call, lib, ptr, ptr_strided, stream
 from .limbops import _errbuf, add_same, shift_round
 from .mpmat import ACC_BUDGET, ctx, headroom_bits, residue_gemm
 from .rns import dev
@@ -114,12 +114,13 @@ def bgemm_nt(A, Bt, out_exp="col", kc=4096, bt_rowexp=None):
     L = A.L
     R = ctx(L)
     eA = A.exp if isinstance(A.exp, int) else A.exp[:, 0]
-    Al = A.limbs.contiguous()
-    Bl = Bt.limbs.contiguous()
-    same = Al.data_ptr() == Bl.data_ptr()
-    fa = _residue_desc(R, Al, nb, M, K)
-    fb = _residue_desc(R, Bl, nb, N, K)
     cache = {}
+    if N > NARROW or L > 63:
+        Al = A.limbs.contiguous()
+        Bl = Bt.limbs.contiguous()
+        same = Al.data_ptr() == Bl.data_ptr()
+        fa = _residue_desc(R, Al, nb, M, K)
+        fb = _residue_desc(R, Bl, nb, N, K)
 
     def fa0(k0, kc_, kp, j0, jw):
         cache["a"] = fa(k0, kc_, kp, None, None)
@@ -142,14 +143,56 @@ def bgemm_nt(A, Bt, out_exp="col", kc=4096, bt_rowexp=None):
     out = torch.empty((L, nb, M, N), dtype=torch.int32, device=dev)
     fixed = isinstance(e_in, int) and isinstance(out_exp, int)
     exps = None if fixed else torch.empty((nb, N), dtype=torch.int64, device=dev)
-    for j0, jw, acc in residue_gemm(R, fa0, fb0, nb, M, N, K, ACC_BUDGET):
-        _bcrt_round_panel(R, acc, nb, M, N, j0, jw, e_in, out_exp, L, out, exps)
+    if N <= NARROW and L <= 63:
+        W, bl = _exact_narrow(A.limbs, Bt.limbs, nb, M, N, K, L)
+        _round_exact(W, bl, nb, M, N, e_in, out_exp, L, out, exps)
+    else:
+        for j0, jw, acc in residue_gemm(R, fa0, fb0, nb, M, N, K, ACC_BUDGET):
+            _bcrt_round_panel(R, acc, nb, M, N, j0, jw, e_in, out_exp, L, out, exps)
     if fixed or isinstance(out_exp, int):
         return BMP(out, out_exp)
     C = BMP(out, exps)
     if out_exp == "mat":
         C = brescale(C, exps.amax(1, keepdim=True).expand(nb, N).contiguous())
     return C
+
+
+# products with at most this many right-hand-side columns are formed directly (sdpv_exact_dot)
+NARROW = 2
+
+
+def _exact_narrow(Al, Bl, nb, M, N, K, L):
+    """the exact products of (L, nb, M, K) and (L, nb, N, K) limb tensors (views with unit stride
+    along k are read in place) as 2L + 2 words per entry (entry (b*M + i)*N + j) and bit lengths:
+    what the reconstruction's word mode gives for the same product"""
+    Al = Al if Al.stride(-1) == 1 else Al.contiguous()
+    Bl = Bl if Bl.stride(-1) == 1 else Bl.contiguous()
+    NW, E = 2 * L + 2, nb * M * N
+    W = torch.empty((NW, E), dtype=torch.int32, device=Al.device)
+    bl = torch.empty((E,), dtype=torch.int64, device=Al.device)
+    nwork = int(lib().sdpv_exact_dot_work(L, nb, M, N, K))
+    work = torch.empty((max(nwork, 1),), dtype=torch.int64, device=Al.device)
+    sa, sb = Al.stride(), Bl.stride()
+    call("sdpv_exact_dot", ptr_strided(Al), sa[0], sa[1], sa[2], sa[3], ptr_strided(Bl), sb[0], sb[1], sb[2], sb[3],
+         L, nb, M, N, K, ptr(W), NW, ptr(bl), ptr(work), nwork, stream())
+    return W, bl
+
+
+def _round_exact(W, bl, nb, M, N, e_in, out_exp, L, out, exps):
+    """the roundings of _bcrt_round_panel on exact words: a fixed output exponent, or per
+    (block, column) headroom_bits(L) below the column's largest bit length"""
+    if isinstance(out_exp, int):
+        if isinstance(e_in, int):
+            s = torch.full((nb * M * N,), out_exp - e_in, dtype=torch.int64, device=dev)
+        else:
+            s = (out_exp - e_in)[:, None, :].expand(nb, M, N).reshape(-1).contiguous()
+            exps[:, :] = out_exp
+    else:
+        ein = e_in if torch.is_tensor(e_in) else torch.full((nb, N), e_in, dtype=torch.int64, device=dev)
+        sj = bl.view(nb, M, N).amax(1) - headroom_bits(L)
+        exps[:, :] = ein + sj
+        s = sj[:, None, :].expand(nb, M, N).reshape(-1).contiguous()
+    out.copy_(shift_round(W, s, L).view(L, nb, M, N))
 
 
 def _bcrt_round_panel(R, acc, nb, M, N, j0, jw, e_in, out_exp, L, out, exps):
@@ -297,6 +340,13 @@ def _stack_common(P1, P2):
     return BMP(torch.cat([brescale(P1, ee).limbs, brescale(P2, ee).limbs], 2), ee)
 
 
+def _view(A, rows, cols, transpose=False):
+    """a sub-block of the uniform-exponent factor as a view (no copy); bgemm_nt copies what it
+    needs contiguous"""
+    limbs = A.limbs[:, :, rows, cols]
+    return BMP(limbs.transpose(2, 3) if transpose else limbs, A.exp)
+
+
 def _left_solve(node, Lf, B, tiles, refine, upper):
     """L^-1 B (upper=False, forward: head first) or L^-T B (upper=True, backward: tail first)"""
     if node.head is None:
@@ -306,14 +356,17 @@ def _left_solve(node, Lf, B, tiles, refine, upper):
             return _apply_tile(XT, t.Lmp.t(), B, refine)
         return _apply_tile(t.X, t.Lmp, B, refine)
     k = node.k
+    head, tail = slice(None, k), slice(k, None)
     if not upper:
-        P1 = _left_solve(node.head, Lf[:k, :k], B[:k, :], tiles, refine, upper)
-        coupled = bgemm_nt(Lf[k:, :k], _as_rows(P1), out_exp="col", bt_rowexp=P1.expt())
-        P2 = _left_solve(node.tail, Lf[k:, k:], badd(B[k:, :], coupled, -1, out_exp="align"), tiles, refine, upper)
+        P1 = _left_solve(node.head, _view(Lf, head, head), B[:k, :], tiles, refine, upper)
+        coupled = bgemm_nt(_view(Lf, tail, head), _as_rows(P1), out_exp="col", bt_rowexp=P1.expt())
+        P2 = _left_solve(node.tail, _view(Lf, tail, tail), badd(B[k:, :], coupled, -1, out_exp="align"), tiles,
+                         refine, upper)
     else:
-        P2 = _left_solve(node.tail, Lf[k:, k:], B[k:, :], tiles, refine, upper)
-        coupled = bgemm_nt(Lf[k:, :k].t(), _as_rows(P2), out_exp="col", bt_rowexp=P2.expt())
-        P1 = _left_solve(node.head, Lf[:k, :k], badd(B[:k, :], coupled, -1, out_exp="align"), tiles, refine, upper)
+        P2 = _left_solve(node.tail, _view(Lf, tail, tail), B[k:, :], tiles, refine, upper)
+        coupled = bgemm_nt(_view(Lf, tail, head, transpose=True), _as_rows(P2), out_exp="col", bt_rowexp=P2.expt())
+        P1 = _left_solve(node.head, _view(Lf, head, head), badd(B[:k, :], coupled, -1, out_exp="align"), tiles,
+                         refine, upper)
     return _stack_common(P1, P2)
 
 

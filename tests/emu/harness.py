@@ -47,13 +47,20 @@ def install():
         assert not t.is_cuda and t.is_contiguous()
         return ctypes.c_void_p(t.data_ptr())
 
+    def emu_ptr_strided(t):
+        assert not t.is_cuda and t.dim() == 4 and t.stride(-1) == 1 and min(t.stride()) >= 0
+        return ctypes.c_void_p(t.data_ptr())
+
     torch.cuda.current_device = lambda: 0
     for mod in (limbops, mpf, mpmat, rns, batched, basecase, solver, steplen, sdp):
         if hasattr(mod, "ptr"):
             mod.ptr = emu_ptr
+        if hasattr(mod, "ptr_strided"):
+            mod.ptr_strided = emu_ptr_strided
         if hasattr(mod, "stream"):
             mod.stream = lambda: None
         if hasattr(mod, "dev"):
             mod.dev = "cpu"
     _lib.ptr = emu_ptr
+    _lib.ptr_strided = emu_ptr_strided
     return h

@@ -96,6 +96,37 @@ def bgemm_cases():
 CASES["bgemm"] = bgemm_cases
 
 
+def bgemm_narrow_cases():
+    """bgemm_nt with one or two right-hand-side columns (the direct exact products) against the oracle:
+    every output mode, per-column input exponents, k over several chunks, extreme limb values"""
+    from sdpvibe_amd import batched as Bm
+    from tests.intutil import to_planes as tp
+    ok = True
+    for L, nb, M, N, K, seed in ((3, 2, 9, 1, 70, 1), (3, 1, 5, 2, 600, 2), (24, 1, 4, 1, 66, 3), (2, 3, 1, 2, 1, 4)):
+        rng = random.Random(seed)
+        top = 1 << (32 * L - 1)
+        pick = lambda: rng.choice([rng.randrange(-top, top), top - 1, -top, 0, -1, 1 << rng.randrange(32 * L - 1)])
+        A = tp([pick() for _ in range(nb * M * K)], L).reshape(L, nb, M, K)
+        Bt = tp([pick() for _ in range(nb * N * K)], L).reshape(L, nb, N, K)
+        eA = -(32 * L - 8)
+        eB = np.array([[rng.randrange(-40, 40) - 32 * L for _ in range(N)] for _ in range(nb)], np.int64)
+        e_in = eA + eB
+        for mode, om in (("col", 1), ("mat", 2), (17, 0)):
+            C = Bm.bgemm_nt(Bm.BMP(torch.from_numpy(A), eA), Bm.BMP(torch.from_numpy(Bt), 0), out_exp=mode,
+                            bt_rowexp=torch.from_numpy(eB))
+            ref, rexp = O.bgemm_nt(A, Bt, e_in, om, np.full((nb, N), 17, np.int64) if om == 0 else None)
+            good = np.array_equal(C.limbs.numpy(), ref)
+            if om != 0:
+                good &= np.array_equal(C.exp.numpy(), rexp)
+            print(f"bgemm_nt narrow L={L} nb={nb} M={M} N={N} K={K} mode={mode}: {'ok' if good else 'MISMATCH'}",
+                  flush=True)
+            ok &= good
+    return ok
+
+
+CASES["bgemm_narrow"] = bgemm_narrow_cases
+
+
 def tile_case(L, nb, BT, logc, seed):
     """sdpv_tile_chol against the oracle's chol_tiles/trinv_tiles restatement, bit for bit"""
     from sdpvibe_amd.basecase import tile_factor

@@ -28,7 +28,8 @@ def _fx(rng, L, shape, bits):
     return to_planes([rng.randrange(-(1 << bits), 1 << bits) for _ in range(n)], L).reshape((L,) + shape)
 
 
-@pytest.mark.parametrize("L,nb,M,N,K", [(3, 2, 4, 5, 6), (24, 3, 32, 32, 32), (24, 5, 40, 7, 96), (34, 2, 64, 100, 33)])
+@pytest.mark.parametrize("L,nb,M,N,K", [(3, 2, 4, 5, 6), (24, 3, 32, 32, 32), (24, 5, 40, 7, 96), (34, 2, 64, 100, 33),
+                                         (24, 3, 40, 1, 300), (32, 2, 17, 2, 65), (5, 4, 1, 1, 1)])
 @pytest.mark.parametrize("mode", ["fixed", "col", "mat", "rowexp_col", "rowexp_fixed"])
 def test_bgemm_nt(L, nb, M, N, K, mode):
     rng = random.Random(L * 100 + M + N + K + len(mode))
@@ -122,3 +123,28 @@ def test_bchol_tile_inverse_exponents():
         s = np.full(BT * BT, int(ex[b, 0]) + 32 * L, np.int64)
         ref, _ = O.shift_round(to_planes(xs, L + 24), s, L)
         _eq(t.X.limbs[:, b].reshape(L, -1).cpu().numpy(), ref, "bchol_tile X")
+
+
+@pytest.mark.parametrize("L,nb,M,N,K", [(24, 1, 600, 1, 1500), (24, 60, 96, 1, 96), (32, 2, 100, 2, 3000),
+                                         (63, 1, 7, 1, 64), (3, 300, 32, 1, 32)])
+def test_narrow_products_equal_the_residue_path(L, nb, M, N, K, monkeypatch):
+    """sdpv_exact_dot (one or two right-hand-side columns) gives bitwise the residue path's results,
+    also from a strided view of the left operand (a sub-block of a larger factor) and at extreme
+    limb values"""
+    rng = random.Random(L + M + N + K)
+    top = 1 << (32 * L - 1)
+    vals = lambda n: [rng.choice([rng.randrange(-top, top), top - 1, -top, 0, -1]) for _ in range(n)]
+    big = to_planes(vals(nb * (M + 3) * (K + 5)), L).reshape(L, nb, M + 3, K + 5)
+    Bt = to_planes(vals(nb * N * K), L).reshape(L, nb, N, K)
+    rowexp = cu(np.array([[rng.randrange(-900, 900) - 32 * L for _ in range(N)] for _ in range(nb)], np.int64))
+    Av = Bm.BMP(cu(big)[:, :, 2:2 + M, 5:5 + K], -(32 * L - 8))            # strided view
+    Bb = Bm.BMP(cu(Bt), 0)
+    fixed = int(rowexp.max()) - (32 * L - 8) + 32 * L + 48                  # fits in L limbs
+    for mode in ("col", fixed):
+        direct = Bm.bgemm_nt(Av, Bb, out_exp=mode, bt_rowexp=rowexp)
+        monkeypatch.setattr(Bm, "NARROW", 0)
+        resid = Bm.bgemm_nt(Bm.BMP(Av.limbs.contiguous(), Av.exp), Bb, out_exp=mode, bt_rowexp=rowexp)
+        monkeypatch.setattr(Bm, "NARROW", 2)
+        _eq(direct.limbs.cpu().numpy(), resid.limbs.cpu().numpy(), f"narrow product limbs ({mode})")
+        if mode == "col":
+            _eq(direct.exp.cpu().numpy(), resid.exp.cpu().numpy(), "narrow product exponents")
