@@ -67,6 +67,8 @@ def parse():
     ap.add_argument("--no-batched", action="store_true", help="skip the uneven-block Cholesky comparison")
     ap.add_argument("--blocks", type=int, default=400)
     ap.add_argument("--no-gram", action="store_true", help="skip the Gram accumulation measurement")
+    ap.add_argument("--no-ipm", action="store_true",
+                    help="skip the IPM iterations (for counter runs of the GEMM leg; value is then null)")
     return ap.parse_args()
 
 
@@ -147,14 +149,27 @@ TRAFFIC_PROFILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "prof
                                "mpgemm_512_L24_hbm_traffic.md")
 
 
+def gemm_source_hash():
+    """64-bit hash of the exact GEMM's kernel sources (rns.hip, common.hpp, the C ABI header); the
+    library loader guarantees that the loaded library was built from these files"""
+    import hashlib
+    h = hashlib.sha256()
+    for f in ("include/sdpvibe_hip.h", "sdpvibe_amd/csrc/common.hpp", "sdpvibe_amd/csrc/rns.hip"):
+        with open(os.path.join(ROOT, f), "rb") as fh:
+            h.update(fh.read())
+    return "%#x" % int(h.hexdigest()[:16], 16)
+
+
 def measured_traffic(kernel, config, n, L):
     """HBM bytes per launch of `kernel` (product tile configuration `config`) at this workload, from
-    the rocprofv3 FETCH_SIZE/WRITE_SIZE summary, if it was measured on this library's sources;
-    None otherwise. Lines: `traffic <kernel> config=<c> n=<n> L=<L> bytes=<B> source=<hash>`."""
+    the rocprofv3 FETCH_SIZE/WRITE_SIZE summary, if it was measured on these kernel sources
+    (gemm_source_hash); None otherwise.
+    Lines: `traffic <kernel> config=<c> n=<n> L=<L> bytes=<B> source=<hash>`."""
     from sdpvibe_amd import _lib
     if not os.path.exists(TRAFFIC_PROFILE):
         return None
-    src = "%#x" % _lib.lib().sdpv_source_hash()
+    _lib.lib()
+    src = gemm_source_hash()
     pat = re.compile(r"^traffic (\S+) config=(-?\d+) n=(\d+) L=(\d+) bytes=(\d+) source=(0x[0-9a-f]+)\s*$")
     for line in open(TRAFFIC_PROFILE):
         m = pat.match(line.strip())
@@ -409,12 +424,13 @@ def main():
     import torch
     torch.cuda.set_device(0)
     L = args.L
-    ipm = ipm_leg(args, L)
+    ipm = None if args.no_ipm else ipm_leg(args, L)
     gemm, roof = gemm_leg(args)
     cpu = None if args.no_cpu_baseline else cpu_baseline(args.n, args.cpu_threads, L)
     line = {
-        "metric": METRIC, "value": round(ipm["s_per_iteration"], 4), "unit": "s/IPM iteration",
-        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": ipm["s_per_iteration"] * 1e3,
+        "metric": METRIC, "value": round(ipm["s_per_iteration"], 4) if ipm else None, "unit": "s/IPM iteration",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": ipm["s_per_iteration"] * 1e3 if ipm else None,
         "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "int8 (exact 768-bit)",
         "data": "synthetic (strictly feasible polynomial-matrix SDP from host PCG64 data, seed 0)",
         "config": {"workload": "Mehrotra predictor-corrector IPM iteration, polynomial-matrix SDP: 300 dim-1 + 60 "
