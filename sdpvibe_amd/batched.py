@@ -115,7 +115,7 @@ def bgemm_nt(A, Bt, out_exp="col", kc=4096, bt_rowexp=None):
     R = ctx(L)
     eA = A.exp if isinstance(A.exp, int) else A.exp[:, 0]
     cache = {}
-    if N > NARROW or L > 63:
+    if not _narrow(M, N, K, L):
         Al = A.limbs.contiguous()
         Bl = Bt.limbs.contiguous()
         same = Al.data_ptr() == Bl.data_ptr()
@@ -143,7 +143,7 @@ def bgemm_nt(A, Bt, out_exp="col", kc=4096, bt_rowexp=None):
     out = torch.empty((L, nb, M, N), dtype=torch.int32, device=dev)
     fixed = isinstance(e_in, int) and isinstance(out_exp, int)
     exps = None if fixed else torch.empty((nb, N), dtype=torch.int64, device=dev)
-    if N <= NARROW and L <= 63:
+    if _narrow(M, N, K, L):
         W, bl = _exact_narrow(A.limbs, Bt.limbs, nb, M, N, K, L)
         _round_exact(W, bl, nb, M, N, e_in, out_exp, L, out, exps)
     else:
@@ -157,8 +157,16 @@ def bgemm_nt(A, Bt, out_exp="col", kc=4096, bt_rowexp=None):
     return C
 
 
-# products with at most this many right-hand-side columns are formed directly (sdpv_exact_dot)
+# products with at most NARROW right-hand-side columns and at most NARROW_MK rows times inner length
+# are formed directly (sdpv_exact_dot); larger ones go through residues, which are faster there
+# (one MI355X, L = 24, one column: direct 0.085 / 0.138 / 0.221 ms against residues 0.135 / 0.139 /
+# 0.140 ms at 32^2 / 94^2 / 188^2)
 NARROW = 2
+NARROW_MK = 16384
+
+
+def _narrow(M, N, K, L):
+    return N <= NARROW and M * K <= NARROW_MK and L <= 63
 
 
 def _exact_narrow(Al, Bl, nb, M, N, K, L):

@@ -109,6 +109,57 @@ class Timers:
         return dict(self.acc)
 
 
+class GraphedQ:
+    """Chol(Q) and the one-vector solves with its factor as HIP graphs (graph.Captured), captured on
+    first use and replayed in later iterations: Q and the right-hand side are copied into the
+    captured buffers, results are cloned out of the graph's buffers. N is fixed for a solver, so the
+    launch sequence is too; every replay is bitwise the eager computation. Attributes not defined
+    here (Lt, k, n, not_pd, ...) are those of the captured factor."""
+
+    def __init__(self, Q):
+        from .graph import Captured
+        self._fac = None
+        self.Qs = MPF(Q.m.clone(), Q.e.clone())
+        self._chol = Captured(lambda A: Chol(A, check=False), self.Qs)
+        self._fac = self._chol.out
+        self.rs = F.zeros((Q.shape[0], 1), Q.L)
+        self._solves = {}
+
+    def __getattr__(self, name):
+        fac = self.__dict__.get("_fac")
+        if fac is None:
+            raise AttributeError(name)
+        return getattr(fac, name)
+
+    def factor(self, Q):
+        self.Qs.m.copy_(Q.m)
+        self.Qs.e.copy_(Q.e)
+        self._chol.replay()
+        if bool(self._fac.not_pd):
+            raise ValueError("matrix is not positive definite (diagonal)")
+        return self
+
+    def _solve(self, which, v):
+        if tuple(v.shape) != tuple(self.rs.shape):
+            return getattr(self._fac, which)(v)
+        if which not in self._solves:
+            from .graph import Captured
+            self._solves[which] = Captured(lambda r: getattr(self._fac, which)(r), self.rs)
+        self.rs.m.copy_(v.m)
+        self.rs.e.copy_(v.e)
+        out = self._solves[which].replay()
+        return MPF(out.m.clone(), out.e.clone())
+
+    def solve_lower(self, v):
+        return self._solve("solve_lower", v)
+
+    def solve_upper(self, v):
+        return self._solve("solve_upper", v)
+
+    def solve(self, v):
+        return self._solve("solve", v)
+
+
 class _State:
     """per-group iterate: x (nb, P, 1), X and Y (2 nb, n, n)"""
     __slots__ = ("x", "X", "Y")
@@ -142,6 +193,9 @@ class Solver:
         self.final_factors = None
         self.timers = Timers()
         self.iterations_done, self.alphaP, self.alphaD = 0, mp.mpf(0), mp.mpf(0)
+        # chol(Q) and the solves with its factor replayed from HIP graphs after the first iteration
+        self.graphs = os.environ.get("SDPB_GPU_GRAPHS", "1") != "0"
+        self._qgraph = None
 
     # ---------------------------------------------------------------- block operators
     def constraint_sum(self, g, a):
@@ -269,7 +323,12 @@ class Solver:
             acc.add(stack_rows(w["P"]))
         Q = mirror_lower(acc.result())
         TM("chol(Q)")
-        cQ = Chol(Q)
+        if self.graphs and self.iterations_done >= 1:    # the first iteration runs eagerly (autotuner)
+            if self._qgraph is None:
+                self._qgraph = GraphedQ(Q)
+            cQ = self._qgraph.factor(Q)
+        else:
+            cQ = Chol(Q)
         qc = mp.mpf(2) ** mp.mpf(float(chol_log2_cond(cQ)[0]))
         return cQ, qc, cond
 

@@ -141,8 +141,9 @@ def test_narrow_products_equal_the_residue_path(L, nb, M, N, K, monkeypatch):
     Bb = Bm.BMP(cu(Bt), 0)
     fixed = int(rowexp.max()) - (32 * L - 8) + 32 * L + 48                  # fits in L limbs
     for mode in ("col", fixed):
+        monkeypatch.setattr(Bm, "NARROW_MK", 1 << 40)                      # the direct path at every size
         direct = Bm.bgemm_nt(Av, Bb, out_exp=mode, bt_rowexp=rowexp)
-        monkeypatch.setattr(Bm, "NARROW", 0)
+        monkeypatch.setattr(Bm, "NARROW", 0)                               # the residue path
         resid = Bm.bgemm_nt(Bm.BMP(Av.limbs.contiguous(), Av.exp), Bb, out_exp=mode, bt_rowexp=rowexp)
         monkeypatch.setattr(Bm, "NARROW", 2)
         _eq(direct.limbs.cpu().numpy(), resid.limbs.cpu().numpy(), f"narrow product limbs ({mode})")

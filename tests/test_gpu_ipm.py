@@ -73,6 +73,23 @@ def test_batched_equals_plain():
     assert torch.equal(A.y.m, B.y.m) and torch.equal(A.y.e, B.y.e)
 
 
+def test_graph_replays_equal_eager_iterations():
+    """chol(Q) and the solves with its factor replayed from HIP graphs (iterations 2..6: five replays,
+    eager work and host synchronisations in between) give bitwise the eager iterations"""
+    sd = SDP.polynomial(n1=4, n3=2, degree=8, N=40, L=24, seed=5)
+    A, B = Solver(sd, 24), Solver(sd, 24)
+    A.graphs, B.graphs = True, False
+    A.run(max_iter=6)
+    B.run(max_iter=6)
+    assert A._qgraph is not None and B._qgraph is None
+    for ra, rb in zip(A.log, B.log):
+        for k in FIELDS.values():
+            assert ra[k] == rb[k], (ra["iteration"], k)
+    assert torch.equal(A.y.m, B.y.m) and torch.equal(A.y.e, B.y.e)
+    for sa, sb in zip(A.state, B.state):
+        assert torch.equal(sa.X.m, sb.X.m) and torch.equal(sa.x.m, sb.x.m)
+
+
 def test_known_answer_sdp():
     bl, b, opt = C.two_by_two()
     L = 8
