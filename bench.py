@@ -69,7 +69,33 @@ def parse():
     ap.add_argument("--no-gram", action="store_true", help="skip the Gram accumulation measurement")
     ap.add_argument("--no-ipm", action="store_true",
                     help="skip the IPM iterations (for counter runs of the GEMM leg; value is then null)")
+    ap.add_argument("--gemm-only", action="store_true",
+                    help="only the 512^3 GEMM leg and its roofline (for the kernel trace and counter runs of the "
+                         "roofline kernel); value is then null")
+    ap.add_argument("--chol-only", type=int, default=0, metavar="N",
+                    help="only time mpf.Chol of an N x N SPD matrix (G G^T + N I, the Schur complement's size "
+                         "in the IPM at N = 3000) twice and print its line (for profiling the tile kernels)")
     return ap.parse_args()
+
+
+def chol_only(N, L):
+    """two mpf.Chol calls on Q = G G^T + N I (G an N x N Gaussian MPF): the first includes the product
+    autotuner's runs at new shapes, the second is the steady state"""
+    import torch
+    from sdpvibe_amd import mpf as F
+    from sdpvibe_amd.synthetic import gaussian_mpf
+    gm, ge = gaussian_mpf((N, N), L, seed=3)
+    G = F.MPF(torch.from_numpy(gm).cuda(), torch.from_numpy(ge).cuda())
+    Q = F.add(F.gemm(G, G, transB=True), F.eye(N, L, val=N))
+    del G
+    times = []
+    for _ in range(2):
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        F.Chol(Q)
+        torch.cuda.synchronize()
+        times.append(round(time.perf_counter() - t, 4))
+    return {"workload": f"mpf.Chol of G G^T + {N} I ({N} x {N}, L={L})", "seconds": times}
 
 
 def gram_acc(L, N=3000, rows=1024, chunks=4):
@@ -273,16 +299,17 @@ def ipm_leg(args, L=24):
            "setup_s": round(setup, 2), "groups": len(sdp.groups), "total_psd_rows": sdp.total_psd_rows,
            "last_record": {k: solver.log[-1][k][:12] for k in ("mu", "P-obj", "D-obj", "P-step", "D-step")}}
     if not args.no_plain and args.plain_iters > 0:
+        # iterations 1 (autotuner) and 2 (graph capture) untimed in both forms
         P = args.plain_iters
         plain = sdp.relayout("plain")
-        psolver, ptimes = timed(plain, 1 + P)                # iteration 1 untimed in both forms
+        psolver, ptimes = timed(plain, 2 + P)
         same = all(ra[k] == rb[k] for ra, rb in zip(solver.log, psolver.log)
                    for k in ("mu", "P-obj", "D-obj", "P-err", "p-err", "D-err", "R-err", "P-step", "D-step", "beta"))
-        bt = sum(times[1:1 + P]) / P
-        pt = sum(ptimes[1:]) / P
+        bt = sum(times[2:2 + P]) / P
+        pt = sum(ptimes[2:]) / P
         out["plain"] = {"groups": len(plain.groups), "s_per_iteration": round(pt, 4),
                         "batched_s_per_iteration_same_iterations": round(bt, 4), "speedup": round(pt / bt, 2),
-                        "iterations": f"2..{1 + P}", "records_bitwise_equal": bool(same)}
+                        "iterations": f"3..{2 + P}", "records_bitwise_equal": bool(same)}
     return out
 
 
@@ -424,6 +451,11 @@ def main():
     import torch
     torch.cuda.set_device(0)
     L = args.L
+    if args.chol_only:
+        print(json.dumps({"chol_only": chol_only(args.chol_only, L)}))
+        return
+    if args.gemm_only:
+        args.no_ipm = args.no_cpu_baseline = args.no_batched = args.no_gram = True
     ipm = None if args.no_ipm else ipm_leg(args, L)
     gemm, roof = gemm_leg(args)
     cpu = None if args.no_cpu_baseline else cpu_baseline(args.n, args.cpu_threads, L)
@@ -437,7 +469,7 @@ def main():
                                "dim-3 positivity constraints, degree 26, N=3000, 768-bit (L=24), batched layout",
                    "L": L, "N": 3000, "blocks": 360},
         "ipm": ipm, "mp_gemm": gemm, "roofline": roof, "cpu_baseline": cpu,
-        "plain_chol": _guard(plain_chol, args.n, L),
+        "plain_chol": None if args.gemm_only else _guard(plain_chol, args.n, L),
         "batched_chol": None if args.no_batched else _guard(batched_chol, L, args.blocks),
         "gram_acc": None if args.no_gram else _guard(gram_acc, L),
     }

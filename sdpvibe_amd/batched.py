@@ -13,7 +13,7 @@ import torch
 
 from ._lib import call, lib, ptr, ptr_strided, stream
 from .limbops import _errbuf, add_same, shift_round
-from .mpmat import ACC_BUDGET, ctx, headroom_bits, residue_gemm
+from .mpmat import ctx, headroom_bits, residue_gemm
 from .rns import dev
 
 
@@ -92,13 +92,11 @@ def _residue_desc(R, X, nb, rows, K):
     L = X.shape[0]
 
     def f(k0, kc, kp, j0, jw, X=X):
-        if j0 is None or (j0, jw) == (0, rows):
-            base, rn = X, rows
-        else:
-            base, rn = X[:, :, j0:j0 + jw].contiguous(), jw
+        r0, rn = (0, rows) if j0 is None else (j0, jw)
         Rh = torch.empty((R.n, kp // 16, nb * rn, 16), dtype=torch.int8, device=X.device)
         Rl = torch.empty_like(Rh)
-        R.split(base, None, None, L, nb * rn * K, nb, rn, rn * K, K, 1, k0, kc, kp, Rh, Rl)
+        # a panel is read in place: rows r0.. of every block, offset r0*K into each limb plane
+        R.split(X, None, None, L, nb * rows * K, nb, rn, rows * K, K, 1, k0, kc, kp, Rh, Rl, off=r0 * K)
         return Rh, Rl
     return f
 
@@ -147,7 +145,7 @@ def bgemm_nt(A, Bt, out_exp="col", kc=4096, bt_rowexp=None):
         W, bl = _exact_narrow(A.limbs, Bt.limbs, nb, M, N, K, L)
         _round_exact(W, bl, nb, M, N, e_in, out_exp, L, out, exps)
     else:
-        for j0, jw, acc in residue_gemm(R, fa0, fb0, nb, M, N, K, ACC_BUDGET):
+        for j0, jw, acc in residue_gemm(R, fa0, fb0, nb, M, N, K):
             _bcrt_round_panel(R, acc, nb, M, N, j0, jw, e_in, out_exp, L, out, exps)
     if fixed or isinstance(out_exp, int):
         return BMP(out, out_exp)

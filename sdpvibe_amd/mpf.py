@@ -17,7 +17,7 @@ import torch
 
 from ._lib import call, ptr, stream
 from .limbops import bitlen_words, shift_round
-from .mpmat import ACC_BUDGET, RES_BUDGET, ctx, residue_gemm
+from .mpmat import RES_BUDGET, ctx, residue_gemm
 from .rns import KMAX_LAUNCH, dev
 
 EZ = -(1 << 40)                                      # exponent of zero
@@ -381,15 +381,12 @@ def gemm(A, B, transA=False, transB=False):
         Rl = torch.empty_like(Rh)
         if (j0, jw) == (0, N):
             R.split(Bm, Be, EB, L, Be.numel(), nb, N, bldb, bldr, bldk, k0, kc, kp, Rh, Rl)
-        else:                                  # column panel: rows j0..j0+jw of op(B)^T
+        else:                                  # column panel: rows j0..j0+jw of op(B)^T, read in place
             Eb = EB.view(nb, N)[:, j0:j0 + jw].contiguous()
-            off = j0 * bldr
-            Bms = Bm.view(L, -1)[:, off:].contiguous()
-            Bes = Be.view(-1)[off:].contiguous()
-            R.split(Bms, Bes, Eb, L, Bes.numel(), nb, jw, bldb, bldr, bldk, k0, kc, kp, Rh, Rl)
+            R.split(Bm, Be, Eb, L, Be.numel(), nb, jw, bldb, bldr, bldk, k0, kc, kp, Rh, Rl, off=j0 * bldr)
         return Rh, Rl
 
-    for j0, jw, acc in residue_gemm(R, fa, fb, nb, M, N, K, ACC_BUDGET):
+    for j0, jw, acc in residue_gemm(R, fa, fb, nb, M, N, K):
         R.crt(acc, nb * M, jw, 0, jw, 1, outm=out_m, oute=out_e, L=L, plane=nb * M * N, ldo=N, jo=j0, Mb=M,
               erow=EA, ecol=EB, scaled=True)
     return MPF(out_m, out_e)
@@ -468,12 +465,22 @@ class Chol:
 
     def _to_bmp(self, B):
         """right-hand sides (nb, n, m) -> fixed point with one exponent per (block, column), zero rows
-        below n"""
+        below n: each column rounded to its largest entry's exponent (the rows of B^T aligned, as
+        _align_rows(B.T) does, read in place instead of through two transposed copies)"""
         from .batched import BMP
-        aligned, colexp = _align_rows(B.T)                            # (L, nb, m, n), (nb, m)
-        P = torch.zeros((self.L, self.nb, self.npad, B.shape[-1]), dtype=torch.int32, device=aligned.device)
-        P[:, :, :self.n] = aligned.transpose(2, 3)
-        return BMP(P, colexp)
+        nb, n, m = B.shape
+        L = self.L
+        bm, be = B.m.contiguous(), B.e.contiguous()
+        E = _row_exp(bm, be, L, be.numel(), nb, m, n, n * m, 1, m).view(nb, m)
+        E = torch.where(E < EZ // 2, torch.zeros_like(E), E)          # all-zero columns: exponent 0
+        s = (E[:, None, :] - be).reshape(-1)
+        s = torch.where(be.reshape(-1) == EZ, torch.full_like(s, 1 << 30), s)
+        aligned = shift_round(bm.reshape(L, -1), s, L).view(L, nb, n, m)
+        if self.npad == n:
+            return BMP(aligned, E)
+        P = torch.zeros((L, nb, self.npad, m), dtype=torch.int32, device=bm.device)
+        P[:, :, :n] = aligned
+        return BMP(P, E)
 
     def _from_bmp(self, P):
         """fixed-point solution (first n rows) -> normalised MPF (nb, n, m)"""

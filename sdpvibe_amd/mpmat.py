@@ -185,12 +185,13 @@ def ctx(L):
     return _ctx[key]
 
 
-def residue_gemm(R, A_desc, B_desc, nb, M, N, K, acc_rows_budget=ACC_BUDGET):
+def residue_gemm(R, A_desc, B_desc, nb, M, N, K, acc_rows_budget=None):
     """Generator of (j0, jw, acc) panels: acc (n, nb*M, jw) int16 holding the CRT coefficients
     (centred residue * (M/m)^-1 mod m, for crt(..., scaled=True)) of the exact products over the
     whole inner dimension K. A_desc/B_desc: callables (k0, kc, kp, j0, jw) ->
     (Rh, Rl) residues (n, nb*rows, kp) of op(A) rows / op(B)^T rows j0..j0+jw."""
     n = R.n
+    acc_rows_budget = ACC_BUDGET if acc_rows_budget is None else acc_rows_budget   # read per call
     w = N if n * nb * M * N * 2 <= acc_rows_budget else max(64, acc_rows_budget // (n * nb * M * 2) // 64 * 64)
     assert K <= R.kmax, f"inner dimension {K} exceeds the RNS bound {R.kmax}"
     for j0 in range(0, N, w):
@@ -218,12 +219,10 @@ def _limb_desc(R, X, rows, K, nb=1):
 
     def f(k0, kc, kp, j0, jw, X=X):
         r0, rn = (0, rows) if j0 is None else (j0, jw)
-        Xs = X.view(L, nb, rows, K)
-        base = Xs[:, :, r0:r0 + rn] if (r0, rn) != (0, rows) else Xs
-        base = base.contiguous()
         Rh = torch.empty((R.n, kp // 16, nb * rn, 16), dtype=torch.int8, device=dev)
         Rl = torch.empty_like(Rh)
-        R.split(base, None, None, L, nb * rn * K, nb, rn, rn * K, K, 1, k0, kc, kp, Rh, Rl)
+        # a panel is read in place: rows r0.. of every block, offset r0*K into each limb plane
+        R.split(X, None, None, L, nb * rows * K, nb, rn, rows * K, K, 1, k0, kc, kp, Rh, Rl, off=r0 * K)
         return Rh, Rl
     return f
 

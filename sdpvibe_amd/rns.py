@@ -17,6 +17,7 @@ The prime set differs from the reference's (primes <= 32639, four products per p
 prime costs three INT8 products; the reconstructed integers, and hence every rounded result, are
 identical.
 """
+import ctypes
 import math
 import os
 
@@ -205,9 +206,9 @@ class RNS:
         return row_major(Rh), row_major(Rl)
 
     def split(self, X, e, Erow, Lin, plane, nb, rows, ldb, ldr, ldk, k0, kc, kp, Rh, Rl, g0=0, G=None,
-              flag=None):
+              flag=None, off=0):
         """Fused rounding shift + residue split (see sdpv_rns_split): entry (b, r, k0 + k) at
-        b*ldb + r*ldr + (k0+k)*ldk of every limb plane of X (Lin limbs) and of e, shifted by
+        off + b*ldb + r*ldr + (k0+k)*ldk of every limb plane of X (Lin limbs) and of e, shifted by
         Erow[b*rows + r] - e (none when e is None) into self.L limbs."""
         G = self.n - g0 if G is None else G
         # host-side extent checks: the kernel trusts these sizes
@@ -219,12 +220,16 @@ class RNS:
                 raise ValueError(f"split: residue planes must be contiguous int8 {(G, kp // 16, nrows, 16)}, "
                                  f"got {tuple(Rx.shape)}")
         if nrows and kc:
-            last = (nb - 1) * ldb + (rows - 1) * ldr + (k0 + kc - 1) * ldk
-            if min(ldb, ldr, ldk, k0) < 0 or last >= plane or X.numel() < Lin * plane:
+            last = off + (nb - 1) * ldb + (rows - 1) * ldr + (k0 + kc - 1) * ldk
+            if min(ldb, ldr, ldk, k0, off) < 0 or last >= plane or X.numel() < Lin * plane:
                 raise ValueError("split: entry offsets exceed the limb planes")
             if e is not None and (e.numel() < plane or Erow is None or Erow.numel() < nrows):
                 raise ValueError("split: exponent arrays too small")
-        call("sdpv_rns_split", ptr(X), ptr(e), ptr(Erow), Lin, self.L, plane, nb, rows, ldb, ldr, ldk, k0, kc,
+        Xp, ep = ptr(X), ptr(e)
+        if off:                                # the entry point reads every plane from X + off
+            Xp = ctypes.c_void_p(Xp.value + off * X.element_size())
+            ep = None if e is None else ctypes.c_void_p(ep.value + off * e.element_size())
+        call("sdpv_rns_split", Xp, ep, ptr(Erow), Lin, self.L, plane, nb, rows, ldb, ldr, ldk, k0, kc,
              kp, ptr(self.Thi), ptr(self.Tlo), self.NBp, ptr(self.C1), ptr(self.m32),
              ptr(self.minv), g0, G, ptr(Rh), ptr(Rl), ptr(flag), stream())
 

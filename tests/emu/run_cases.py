@@ -306,6 +306,45 @@ def small_round_cases():
 CASES["small_round"] = small_round_cases
 
 
+def panel_cases():
+    """mpf.gemm, mpmat.gemm_nt and bgemm_nt with column panels (in-place split offsets) and k chunks
+    forced by small budgets: bitwise the one-pass results, and mpf.gemm equal to the oracle"""
+    from sdpvibe_amd import batched as Bm
+    L, nb, M, K, N = 2, 2, 9, 150, 140
+    rng = random.Random(21)
+    a, ea = rand_mpf_ints(rng, nb * M * K, L, 60)
+    b, eb = rand_mpf_ints(rng, nb * K * N, L, 60)
+    Am, Bmm = to_planes(a, L).reshape(L, nb, M, K), to_planes(b, L).reshape(L, nb, K, N)
+    Ae, Be = np.array(ea, np.int64).reshape(nb, M, K), np.array(eb, np.int64).reshape(nb, K, N)
+    Ae[np.asarray(a).reshape(nb, M, K) == 0] = EZ
+    Be[np.asarray(b).reshape(nb, K, N) == 0] = EZ
+    A = F.MPF(torch.from_numpy(Am), torch.from_numpy(Ae))
+    B = F.MPF(torch.from_numpy(Bmm), torch.from_numpy(Be))
+    fx = lambda shape: torch.from_numpy(np.ascontiguousarray(to_planes(
+        [rng.randrange(-(1 << (32 * L - 40)), 1 << (32 * L - 40)) for _ in range(int(np.prod(shape)))], L)
+        .reshape((L,) + shape)))
+    Af, Bf = Bm.BMP(fx((nb, M, K)), -(32 * L - 40)), Bm.BMP(fx((nb, N, K)), -(32 * L - 41))
+    A2, B2 = mpmat.MP(Af.limbs[:, 0].contiguous(), -100), mpmat.MP(Bf.limbs[:, 1].contiguous(), -106)
+    one, oneb, onem = F.gemm(A, B), Bm.bgemm_nt(Af, Bf, out_exp="col"), mpmat.gemm_nt(A2, B2, out_exp="col")
+    R = F.ctx(L)
+    budgets = mpmat.ACC_BUDGET, mpmat.RES_BUDGET
+    mpmat.ACC_BUDGET, mpmat.RES_BUDGET = R.n * nb * M * 2 * 64, 2 * R.n * nb * (M + 64) * 2 * 64
+    try:
+        pan, panb, panm = F.gemm(A, B), Bm.bgemm_nt(Af, Bf, out_exp="col"), mpmat.gemm_nt(A2, B2, out_exp="col")
+    finally:
+        mpmat.ACC_BUDGET, mpmat.RES_BUDGET = budgets
+    rm, re_ = O.mpf_gemm(Am, Ae, Bmm, Be, False, False)
+    ok = (torch.equal(one.m, pan.m) and torch.equal(one.e, pan.e) and np.array_equal(pan.m.numpy(), rm)
+          and np.array_equal(pan.e.numpy(), re_) and torch.equal(oneb.limbs, panb.limbs)
+          and torch.equal(oneb.exp, panb.exp) and torch.equal(onem.limbs, panm.limbs)
+          and torch.equal(onem.exp, panm.exp))
+    print(f"column panels and k chunks: {'ok' if ok else 'MISMATCH'}", flush=True)
+    return ok
+
+
+CASES["panels"] = panel_cases
+
+
 def bucketed_cases():
     from tests.test_gpu_bucketed import bucketed_case
     bucketed_case(lambda a: torch.from_numpy(np.ascontiguousarray(a)), 2, [3, 33, 9], 2, seed=5)

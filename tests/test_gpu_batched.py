@@ -149,3 +149,40 @@ def test_narrow_products_equal_the_residue_path(L, nb, M, N, K, monkeypatch):
         _eq(direct.limbs.cpu().numpy(), resid.limbs.cpu().numpy(), f"narrow product limbs ({mode})")
         if mode == "col":
             _eq(direct.exp.cpu().numpy(), resid.exp.cpu().numpy(), "narrow product exponents")
+
+
+def test_column_panels_and_k_chunks_equal_one_pass(monkeypatch):
+    """mpf.gemm, mpmat.gemm_nt and bgemm_nt with the residue accumulator split into column panels (a
+    small ACC_BUDGET) and the inner dimension into chunks (a small RES_BUDGET) give bitwise the
+    one-pass results: the in-place panel reads (split offsets), the per-panel splits and the
+    per-panel rounding are exercised"""
+    from sdpvibe_amd import mpf as F
+    from sdpvibe_amd import mpmat
+    from sdpvibe_amd.synthetic import gaussian_mpf
+    L, nb, M, K, N = 5, 2, 70, 300, 200
+    am, ae = gaussian_mpf((nb, M, K), L, seed=31)
+    bm, be = gaussian_mpf((nb, K, N), L, seed=32)
+    A = F.MPF(torch.from_numpy(am).cuda(), torch.from_numpy(ae).cuda())
+    B = F.MPF(torch.from_numpy(bm).cuda(), torch.from_numpy(be).cuda())
+    rng = random.Random(33)
+    Af = Bm.BMP(cu(_fx(rng, L, (nb, M, K), 32 * L - 40)), -(32 * L - 40))
+    Bf = Bm.BMP(cu(_fx(rng, L, (nb, N, K), 32 * L - 40)), -(32 * L - 41))
+    A2 = mpmat.MP(Af.limbs[:, 0].contiguous(), -1000)
+    B2 = mpmat.MP(Bf.limbs[:, 1].contiguous(), -1060)
+    one = F.gemm(A, B)
+    oneb = Bm.bgemm_nt(Af, Bf, out_exp="col")
+    onem = {mode: mpmat.gemm_nt(A2, B2, out_exp=mode) for mode in ("col", "mat", -1900)}
+    R = F.ctx(L)
+    monkeypatch.setattr(mpmat, "ACC_BUDGET", R.n * nb * M * 2 * 64)               # 64-column panels
+    monkeypatch.setattr(mpmat, "RES_BUDGET", 2 * R.n * nb * (M + 64) * 2 * 64)    # 64-deep k chunks
+    panels = F.gemm(A, B)
+    panelsb = Bm.bgemm_nt(Af, Bf, out_exp="col")
+    assert torch.equal(one.m, panels.m) and torch.equal(one.e, panels.e)
+    assert torch.equal(oneb.limbs, panelsb.limbs) and torch.equal(oneb.exp, panelsb.exp)
+    for mode, C in onem.items():
+        D = mpmat.gemm_nt(A2, B2, out_exp=mode)
+        assert torch.equal(C.limbs, D.limbs), mode
+        if torch.is_tensor(C.exp):
+            assert torch.equal(C.exp, D.exp), mode
+        else:
+            assert C.exp == D.exp, mode
