@@ -1,15 +1,14 @@
 // Host emulation of the small HIP surface used by sdpvibe_amd/csrc (CPU testing only).
-// Each kernel block runs as blockDim std::threads with real barriers; MFMA and cross-lane
-// shuffles are emulated per 64-lane wave. Built with AddressSanitizer, it turns an
-// out-of-bounds global access of a kernel into a host report instead of a GPU fault.
+// Each kernel block runs as blockDim cooperative fibers with real barriers (a deadlock is
+// reported); MFMA and cross-lane shuffles are emulated per 64-lane wave. Built with
+// AddressSanitizer, it turns an out-of-bounds global access of a kernel into a host report
+// instead of a GPU fault.
 #pragma once
-#include <barrier>
 #include <cmath>
 #include <cstddef>
 #include <cstdint>
 #include <cstring>
 #include <functional>
-#include <thread>
 #include <vector>
 
 #define __global__
