@@ -40,6 +40,13 @@ int sdpv_version(void);
  * in). Loaders compare it with the sources they ship, so a stale binary is refused. */
 uint64_t sdpv_source_hash(void);
 
+/* Page-locked host memory of exactly `bytes` bytes (hipHostMalloc), NULL when it cannot be had;
+ * release it with sdpv_host_free. The streaming store keeps the SDP's large per-chunk arrays there
+ * and copies them to and from the device asynchronously. Replaces the cudaHostRegister of
+ * torch-allocated buffers in Store._alloc_host (sdpvibe/store.py:52-65). */
+void* sdpv_host_alloc(int64_t bytes);
+int sdpv_host_free(void* p);
+
 /* ---------------------------------------------------------------- limb bookkeeping */
 
 /* out = round_half_up(X / 2^s) per entry as Lout limbs (s <= 0: exact left shift).

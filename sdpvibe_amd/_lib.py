@@ -16,6 +16,8 @@ _P, _I64, _I32 = ctypes.c_void_p, ctypes.c_int64, ctypes.c_int32
 SIGNATURES = {
     "sdpv_version": ([], _I32),
     "sdpv_source_hash": ([], ctypes.c_uint64),
+    "sdpv_host_alloc": ([_I64], _P),
+    "sdpv_host_free": ([_P], _I32),
     "sdpv_shift_round": ([_P, _I64, _I64, _P, _P, _I64, _P, _I32, _P], _I32),
     "sdpv_add_same": ([_P, _P, _P, _I64, _I64, _I32, _P], _I32),
     "sdpv_bitlen": ([_P, _I64, _I64, _P, _P], _I32),

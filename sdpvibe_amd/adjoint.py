@@ -43,7 +43,7 @@ def refactor(solver):
     cQ, _, _ = solver._schur_and_q(r["work"])
     w = r["work"]
     solver.final_factors = dict(X=[v["cX"] for v in w], Y=[v["cY"] for v in w], S=[v["cS"] for v in w],
-                                P=[v["P"] for v in w], Q=cQ)
+                                P=[solver._P(g, v) for g, v in zip(solver.sdp.groups, w)], Q=cQ)
     return solver.final_factors
 
 
@@ -199,13 +199,14 @@ def perturbed(sdp, dc, dB, db, t):
     ts = F.scalar(t, L)
     new = copy.copy(sdp)
     new.groups = []
+    new.store = None                                  # the perturbed copy keeps everything on the device
     o = 0
     for g in sdp.groups:
         n = g.nb * g.P
         ng = copy.copy(g)
         sl = lambda X: MPF(X.m[:, o:o + n].contiguous(), X.e[o:o + n].contiguous())
         ng.c = add(g.c, mul(sl(dc), ts).reshape(g.nb, g.P, 1))
-        ng.B = add(g.B, mul(sl(dB), ts).reshape(g.nb, g.P, sdp.N))
+        ng.B = add(sdp.B_of(g), mul(sl(dB), ts).reshape(g.nb, g.P, sdp.N))
         new.groups.append(ng)
         o += n
     new.b = add(sdp.b, mul(db, ts))

@@ -393,6 +393,18 @@ int sdpv_version(void) { return 1; }
 #endif
 uint64_t sdpv_source_hash(void) { return (uint64_t)SDPV_SOURCE_HASH; }
 
+void* sdpv_host_alloc(int64_t bytes) {
+    if (bytes <= 0) return nullptr;
+    void* p = nullptr;
+    if (hipHostMalloc(&p, (size_t)bytes, hipHostMallocDefault) != hipSuccess) return nullptr;
+    return p;
+}
+
+int sdpv_host_free(void* p) {
+    if (!p) return SDPV_OK;
+    return hipHostFree(p) == hipSuccess ? SDPV_OK : SDPV_EARG;
+}
+
 int sdpv_shift_round(const int32_t* X, int64_t Lin, int64_t B, const int64_t* s, int32_t* out, int64_t Lout,
                      int32_t* flag, int32_t per_entry, void* stream) {
     if (B < 0 || Lin < 1 || Lout < 1) return SDPV_EARG;

@@ -25,7 +25,7 @@ import torch
 from . import mpf as F
 from .blockops import diagonal, keep, mirror_lower, parity_sum, repeat_batch, scale_columns, shift_diagonal, \
     fill_diagonal, stack_rows, twice
-from .mpf import EZ, MPF, Chol, add, gemm, max_abs, mul, symmetrize, total
+from .mpf import EZ, MPF, Chol, KSum, add, col_exps, gemm, max_abs, mul, symmetrize, total
 from .steplen import min_eig, step_from_eig
 
 DEFAULTS = dict(maxIterations=500, findPrimalFeasible=False, findDualFeasible=False,
@@ -74,11 +74,6 @@ def chol_log2_cond_parts(ch):
     """(max, min) log2 of the diagonal of L for a single-matrix Chol"""
     lg = _diag_log2(ch)
     return float(lg.max()), float(lg.min())
-
-
-def _cat_rows(parts):
-    """MPF (R_i, C) list -> (sum R_i, C)"""
-    return MPF(torch.cat([p.m for p in parts], 1), torch.cat([p.e for p in parts], 0))
 
 
 def _flat(parts):
@@ -196,6 +191,45 @@ class Solver:
         # chol(Q) and the solves with its factor replayed from HIP graphs after the first iteration
         self.graphs = os.environ.get("SDPB_GPU_GRAPHS", "1") != "0"
         self._qgraph = None
+        self._b_exps = None                                  # column exponents of all of B (constant)
+
+    # ---------------------------------------------------------------- streamed per-group arrays
+    def _groups_prefetch(self, fetch):
+        """(group, its index) in order, calling fetch(next group) before each group is used, so a
+        streamed array of the next group is copied in while the current one is worked on"""
+        groups = self.sdp.groups
+        for i, g in enumerate(groups):
+            if i + 1 < len(groups):
+                fetch(groups[i + 1])
+            yield i, g
+
+    def _streamed(self, name, g):
+        st = self.sdp.store
+        return st is not None and st.is_host((name, g.gid))
+
+    def _keep_P(self, g, w, P):
+        if self._streamed("P", g):
+            self.sdp.store.put(("P", g.gid), P)
+        else:
+            w["P"] = P
+
+    def _P(self, g, w):
+        """P = L_S^-1 B of the group on the device (resident, or copied in from host memory)"""
+        return self.sdp.store.get(("P", g.gid)) if self._streamed("P", g) else w["P"]
+
+    def _prefetch_P(self, g):
+        if self._streamed("P", g):
+            self.sdp.store.prefetch(("P", g.gid))
+
+    def _B_exps(self):
+        """row exponents of B^T over all blocks (B is constant: computed once)"""
+        if self._b_exps is None:
+            e = None
+            for _, g in self._groups_prefetch(self.sdp.prefetch_B):
+                t = col_exps(stack_rows(self.sdp.B_of(g)))
+                e = t if e is None else torch.maximum(e, t)
+            self._b_exps = e
+        return self._b_exps
 
     # ---------------------------------------------------------------- block operators
     def constraint_sum(self, g, a):
@@ -261,10 +295,14 @@ class Solver:
         """Cholesky factors of X and Y, pairings, residues and the iteration's scalars"""
         sdp, L = self.sdp, self.L
         out = dict(cond=(mp.mpf(0), ""), work=[])
-        cx_terms, diag_terms, pvec_rows, x_rows = [], [], [], []
+        cx_terms, diag_terms = [], []
         dual_err, primal_err_P = mp.mpf(0), mp.mpf(0)
         best = -1.0
-        for g, st in zip(sdp.groups, self.state):
+        xs = [stack_rows(st.x) for st in self.state]
+        Bx = KSum(self._B_exps(), torch.stack([col_exps(x) for x in xs]).amax(0), L)   # B^T x over all blocks
+        for gi, g in self._groups_prefetch(sdp.prefetch_B):
+            st = self.state[gi]
+            Bg = sdp.B_of(g)
             cX, cY = Chol(st.X), Chol(st.Y)
             for tag, ch in (("X", cX), ("Y", cY)):
                 lg = chol_log2_cond(ch, g.Xmask).tolist()
@@ -275,7 +313,9 @@ class Solver:
             T = cX.solve_lower(g.Vx)
             AXI = mirror_lower(gemm(T, T, transA=True))
             AY = mirror_lower(gemm(g.Vx, gemm(st.Y, g.Vx), transA=True))
-            By = gemm(g.B, repeat_batch(self.y, g.nb))
+            By = gemm(Bg, repeat_batch(self.y, g.nb))
+            Bx.add(stack_rows(Bg), xs[gi])
+            del Bg
             dres = add(add(-self.segment_trace(g, AY), By, -1), g.c)
             Pres = keep(add(self.constraint_sum(g, st.x), st.X, -1), g.mask2)
             mXY = keep(-gemm(st.X, st.Y), g.mask2)
@@ -283,12 +323,10 @@ class Solver:
             primal_err_P = max(primal_err_P, max_abs(Pres))
             cx_terms.append(mul(g.c, st.x))
             diag_terms.append(diagonal(mXY))
-            pvec_rows.append(stack_rows(g.B))
-            x_rows.append(stack_rows(st.x))
             out["work"].append(dict(cX=cX, cY=cY, AXI=AXI, AY=AY, dres=dres, Pres=Pres, mXY=mXY))
         out["pobj"] = sdp.const + total(_flat(cx_terms))
         out["dobj"] = sdp.const + total(mul(sdp.b, self.y))
-        out["pvec"] = add(sdp.b, gemm(_cat_rows(pvec_rows), _cat_rows(x_rows), transA=True), -1)
+        out["pvec"] = add(sdp.b, Bx.result(), -1)
         out["mu"] = -total(_flat(diag_terms)) / sdp.total_psd_rows
         out["dual_err"], out["primal_err_P"] = dual_err, primal_err_P
         out["primal_err_p"] = max_abs(out["pvec"])
@@ -299,7 +337,9 @@ class Solver:
         sdp, L = self.sdp, self.L
         TM = self.timers
         best, cond = -1.0, (mp.mpf(0), "")
-        for g, w in zip(sdp.groups, work):
+        tops = None
+        for gi, g in self._groups_prefetch(sdp.prefetch_B):
+            w = work[gi]
             TM("schur: assemble S")
             S = self.schur(g, w.pop("AXI"), w.pop("AY"))
             TM("schur: chol(S)")
@@ -312,15 +352,17 @@ class Solver:
                 cond = (mp.mpf(2) ** mp.mpf(lg[q]), f"schur_complement_cholesky.block_{g.idx[q]}")
             w["cS"] = cS
             TM("schur: P = L^-1 B")
-            w["P"] = cS.solve_lower(g.B)
-        TM("Q: accumulate")
-        tops = None
-        for w in work:
-            t = F._top(stack_rows(w["P"])).amax(0)
+            P = cS.solve_lower(sdp.B_of(g))
+            Ps = stack_rows(P)
+            t = F._top(Ps).amax(0)
             tops = t if tops is None else torch.maximum(tops, t)
+            w["Pexp"] = col_exps(Ps)
+            self._keep_P(g, w, P)
+            del P, Ps
+        TM("Q: accumulate")
         acc = F.GramAcc(sdp.N, L, tops)
-        for w in work:
-            acc.add(stack_rows(w["P"]))
+        for gi, g in self._groups_prefetch(self._prefetch_P):
+            acc.add(stack_rows(self._P(g, work[gi])))
         Q = mirror_lower(acc.result())
         TM("chol(Q)")
         if self.graphs and self.iterations_done >= 1:    # the first iteration runs eagerly (autotuner)
@@ -344,14 +386,20 @@ class Solver:
             VZV = gemm(g.Vx, gemm(Z, g.Vx), transA=True)
             rhs = add(-w["dres"], self.segment_trace(g, VZV), -1)
             w["R"], w["w"] = R, w["cS"].solve_lower(rhs)
-        Pall = _cat_rows([stack_rows(w["P"]) for w in work])
-        wall = _cat_rows([stack_rows(w["w"]) for w in work])
-        dy = cQ.solve(add(pvec, gemm(Pall, wall, transA=True), -1))
-        del Pall, wall
+        ws = [stack_rows(w["w"]) for w in work]
+        Pw = KSum(torch.stack([w["Pexp"] for w in work]).amax(0), torch.stack([col_exps(v) for v in ws]).amax(0),
+                  self.L)                                                  # P^T w over all blocks
+        for gi, g in self._groups_prefetch(self._prefetch_P):
+            Pw.add(stack_rows(self._P(g, work[gi])), ws[gi])
+        if sdp.groups:
+            self._prefetch_P(sdp.groups[0])
+        dy = cQ.solve(add(pvec, Pw.result(), -1))
+        del ws, Pw
         fr_terms, dirs = [], []
         lamP = lamD = None
-        for g, st, w in zip(sdp.groups, self.state, work):
-            Pdy = gemm(w["P"], repeat_batch(dy, g.nb))
+        for gi, g in self._groups_prefetch(self._prefetch_P):
+            st, w = self.state[gi], work[gi]
+            Pdy = gemm(self._P(g, w), repeat_batch(dy, g.nb))
             dx = w["cS"].solve_upper(add(w["w"], Pdy))
             dX = add(self.constraint_sum(g, dx), w["Pres"])
             dY = keep(-symmetrize(w["cX"].solve(add(gemm(dX, st.Y), w["R"], -1))), g.mask2)

@@ -392,6 +392,64 @@ def gemm(A, B, transA=False, transB=False):
     return MPF(out_m, out_e)
 
 
+def col_exps(A):
+    """row exponents of A^T for a 2-d MPF A (K, M): per column, the largest entry top less NB (EZ
+    for an all-zero column), as gemm computes them for op(A) = A^T"""
+    K, M = A.shape
+    if K == 0:
+        return torch.full((M,), EZ, dtype=torch.int64, device=A.e.device)
+    return _row_exp(A.m.contiguous(), A.e.contiguous(), A.L, K * M, 1, M, K, K * M, 1, M)
+
+
+class KSum:
+    """C = sum over segments s of A_s^T B_s (A_s (K_s, M), B_s (K_s, N), 2-d MPF), equal bit for bit
+    to gemm(cat(A_s), cat(B_s), transA=True) over the segments stacked along K: every entry is
+    rounded at the row exponents of the whole stack (EA = max over segments of col_exps(A_s), EB
+    likewise), the exact products are summed as residues over all segments, and one reconstruction
+    rounds the sum. The segments need not be on the device together (a segment may be fetched,
+    added and dropped), and the result does not depend on how the rows are split."""
+
+    def __init__(self, EA, EB, L):
+        self.EA, self.EB, self.L = EA.contiguous(), EB.contiguous(), L
+        self.M, self.N = EA.numel(), EB.numel()
+        self.R = ctx(L)
+        self.K = 0
+        self.acc = torch.zeros((self.R.n, self.M, self.N), dtype=torch.int16, device=dev)
+
+    def add(self, A, B):
+        R, L, M, N = self.R, self.L, self.M, self.N
+        K = A.shape[0]
+        if A.shape != (K, M) or B.shape != (K, N) or A.L != L or B.L != L:
+            raise ValueError(f"KSum.add: segment shapes {A.shape}, {B.shape} do not match ({M}, {N}) at L={L}")
+        if self.K + K > R.kmax:
+            raise ValueError(f"KSum: inner dimension {self.K + K} exceeds the RNS bound {R.kmax}")
+        Am, Ae, Bm, Be = A.m.contiguous(), A.e.contiguous(), B.m.contiguous(), B.e.contiguous()
+        per_k = 2 * R.n * (M + N) * 2
+        kc = max(64, min(KMAX_LAUNCH, RES_BUDGET // max(per_k, 1) // 64 * 64))
+        for k0 in range(0, K, kc):
+            kk = min(kc, K - k0)
+            kp = R.kp(kk)
+            Ah = torch.empty((R.n, kp // 16, M, 16), dtype=torch.int8, device=dev)
+            Al = torch.empty_like(Ah)
+            Bh = torch.empty((R.n, kp // 16, N, 16), dtype=torch.int8, device=dev)
+            Bl = torch.empty_like(Bh)
+            # rows of the splits = columns of A_s and B_s (stride 1), inner index = their rows
+            R.split(Am, Ae, self.EA, L, K * M, 1, M, K * M, 1, M, k0, kk, kp, Ah, Al)
+            R.split(Bm, Be, self.EB, L, K * N, 1, N, K * N, 1, N, k0, kk, kp, Bh, Bl)
+            R.product(Ah, Al, Bh, Bl, self.acc, 1, M, N, M * N, accumulate=True, scaled=True)
+            del Ah, Al, Bh, Bl
+        self.K += K
+
+    def result(self):
+        """C (M, N), normalised MPF"""
+        L, M, N = self.L, self.M, self.N
+        out_m = torch.empty((L, 1, M, N), dtype=torch.int32, device=dev)
+        out_e = torch.empty((1, M, N), dtype=torch.int64, device=dev)
+        self.R.crt(self.acc, M, N, 0, N, 1, outm=out_m, oute=out_e, L=L, plane=M * N, ldo=N, jo=0, Mb=M,
+                   erow=self.EA, ecol=self.EB, scaled=True)
+        return MPF(out_m[:, 0], out_e[0])
+
+
 def symmetrize(A):
     """(A + A^T)/2"""
     return add(A, A.T).shift(-1)

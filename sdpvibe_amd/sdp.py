@@ -135,6 +135,8 @@ class SDP:
         mp.mp.prec = F.NBITS(L) + 64                        # the solver's scalar precision
         self.L, self.N, self.const, self.layout = L, N, mp.mpf(const), layout
         self.normalization = None
+        self.store = None
+        self.layout_plan = None
         return self
 
     def _plan(self, shapes):
@@ -147,9 +149,42 @@ class SDP:
             cur.append(j)
         if cur:
             groups.append(cur)
+        self._set_groups(groups, shapes)
+
+    def _set_groups(self, groups, shapes):
         self.groups = [Group(c, *shapes[c[0]], self.L, self.N) for c in groups]
+        for gi, g in enumerate(self.groups):
+            g.gid = gi
         self.shapes = shapes
         self.nblocks = len(shapes)
+
+    # ---------------------------------------------------------------- placement (streaming)
+    def B_of(self, g):
+        """the group's B (nb, P, N) on the device: resident, or copied in from host memory"""
+        return g.B if g.B is not None else self.store.get(("B", g.gid))
+
+    def prefetch_B(self, g):
+        if g.B is None:
+            self.store.prefetch(("B", g.gid))
+
+    def placed(self, layout=None):
+        """this SDP grouped into the chunks of a memory plan (layout.Layout; by default the plan for
+        this device and the SDPB_GPU_* settings), with the B of host-placed chunks moved to a
+        store.Store in page-locked host memory. The iterates do not depend on the plan."""
+        from .layout import Layout
+        from .store import Store
+        if layout is None:
+            layout = Layout(self.shapes, self.N, self.L)
+        new = _regroup(self, layout.chunks, "batched")
+        new.layout_plan = layout
+        new.store = Store(layout.host_keys) if layout.host_keys else None
+        for g in new.groups:
+            if new.store is not None and new.store.is_host(("B", g.gid)):
+                new.store.put(("B", g.gid), g.B)
+                g.B = None
+        if new.store is not None:
+            torch.cuda.synchronize()
+        return new
 
     def _assemble(self, g, Vlist, clist, Blist, hlist):
         """per-group device arrays from per-block MPF pieces (V_even, V_odd (h_par, K)), c (P, 1), B (P, N)"""
@@ -299,8 +334,16 @@ def _pick(X, idx, dim):
 
 def _relayout(self, layout):
     """the same SDP grouped another way ("batched" or "plain"); data are copied, blocks keep their order"""
+    plan = SDP._empty(self.L, self.N, self.const, layout)
+    plan._plan(self.shapes)
+    return _regroup(self, [g.idx for g in plan.groups], layout)
+
+
+def _regroup(self, chunks, layout):
+    """the same SDP with groups of the given consecutive equal-shape block lists"""
     new = SDP._empty(self.L, self.N, self.const, layout)
-    new._plan(self.shapes)
+    new._set_groups(chunks, self.shapes)
+    new.normalization = self.normalization
     where = {}
     for g in self.groups:
         for q, j in enumerate(g.idx):
@@ -313,7 +356,7 @@ def _relayout(self, layout):
         ng.Xmask = torch.cat([g.Xmask[q + half * g.nb][None] for half in (0, 1) for g, q in srcs], 0)
         ng.mask2 = ng.Xmask[:, :, None] & ng.Xmask[:, None, :]
         ng.c = _cat_batch([_pick(g.c, [q], 0) for g, q in srcs], 0)
-        ng.B = _cat_batch([_pick(g.B, [q], 0) for g, q in srcs], 0)
+        ng.B = _cat_batch([_pick(self.B_of(g), [q], 0) for g, q in srcs], 0)
         ng.psd_rows = int(ng.Xmask.sum())
     new._finish(self.b)
     return new

@@ -78,6 +78,6 @@ def c_minus_By_blocks(solver):
     y = solver.y
     for g in solver.sdp.groups:
         ys = MPF(y.m[:, None].expand(-1, g.nb, -1, -1).contiguous(), y.e[None].expand(g.nb, -1, -1).contiguous())
-        r = add(g.c, gemm(g.B, ys), -1)
+        r = add(g.c, gemm(solver.sdp.B_of(g), ys), -1)
         out += [MPF(r.m[:, q].contiguous(), r.e[q].contiguous()) for q in range(g.nb)]
     return out

@@ -5,6 +5,7 @@
 // instead of a GPU fault.
 #pragma once
 #include <cmath>
+#include <cstdlib>
 #include <cstddef>
 #include <cstdint>
 #include <cstring>
@@ -33,6 +34,9 @@ inline hipError_t hipGetDevice(int* d) { *d = 0; return hipSuccess; }
 #define HIP_SYMBOL(x) (&(x))
 inline hipError_t hipGetSymbolAddress(void** p, const void* sym) { *p = const_cast<void*>(sym); return hipSuccess; }
 inline hipError_t hipFuncSetAttribute(const void*, int, int) { return hipSuccess; }
+#define hipHostMallocDefault 0
+inline hipError_t hipHostMalloc(void** p, size_t n, unsigned) { *p = malloc(n); return *p ? hipSuccess : 1; }
+inline hipError_t hipHostFree(void* p) { free(p); return hipSuccess; }
 
 extern thread_local dim3 threadIdx, blockIdx, blockDim, gridDim;
 void emu_syncthreads();

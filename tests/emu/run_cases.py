@@ -345,6 +345,19 @@ def panel_cases():
 CASES["panels"] = panel_cases
 
 
+def ksum_cases():
+    """mpf.KSum over row segments against one gemm over the stacked segments, bitwise"""
+    from tests.test_gpu_stream import ksum_case
+    put = lambda a: torch.from_numpy(np.ascontiguousarray(a))
+    ksum_case(put, 3, 40, 1, [7, 0, 30, 1], seed=5)
+    ksum_case(put, 2, 9, 3, [70, 65], seed=8)
+    print("ksum: ok", flush=True)
+    return True
+
+
+CASES["ksum"] = ksum_cases
+
+
 def bucketed_cases():
     from tests.test_gpu_bucketed import bucketed_case
     bucketed_case(lambda a: torch.from_numpy(np.ascontiguousarray(a)), 2, [3, 33, 9], 2, seed=5)

@@ -13,7 +13,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 def _declared():
     txt = open(os.path.join(ROOT, "include", "sdpvibe_hip.h")).read()
-    return sorted(set(re.findall(r"^(?:int|int64_t|uint64_t)\s+(sdpv_\w+)\s*\(", txt, flags=re.M)))
+    return sorted(set(re.findall(r"^(?:int|int64_t|uint64_t|void\s*\*)\s*(sdpv_\w+)\s*\(", txt, flags=re.M)))
 
 
 def test_header_declares_entry_points():
