@@ -60,6 +60,8 @@ def parse():
     ap.add_argument("--gemm-steps", type=int, default=50)
     ap.add_argument("--plain-iters", type=int, default=2, help="iterations timed in the per-block layout")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--stream-iters", type=int, default=2,
+                    help="iterations timed with every B and P streamed from host memory (0: skip)")
     ap.add_argument("--n", type=int, default=512)
     ap.add_argument("--L", type=int, default=24)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -176,11 +178,12 @@ TRAFFIC_PROFILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "prof
 
 
 def gemm_source_hash():
-    """64-bit hash of the exact GEMM's kernel sources (rns.hip, common.hpp, the C ABI header); the
-    library loader guarantees that the loaded library was built from these files"""
+    """64-bit hash of the exact GEMM's kernel code (rns.hip and common.hpp; the C ABI header holds
+    declarations only); the library loader guarantees that the loaded library was built from these
+    files"""
     import hashlib
     h = hashlib.sha256()
-    for f in ("include/sdpvibe_hip.h", "sdpvibe_amd/csrc/common.hpp", "sdpvibe_amd/csrc/rns.hip"):
+    for f in ("sdpvibe_amd/csrc/common.hpp", "sdpvibe_amd/csrc/rns.hip"):
         with open(os.path.join(ROOT, f), "rb") as fh:
             h.update(fh.read())
     return "%#x" % int(h.hexdigest()[:16], 16)
@@ -310,6 +313,25 @@ def ipm_leg(args, L=24):
         out["plain"] = {"groups": len(plain.groups), "s_per_iteration": round(pt, 4),
                         "batched_s_per_iteration_same_iterations": round(bt, 4), "speedup": round(pt / bt, 2),
                         "iterations": f"3..{2 + P}", "records_bitwise_equal": bool(same)}
+        del plain, psolver
+    if args.stream_iters > 0:
+        # the same iterations with every B and P = L_S^-1 B in page-locked host memory, copied in per
+        # chunk (host streaming for SDPs larger than HBM); iterations 1 and 2 untimed
+        from sdpvibe_amd.layout import Layout
+        S = args.stream_iters
+        streamed = sdp.placed(Layout(sdp.shapes, sdp.N, sdp.L, stream="all"))
+        st = streamed.store
+        h0, d0 = st.bytes_h2d, st.bytes_d2h                   # the placement's own copies
+        ssolver, stimes = timed(streamed, 2 + S)
+        same = all(ra[k] == rb[k] for ra, rb in zip(solver.log, ssolver.log)
+                   for k in ("mu", "P-obj", "D-obj", "P-err", "p-err", "D-err", "R-err", "P-step", "D-step", "beta"))
+        out["streamed"] = {"plan": streamed.layout_plan.describe(), "s_per_iteration": round(sum(stimes[2:]) / S, 4),
+                           "resident_s_per_iteration_same_iterations": round(sum(times[2:2 + S]) / S, 4),
+                           "host_to_device_GB_per_iteration": round((st.bytes_h2d - h0) / (2 + S) / 2 ** 30, 1),
+                           "device_to_host_GB_per_iteration": round((st.bytes_d2h - d0) / (2 + S) / 2 ** 30, 1),
+                           "iterations": f"3..{2 + S}", "records_bitwise_equal": bool(same)}
+        del ssolver
+        st.close()
     return out
 
 

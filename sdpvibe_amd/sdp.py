@@ -102,29 +102,59 @@ class SDP:
     shapes) or SDP.polynomial (the strictly feasible polynomial-matrix program of the benchmark)."""
 
     def __init__(self, path, L, workers=None, budget=None, chunk=None, stream=None, layout="batched"):
-        """the SDP stored at `path` (pmp2sdp output: directory or archive, binary or JSON blocks);
-        workers/budget/chunk/stream are accepted for the reference's signature (the whole SDP is
-        resident on the one GPU)"""
-        self.__dict__.update(SDP.from_path(path, L, layout).__dict__)
+        """the SDP stored at `path` (pmp2sdp output: directory or archive, binary or JSON blocks),
+        placed by the memory plan for budget / chunk / stream (bytes, bytes, auto|all|none, as
+        layout.Layout and the SDPB_GPU_* settings); `workers` is accepted for the reference's
+        signature (blocks are decoded by the C++ codec in this process)"""
+        self.__dict__.update(SDP.from_path(path, L, layout, budget=budget, chunk=chunk, stream=stream).__dict__)
 
     @classmethod
-    def from_path(cls, path, L, layout="batched"):
+    def from_path(cls, path, L, layout="batched", budget=None, chunk=None, stream=None):
+        """Blocks are decoded into host memory, the memory plan groups them into chunks, and the
+        chunks are assembled on the device one after the other; the B of a chunk the plan places in
+        host memory moves there as soon as it is assembled, so the SDP may exceed HBM."""
+        from .layout import Layout
         from .sdp_io import SdpSource, read_block, read_normalization, read_objectives
+        from .store import Store
         src = SdpSource(path)
         nblocks = src.json("control.json")["num_blocks"]
         const, b_strs = read_objectives(src)
         N = len(b_strs)
-        blocks = []
+        raw, shapes, hs = [], [], []
         for j in range(nblocks):
             r = read_block(src, j, L)
+            he, ho = r["bilinear_bases_even"][0], r["bilinear_bases_odd"][0]
+            raw.append(r)
+            shapes.append((r["dim"], r["K"], max(he, ho)))
+            hs.append((he, ho))
+        mp.mp.prec = F.NBITS(L) + 64
+        self = cls._empty(L, N, mp.mpf(const), layout)
+        if layout == "plain":
+            self._plan(shapes)
+        else:
+            plan = Layout(shapes, N, L, budget=budget, chunk=chunk, stream=stream)
+            self._set_groups(plan.chunks, shapes)
+            self.layout_plan = plan
+            self.store = Store(plan.host_keys) if plan.host_keys else None
+
+        def on_device(j):
+            r = raw[j]
             d, K = r["dim"], r["K"]
             P = K * d * (d + 1) // 2
             V = [None if arr is None else F.host_to_mpf(*arr, L, (h, K))
                  for h, arr in (r["bilinear_bases_even"], r["bilinear_bases_odd"])]
-            blocks.append(dict(dim=d, K=K, Ve=V[0], Vo=V[1], c=F.host_to_mpf(*r["c"], L, (P, 1)),
-                               B=F.host_to_mpf(*r["B"], L, (P, N))))
-        mp.mp.prec = F.NBITS(L) + 64
-        self = cls.from_blocks(blocks, F.from_strings(b_strs, L, (N, 1)), L, mp.mpf(const), layout)
+            return V, F.host_to_mpf(*r["c"], L, (P, 1)), F.host_to_mpf(*r["B"], L, (P, N))
+        for g in self.groups:
+            parts = [on_device(j) for j in g.idx]
+            self._assemble(g, [p[0] for p in parts], [p[1] for p in parts], [p[2] for p in parts],
+                           [hs[j] for j in g.idx])
+            del parts
+            if self.store is not None and self.store.is_host(("B", g.gid)):
+                self.store.put(("B", g.gid), g.B)
+                g.B = None
+            for j in g.idx:
+                raw[j] = None
+        self._finish(F.from_strings(b_strs, L, (N, 1)))
         self.normalization = read_normalization(src)
         return self
 

@@ -68,3 +68,24 @@ def test_binary_sdp_round_trip(tmp_path):
     write_out_txt(str(out), B.result, B.digits, 1.0)
     txt = open(out / "out.txt").read()
     assert txt.startswith('terminateReason = "maxIterations exceeded";') and "Solver runtime  = 1;" in txt
+
+
+@pytest.mark.parametrize("stream,chunk", [("all", 1), ("none", 1)])
+def test_streamed_load_iterates_like_the_resident_load(tmp_path, monkeypatch, stream, chunk):
+    """SDP(path) under a memory plan of one-block chunks, with every B and P in page-locked host
+    memory or all resident: the iterations equal those of the default load bit for bit"""
+    L, blocks, b = write_golden_sdp(tmp_path / "sdp")
+    base = SDP(str(tmp_path / "sdp"), L)
+    monkeypatch.setenv("SDPB_GPU_STREAM", stream)
+    placed = SDP(str(tmp_path / "sdp"), L, chunk=chunk)
+    assert len(placed.groups) == placed.nblocks > len(base.groups)
+    assert (placed.store is not None) == (stream == "all")
+    if stream == "all":
+        assert all(g.B is None for g in placed.groups)
+    A, B = Solver(base, L), Solver(placed, L)
+    A.run(max_iter=4)
+    B.run(max_iter=4)
+    for ra, rb in zip(A.log, B.log):
+        for k in FIELDS.values():
+            assert ra[k] == rb[k], (ra["iteration"], k)
+    assert torch.equal(A.y.m, B.y.m) and torch.equal(A.y.e, B.y.e)
