@@ -161,7 +161,8 @@ int sdpv_rns_split(const int32_t* X, const int64_t* e, const int64_t* Erow, int6
  * gives the CRT coefficients that sdpv_rns_crt takes with SDPV_CRT_SCALED.
  * Replaces RNS.accumulate / _gemm_kernel (sdpvibe/rns.py:217-224, :107-137),
  * _bgemm_kernel (sdpvibe/batched.py:97-127) and _syrk_lower_kernel (sdpvibe/batched.py:131-162). */
-/* config: -1 picks the tile by size (and is required with SDPV_GEMM_SYRK_LOWER); 0..5 select
+/* config: -1 picks the tile by size; with SDPV_GEMM_SYRK_LOWER only -1 and the square tiles
+ * 0, 1, 2, 3 and 6 are accepted; 0..5 select
  * 128x128 (2 stages), 128x128 (3 stages), 64x64, 32x32, 256x128 and 128x256 tiles with three
  * INT8 products per prime (the sum plane h + l formed in registers); 6 and 7 are the 128x128 and
  * 128x256 tiles with four products (HH, LL, HL + LH; no bytewise adds); 8 and 9 the 128x256 tile

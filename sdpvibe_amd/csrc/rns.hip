@@ -856,7 +856,10 @@ int sdpv_rns_gemm(const int8_t* Ah, const int8_t* Al, const int8_t* Bh, const in
     if (config < -1 || config > 12) return SDPV_EARG;
     if (kp % 64 != 0 || kp > 65536) return SDPV_ESIZE;
     if (nb * M > (1LL << 26) || nb * N > (1LL << 26)) return SDPV_ESIZE;   // 32-bit offsets in a stage
-    if ((mode & SDPV_GEMM_SYRK_LOWER) && (M != N || config != -1)) return SDPV_EARG;
+    // the syrk mode skips tiles above the diagonal by tile index: square tiles only
+    if ((mode & SDPV_GEMM_SYRK_LOWER) &&
+        (M != N || !(config == -1 || config == 0 || config == 1 || config == 2 || config == 3 || config == 6)))
+        return SDPV_EARG;
     if (G == 0 || nb == 0 || M == 0 || N == 0) return SDPV_OK;
     if (!Ah || !Al || !Bh || !Bl || !acc || !mods || !minv || !kcoef) return SDPV_EARG;
     hipStream_t st = as_stream(stream);

@@ -80,6 +80,12 @@ def _bucket(x):
     return 1 << max(0, int(x - 1).bit_length())
 
 
+# square output tiles: the syrk mode's lower-tile test needs BM = BN. Timed in the syrk mode at
+# N = 3000 (GramAcc), none beats the 128 x 128 four-product tile of the size rule, so syrk products
+# are not tuned (no first-iteration tuning cost); a caller may still pin one of these.
+SYRK_CONFIGS = (0, 1, 2, 3, 6)
+
+
 def tuned_config(R, nb, M, N, kp):
     """fastest product tile configuration for this shape class, measured once per process on
     scratch residues (every configuration computes identical residues); -1 (size rule) when
@@ -249,8 +255,12 @@ class RNS:
         if g0 < 0 or g0 + G > self.n or ld_g < nb * M * N or acc.numel() < (g0 + G - 1) * ld_g + nb * M * N:
             raise ValueError("product: accumulator too small for the primes and output block")
         mode = (1 if accumulate else 0) | (2 if syrk else 0) | (4 if scaled else 0)
+        if syrk and M != N:
+            raise ValueError("product: the syrk mode needs M == N")
         if config is None:
             config = -1 if syrk else tuned_config(self, nb, M, N, kp)
+        elif syrk and config not in (-1,) + SYRK_CONFIGS:
+            raise ValueError(f"product: configuration {config} has no square tile for the syrk mode")
         call("sdpv_rns_gemm", ptr(Ah), ptr(Al), ptr(Bh), ptr(Bl), G, nb, M, N, kp, ptr(acc), ld_g,
              ptr(self.m32), ptr(self.minv), ptr(self.kcoef_crt if scaled else self.kcoef), g0, mode, config,
              stream())

@@ -254,14 +254,18 @@ def product_configs_case(dev_put, M=300, N=200, K=130, nb=2, L=10, configs=None,
             f"config {cfg} scaled accumulate")
     if not syrk:
         return
-    # syrk: A = B, lower tiles (tile size by N as the reference: 128 here)
-    acc = dev_put(np.zeros((G, M, M), np.int16))
+    # syrk: A = B, lower tiles, with the size rule (-1: 128 here, as the reference) and every square tile
+    from sdpvibe_amd.rns import SYRK_CONFIGS
     A1 = [chunk_major(dev_put(x.astype(np.int8)[:, :M])) for x in (ha, la)]
-    R.product(A1[0], A1[1], A1[0], A1[1], acc, 1, M, M, M * M, accumulate=True, syrk=True)
     full = np.einsum("gik,gjk->gij", ra[:, 0], ra[:, 0]) % m[:, 0]
-    got = acc.cpu().numpy().astype(np.int64)
     low = np.tril(np.ones((M, M), bool))
-    _eq(np.where(low, got, 0), np.where(low, full, 0), "syrk lower")
+    for cfg in (-1,) + SYRK_CONFIGS:
+        acc = dev_put(np.zeros((G, M, M), np.int16))
+        R.product(A1[0], A1[1], A1[0], A1[1], acc, 1, M, M, M * M, accumulate=True, syrk=True, config=cfg)
+        got = acc.cpu().numpy().astype(np.int64)
+        _eq(np.where(low, got, 0), np.where(low, full, 0), f"syrk lower, configuration {cfg}")
+    with pytest.raises(ValueError):
+        R.product(A1[0], A1[1], A1[0], A1[1], acc, 1, M, M, M * M, accumulate=True, syrk=True, config=12)
 
 
 def carry_edge_case(dev_put, L):
