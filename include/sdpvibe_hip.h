@@ -168,8 +168,9 @@ int sdpv_rns_split(const int32_t* X, const int64_t* e, const int64_t* Erow, int6
  * 128x256 tiles with four products (HH, LL, HL + LH; no bytewise adds); 8 and 9 the 128x256 tile
  * with 3 stages, four and three products; 10 and 11 the 128x64 and 64x128 tiles with four
  * products (fewer registers, more workgroups per CU), 12 the 64x128 tile in 3 stages of 32 bytes of
- * k (36 KB of LDS, four workgroups per CU). Every configuration computes the same
- * residues (an autotuner picks by time). */
+ * k (36 KB of LDS, four workgroups per CU); 13 is configuration 12 with its epilogue staged
+ * through LDS, so that each lane stores 16 contiguous bytes (full tiles, N a multiple of 8).
+ * Every configuration computes the same residues (an autotuner picks by time). */
 int sdpv_rns_gemm(const int8_t* Ah, const int8_t* Al, const int8_t* Bh, const int8_t* Bl,
                   int64_t G, int64_t nb, int64_t M, int64_t N, int64_t kp, int16_t* acc,
                   int64_t ld_g, const int32_t* mods, const double* minv, const int32_t* kcoef,

@@ -277,7 +277,10 @@ __global__ void __launch_bounds__(256, (KSC >= 1 && KSC <= 3) ? 4 : 1) k_rns_spl
 // LDS stage layout: [A h | A l | B h | B l], each plane [chunk][tile row][16 B]: fragment reads of
 // 32 consecutive rows are contiguous and conflict-free. The s = h + l operand is formed in
 // registers (bytewise add) from the h and l fragments.
-template <int WM, int WN, int BLM, int BLN, int NST, int P4, int BKB = 64>
+// STAGE: for a full tile whose rows are 16-byte aligned, each wave writes its reduced results
+// through LDS (the stage buffers, free after the main loop) and stores 16 contiguous bytes per lane
+// instead of one 2-byte entry per lane per row.
+template <int WM, int WN, int BLM, int BLN, int NST, int P4, int BKB = 64, bool STAGE = false>
 __global__ void __launch_bounds__(WM* WN * 64, BKB == 32 ? 4 : ((WM * WN <= 4) ? 2 : 1)) k_rns_gemm(
     const int8_t* __restrict__ Ah, const int8_t* __restrict__ Al, const int8_t* __restrict__ Bh,
     const int8_t* __restrict__ Bl, int G, int nb, i64 M, i64 N, i64 kp, int16_t* __restrict__ acc, i64 ld_g,
@@ -441,6 +444,69 @@ __global__ void __launch_bounds__(WM* WN * 64, BKB == 32 ? 4 : ((WM * WN <= 4) ?
     const bool centred = (mode & SDPV_GEMM_CENTRED) != 0;
     int16_t* out = acc + (i64)g * ld_g + b * M * N;
     const bool full = r0 + BM <= M && c0 + BN <= N;  // uniform: no per-entry tests inside the matrix
+    if (STAGE) {
+        constexpr int SROW = 40;                     // int16 per staged row: 80 bytes, 16-byte aligned
+        static_assert((size_t)WM * WN * BLM * BLN * 32 * SROW * 2 <= (size_t)NST * SBYTES, "staging area");
+        __syncthreads();                             // every wave is done reading the stage buffers
+        int16_t* sw = (int16_t*)smem8 + (size_t)wave * (BLM * BLN * 32 * SROW);
+#pragma unroll
+        for (int i = 0; i < BLM; i++)
+#pragma unroll
+            for (int j = 0; j < BLN; j++)
+#pragma unroll
+                for (int q = 0; q < 16; q++) {
+                    const int rb = 8 * (q >> 2) + 4 * hsel + (q & 3);
+                    const double v = fma(k0c, (double)cHH[i][j][q], fma(k1c, (double)cSS[i][j][q], k2c * (double)cLL[i][j][q]));
+                    // centred representative of the product part (the nearest quotient of the per-entry path)
+                    sw[((i * BLN + j) * 32 + rb) * SROW + l32] = (int16_t)(int)fma(-rint(v * mi), md, v);
+                }
+        __syncthreads();
+        // lane -> 8 consecutive columns of one row; (r + old) is reduced again when accumulating, which
+        // gives the residue of the per-entry path's single reduction
+        auto finish = [&](int r, const int16_t* po) {
+            if (accum) {
+                const double x = (double)r + (double)*po;
+                r = (int)fma(-rint(x * mi), md, x);
+            }
+            if (!centred && r < 0) r += m;
+            return r;
+        };
+        const bool vec = full && N % 8 == 0 && (((size_t)out) & 15) == 0;   // uniform
+        const int seg = lane & 3;
+#pragma unroll
+        for (int i = 0; i < BLM; i++)
+#pragma unroll
+            for (int j = 0; j < BLN; j++)
+#pragma unroll
+                for (int h = 0; h < 2; h++) {
+                    const int rb = 16 * h + (lane >> 2);
+                    const v4i w = *(const v4i*)(sw + ((i * BLN + j) * 32 + rb) * SROW + seg * 8);
+                    const i64 row = r0 + (wm * BLM + i) * 32 + rb;
+                    const i64 col = c0 + (wn * BLN + j) * 32 + seg * 8;
+                    int16_t* o = out + row * N + col;
+                    if (vec) {
+                        v4i y = w;
+                        if (accum || !centred) {
+#pragma unroll
+                            for (int u = 0; u < 4; u++) {
+                                const int lo = finish((int)(int16_t)(w[u] & 0xffff), o + 2 * u);
+                                const int hi = finish(w[u] >> 16, o + 2 * u + 1);
+                                y[u] = (int)(((u32)hi << 16) | ((u32)lo & 0xffffu));
+                            }
+                        }
+                        *(v4i*)o = y;
+                    } else if (row < M) {
+#pragma unroll
+                        for (int u = 0; u < 8; u++) {
+                            if (col + u < N) {
+                                const int r = (int)(int16_t)((u & 1) ? (w[u >> 1] >> 16) : (w[u >> 1] & 0xffff));
+                                o[u] = (int16_t)finish(r, o + u);
+                            }
+                        }
+                    }
+                }
+        return;
+    }
 #pragma unroll
     for (int i = 0; i < BLM; i++)
 #pragma unroll
@@ -461,7 +527,7 @@ __global__ void __launch_bounds__(WM* WN * 64, BKB == 32 ? 4 : ((WM * WN <= 4) ?
         }
 }
 
-template <int WM, int WN, int BLM, int BLN, int NST, int P4, int BKB = 64>
+template <int WM, int WN, int BLM, int BLN, int NST, int P4, int BKB = 64, bool STAGE = false>
 int launch_gemm(const int8_t* Ah, const int8_t* Al, const int8_t* Bh, const int8_t* Bl, int G, int nb, i64 M, i64 N,
                 i64 kp, int16_t* acc, i64 ld_g, const i32* mods, const double* minv, const i32* kcoef, int g0,
                 int mode, hipStream_t st) {
@@ -473,11 +539,11 @@ int launch_gemm(const int8_t* Ah, const int8_t* Al, const int8_t* Bh, const int8
     const size_t lds = (size_t)NST * 2 * BKB * (BM + BN);
     static bool attr_set = false;
     if (!attr_set) {
-        (void)hipFuncSetAttribute((const void*)k_rns_gemm<WM, WN, BLM, BLN, NST, P4, BKB>,
+        (void)hipFuncSetAttribute((const void*)k_rns_gemm<WM, WN, BLM, BLN, NST, P4, BKB, STAGE>,
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         attr_set = true;
     }
-    hipLaunchKernelGGL((k_rns_gemm<WM, WN, BLM, BLN, NST, P4, BKB>), dim3((unsigned)total), dim3(WM * WN * 64), lds, st, Ah,
+    hipLaunchKernelGGL((k_rns_gemm<WM, WN, BLM, BLN, NST, P4, BKB, STAGE>), dim3((unsigned)total), dim3(WM * WN * 64), lds, st, Ah,
                        Al, Bh, Bl, G, nb, M, N, kp, acc, ld_g, mods, minv, kcoef, g0, mode, tiles_m, tiles_n);
     SDPV_TRY_LAUNCH();
     return SDPV_OK;
@@ -853,7 +919,7 @@ int sdpv_rns_gemm(const int8_t* Ah, const int8_t* Al, const int8_t* Bh, const in
                   int64_t M, int64_t N, int64_t kp, int16_t* acc, int64_t ld_g, const int32_t* mods,
                   const double* minv, const int32_t* kcoef, int64_t g0, int32_t mode, int32_t config, void* stream) {
     if (G < 0 || nb < 0 || M < 0 || N < 0 || kp < 0 || g0 < 0 || (mode & ~7)) return SDPV_EARG;
-    if (config < -1 || config > 12) return SDPV_EARG;
+    if (config < -1 || config > 13) return SDPV_EARG;
     if (kp % 64 != 0 || kp > 65536) return SDPV_ESIZE;
     if (nb * M > (1LL << 26) || nb * N > (1LL << 26)) return SDPV_ESIZE;   // 32-bit offsets in a stage
     // the syrk mode skips tiles above the diagonal by tile index: square tiles only
@@ -882,7 +948,9 @@ int sdpv_rns_gemm(const int8_t* Ah, const int8_t* Al, const int8_t* Bh, const in
         case 9: return launch_gemm<2, 4, 2, 2, 3, 0>(Ah, Al, Bh, Bl, g, b, M, N, kp, acc, ld_g, mods, minv, kcoef, q0, mode, st);
         case 10: return launch_gemm<2, 2, 2, 1, 2, 1>(Ah, Al, Bh, Bl, g, b, M, N, kp, acc, ld_g, mods, minv, kcoef, q0, mode, st);
         case 11: return launch_gemm<2, 2, 1, 2, 2, 1>(Ah, Al, Bh, Bl, g, b, M, N, kp, acc, ld_g, mods, minv, kcoef, q0, mode, st);
-        default: return launch_gemm<2, 2, 1, 2, 3, 1, 32>(Ah, Al, Bh, Bl, g, b, M, N, kp, acc, ld_g, mods, minv, kcoef, q0, mode, st);
+        case 12: return launch_gemm<2, 2, 1, 2, 3, 1, 32>(Ah, Al, Bh, Bl, g, b, M, N, kp, acc, ld_g, mods, minv, kcoef, q0, mode, st);
+        // configuration 12 with the epilogue staged through LDS (16-byte stores)
+        default: return launch_gemm<2, 2, 1, 2, 3, 1, 32, true>(Ah, Al, Bh, Bl, g, b, M, N, kp, acc, ld_g, mods, minv, kcoef, q0, mode, st);
     }
 }
 

@@ -66,7 +66,7 @@ def gemm_config(M, N, K):
 
 
 # ------------------------------------------------------------------ product tile autotuner
-GEMM_CONFIGS = tuple(range(13))   # see sdpv_rns_gemm: tile shapes, stages, three or four products
+GEMM_CONFIGS = tuple(range(14))   # see sdpv_rns_gemm: tile shapes, stages, three or four products, epilogue
 _tuned = {}
 TUNE = os.environ.get("SDPV_TUNE", "1") != "0"
 # SDPV_GEMM_CONFIG=<c> pins every tuned product to configuration c (measurements of one tile shape)

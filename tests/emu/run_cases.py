@@ -284,6 +284,21 @@ CASES["product_a"] = lambda: product_cases(range(0, 6), True)
 CASES["product_b"] = lambda: product_cases(range(6, 13), False)
 
 
+def product_staged_cases():
+    """the configuration with the LDS-staged epilogue (13) at a size with full tiles (its staged
+    path) and edge tiles (the per-entry path)"""
+    from tests.test_gpu_kernels import product_configs_case
+    put = lambda a: torch.from_numpy(np.ascontiguousarray(a))
+    product_configs_case(put, M=130, N=136, K=65, nb=1, L=1, configs=[13], syrk=False)
+    # N not a multiple of 8: full tiles store entry by entry as well
+    product_configs_case(put, M=70, N=133, K=64, nb=2, L=1, configs=[13], syrk=False)
+    print("product config 13: ok", flush=True)
+    return True
+
+
+CASES["product_c"] = product_staged_cases
+
+
 def carry_cases():
     from tests.test_gpu_kernels import carry_edge_case
     for L in (1, 2, 3, 24, 25):

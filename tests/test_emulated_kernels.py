@@ -14,7 +14,7 @@ import pytest
 from tests.emu import harness
 
 # longest first, so that the short cases fill in behind the long ones
-CASES = ["bucketed", "product_a", "product_b", "chol", "tile_wide", "small_round", "linalg", "gram", "bgemm",
+CASES = ["bucketed", "product_a", "product_b", "product_c", "chol", "tile_wide", "small_round", "linalg", "gram", "bgemm",
          "gemm_regwords", "tile", "carry", "gemm_batched", "gemm_ksplit", "gemm_ksplit_long", "eig", "crt",
          "residues", "gemm_trans", "gemm_small", "bgemm_narrow", "panels", "ksum"]
 WORKERS = max(1, min(6, (os.cpu_count() or 2) - 2))

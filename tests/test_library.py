@@ -37,7 +37,7 @@ def test_bad_arguments_are_rejected_without_launch():
     assert lib.sdpv_shift_round(None, 1, -1, None, None, 1, None, 0, None) == -1
     assert lib.sdpv_add_same(None, None, None, 1, 4, 3, None) == -1
     assert lib.sdpv_rns_gemm(None, None, None, None, 1, 1, 4, 4, 100, None, 0, None, None, None, 0, 0, -1, None) == -2
-    assert lib.sdpv_rns_gemm(None, None, None, None, 1, 1, 4, 4, 64, None, 0, None, None, None, 0, 0, 13, None) == -1
+    assert lib.sdpv_rns_gemm(None, None, None, None, 1, 1, 4, 4, 64, None, 0, None, None, None, 0, 0, 14, None) == -1
     # the syrk mode (2) takes square tiles only: configuration 12 is 64 x 128
     assert lib.sdpv_rns_gemm(None, None, None, None, 1, 1, 4, 4, 64, None, 0, None, None, None, 0, 2, 12, None) == -1
     assert lib.sdpv_mpf_mul(None, None, None, None, None, None, 200, 4, 0, None) == -2
