@@ -64,3 +64,25 @@ def test_mixed_shapes_never_share_a_chunk(monkeypatch):
     lay = plan(monkeypatch, "all", 1e12, shapes=shapes, N=10)
     covers(lay, shapes)
     assert [len(c) for c in lay.chunks] == [4, 3, 2]
+
+
+def test_store_tree_utilities():
+    """tmap / tleaves / nbytes walk MPF slots, lists, tuples, dicts and plain objects (CPU tensors)"""
+    import torch
+
+    from sdpvibe_amd.mpf import MPF
+    from sdpvibe_amd.store import nbytes, tleaves, tmap
+
+    class Holder:
+        def __init__(self):
+            self.a = torch.zeros(3, dtype=torch.int64)
+            self.name = "x"
+    A = MPF(torch.zeros((2, 4, 5), dtype=torch.int32), torch.zeros((4, 5), dtype=torch.int64))
+    tree = {"A": A, "l": [torch.ones(7, dtype=torch.int16), (None, 3, Holder())]}
+    leaves = tleaves(tree)
+    assert [tuple(t.shape) for t in leaves] == [(2, 4, 5), (4, 5), (7,), (3,)]
+    assert nbytes(tree) == 2 * 4 * 5 * 4 + 4 * 5 * 8 + 7 * 2 + 3 * 8
+    doubled = tmap(lambda t: t + 1, tree)
+    assert isinstance(doubled["A"], MPF) and int(doubled["A"].m.sum()) == 40
+    assert doubled["l"][1][0] is None and doubled["l"][1][1] == 3 and doubled["l"][1][2].name == "x"
+    assert int(doubled["l"][1][2].a.sum()) == 3 and int(tree["l"][1][2].a.sum()) == 0
