@@ -564,10 +564,11 @@ struct Row {  // consecutive words in LDS
 
 constexpr int TWAVES = TT / 64;
 
-// words of the LDS-resident factorisation for BT, WL (entries padded to an odd stride WS)
-__host__ __device__ inline i64 lds_factor_words(int BT, int WL) {
+// words of the LDS-resident factorisation for BT, WL and nw waves (entries padded to an odd
+// stride WS)
+__host__ __device__ inline i64 lds_factor_words(int BT, int WL, int nw) {
     const i64 WS = WL | 1;
-    return (i64)BT * (BT + 1) / 2 * WS + (i64)BT * WS + WL + 2 * BT + 128 * TWAVES;
+    return (i64)BT * (BT + 1) / 2 * WS + (i64)BT * WS + WL + 2 * BT + 128 * nw;
 }
 
 // row i of the packed lower triangle starts at entry i (i + 1) / 2
@@ -663,8 +664,12 @@ __device__ __forceinline__ void trail_update(const u32* x, const u32* y, int ta,
 // sign for the trailing products. Trailing update over the pairs k < j <= i: wave 0 updates the
 // next diagonal entry and goes on to pivot k + 1 (look-ahead), the other three waves update the
 // remaining pairs, one pair per thread while there are more than `lane_pairs`, else one per wave.
-__global__ void __launch_bounds__(TT) k_tile_chol_lds(u32* __restrict__ Wg, u32* __restrict__ Rg, int BT, int WL,
+// NT threads: 256 (one wave per SIMD) or 512 (two, so that each SIMD has a second instruction
+// stream and the trailing update has 448 threads).
+template <int NT>
+__global__ void __launch_bounds__(NT) k_tile_chol_lds(u32* __restrict__ Wg, u32* __restrict__ Rg, int BT, int WL,
                                                      int LF, int lane_pairs) {
+    constexpr int NW = NT / 64;
     SDPV_DYNAMIC_LDS(u32, lds);
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const i64 tile = blockIdx.x;
@@ -681,16 +686,16 @@ __global__ void __launch_bounds__(TT) k_tile_chol_lds(u32* __restrict__ Wg, u32*
     auto te = [&](int i, int j) { return T + (i64)(tri_base(i) + j) * WS; };
     const int total = BT * BT * WL;
     // the lower triangle into LDS, 16 independent loads per thread in flight
-    for (int base = tid; base < total; base += 16 * TT) {
+    for (int base = tid; base < total; base += 16 * NT) {
         u32 v[16];
 #pragma unroll
         for (int u = 0; u < 16; u++) {
-            const int idx = base + u * TT;
+            const int idx = base + u * NT;
             v[u] = idx < total ? W[idx] : 0u;
         }
 #pragma unroll
         for (int u = 0; u < 16; u++) {
-            const int idx = base + u * TT;
+            const int idx = base + u * NT;
             if (idx < total) {
                 const int e = idx / WL, q = idx - e * WL, i = e / BT, j = e - i * BT;
                 if (j <= i) te(i, j)[q] = v[u];
@@ -712,10 +717,10 @@ __global__ void __launch_bounds__(TT) k_tile_chol_lds(u32* __restrict__ Wg, u32*
     if (wave == 0) pivot(0, lane < WL ? te(0, 0)[lane] : 0u);
     __syncthreads();
     for (int k = 0; k < BT; k++) {
-        // column scale, entry i on wave (i - k - 1) mod 4: L_ik = mulfx(A_ik, r)
+        // column scale, entry i on wave (i - k - 1) mod NW: L_ik = mulfx(A_ik, r)
         {
             const u32 r = lane < WL ? RK[lane] : 0u;
-            for (int i = k + 1 + wave; i < BT; i += TWAVES) {
+            for (int i = k + 1 + wave; i < BT; i += NW) {
                 u32* a = te(i, k);
                 const u32 lv = lane_mulfx(lane < WL ? a[lane] : 0u, r, 0, WL, LF, PX, PY);
                 const bool neg = (lane_get(lv, WL - 1) >> 31) != 0;
@@ -734,7 +739,7 @@ __global__ void __launch_bounds__(TT) k_tile_chol_lds(u32* __restrict__ Wg, u32*
         __syncthreads();
         if (k == BT - 1) break;
         // trailing update over the lower pairs k < j <= i: A_ij -= L_ik L_jk. Pair 0, the next
-        // diagonal entry, on wave 0, which then computes pivot k + 1 while waves 1..3 update the
+        // diagonal entry, on wave 0, which then computes pivot k + 1 while the other waves update the
         // other pairs (one per wave while there are at most `lane_pairs`, else one per thread)
         const int m = BT - k - 1, npairs = m * (m + 1) / 2;
         if (wave == 0) {
@@ -745,7 +750,7 @@ __global__ void __launch_bounds__(TT) k_tile_chol_lds(u32* __restrict__ Wg, u32*
             if (lane < WL) te(r, r)[lane] = d;
             pivot(r, d);
         } else if (npairs - 1 <= lane_pairs) {
-            for (int p = wave; p < npairs; p += TWAVES - 1) {
+            for (int p = wave; p < npairs; p += NW - 1) {
                 int ii, jj;
                 tri_pair(p, ii, jj);
                 const int r = k + 1 + ii, c = k + 1 + jj;
@@ -756,7 +761,7 @@ __global__ void __launch_bounds__(TT) k_tile_chol_lds(u32* __restrict__ Wg, u32*
                 if (lane < WL) aij[lane] = nv;
             }
         } else {
-            for (int p = 1 + tid - 64; p < npairs; p += TT - 64) {
+            for (int p = 1 + tid - 64; p < npairs; p += NT - 64) {
                 int ii, jj;
                 tri_pair(p, ii, jj);
                 const int r = k + 1 + ii, c = k + 1 + jj;
@@ -766,7 +771,7 @@ __global__ void __launch_bounds__(TT) k_tile_chol_lds(u32* __restrict__ Wg, u32*
         __syncthreads();
     }
     // the factor back, zeros above the diagonal
-    for (int idx = tid; idx < total; idx += TT) {
+    for (int idx = tid; idx < total; idx += NT) {
         const int e = idx / WL, q = idx - e * WL, i = e / BT, j = e - i * BT;
         W[idx] = j <= i ? te(i, j)[q] : 0u;
     }
@@ -926,6 +931,15 @@ inline int tile_lane_pairs() {
     return v;
 }
 
+// threads of the LDS-resident factorisation: 512 unless SDPV_TILE_THREADS=256
+inline int tile_threads() {
+    static const int v = [] {
+        const char* e = getenv("SDPV_TILE_THREADS");
+        return (e && atoi(e) == 256) ? 256 : 512;
+    }();
+    return v;
+}
+
 }  // namespace
 
 extern "C" {
@@ -949,16 +963,23 @@ int sdpv_tile_chol(const int32_t* A, int64_t L, int64_t nb, int64_t BT, int32_t*
     static bool attr = false;
     if (!attr) {
         (void)hipFuncSetAttribute((const void*)k_tile_factor, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-        (void)hipFuncSetAttribute((const void*)k_tile_chol_lds, hipFuncAttributeMaxDynamicSharedMemorySize,
+        (void)hipFuncSetAttribute((const void*)k_tile_chol_lds<256>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  160 * 1024);
+        (void)hipFuncSetAttribute((const void*)k_tile_chol_lds<512>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                   160 * 1024);
         attr = true;
     }
-    const size_t lds_f = (size_t)lds_factor_words((int)BT, WL) * 4;
+    const int nt = tile_threads();
+    const size_t lds_f = (size_t)lds_factor_words((int)BT, WL, nt / 64) * 4;
     const size_t lds_i = (size_t)lds_inverse_words((int)BT, WL) * 4;
     if (WL <= 64 && lds_f <= 160 * 1024 && BT * WL <= INV_PRE * TT) {
         // the tile stays in LDS; the inverse one workgroup per column
-        hipLaunchKernelGGL(k_tile_chol_lds, dim3((unsigned)nb), dim3(TT), lds_f, st, W, RI, (int)BT, WL, (int)L,
-                           tile_lane_pairs());
+        if (nt == 512)
+            hipLaunchKernelGGL(k_tile_chol_lds<512>, dim3((unsigned)nb), dim3(512), lds_f, st, W, RI, (int)BT, WL,
+                               (int)L, tile_lane_pairs());
+        else
+            hipLaunchKernelGGL(k_tile_chol_lds<256>, dim3((unsigned)nb), dim3(256), lds_f, st, W, RI, (int)BT, WL,
+                               (int)L, tile_lane_pairs());
         SDPV_TRY_LAUNCH();
         hipLaunchKernelGGL(k_tile_inverse, dim3((unsigned)nb, (unsigned)BT), dim3(TT), lds_i, st, W, RI, X, (int)BT,
                            WL, (int)L);
