@@ -15,7 +15,7 @@
 
 namespace {
 
-constexpr int TT = 256;  // threads per tile workgroup
+constexpr int TT = 256;  // threads per workgroup of k_tile_factor and k_tile_trsv
 
 struct Slot {  // thread-strided word array in LDS
     u32* p;
@@ -662,10 +662,10 @@ __device__ __forceinline__ void trail_update(const u32* x, const u32* y, int ta,
 // Pivot k: wave 0 computes r = d^-1/2 and d r (lane_rsqrt, lane_mulfx). Column scale: the waves
 // take the entries i > k in turn, L_ik = lane_mulfx(A_ik, r), and keep |L_ik|, its top word and its
 // sign for the trailing products. Trailing update over the pairs k < j <= i: wave 0 updates the
-// next diagonal entry and goes on to pivot k + 1 (look-ahead), the other three waves update the
+// next diagonal entry and goes on to pivot k + 1 (look-ahead), the other waves update the
 // remaining pairs, one pair per thread while there are more than `lane_pairs`, else one per wave.
-// NT threads: 256 (one wave per SIMD) or 512 (two, so that each SIMD has a second instruction
-// stream and the trailing update has 448 threads).
+// NT threads: 256 (one wave per SIMD), 512 or 1024 (two or four waves per SIMD: more instruction
+// streams per SIMD, and 448 or 960 threads for the trailing products).
 template <int NT>
 __global__ void __launch_bounds__(NT) k_tile_chol_lds(u32* __restrict__ Wg, u32* __restrict__ Rg, int BT, int WL,
                                                      int LF, int lane_pairs) {
@@ -777,20 +777,22 @@ __global__ void __launch_bounds__(NT) k_tile_chol_lds(u32* __restrict__ Wg, u32*
     }
 }
 
-// words of k_tile_inverse's LDS for BT, WL; the row prefetch takes BT WL <= 8 TT
-__host__ __device__ inline i64 lds_inverse_words(int BT, int WL) {
-    return 4 * (i64)BT * WL + (i64)TWAVES * WL + 128 * TWAVES;
+// words of k_tile_inverse's LDS for BT, WL and nw waves; the row prefetch takes BT WL <= 8 threads
+__host__ __device__ inline i64 lds_inverse_words(int BT, int WL, int nw) {
+    return 4 * (i64)BT * WL + (i64)nw * WL + 128 * nw;
 }
 constexpr int INV_PRE = 8;  // words of the next row each thread prefetches
 
 // Column k of X = L^-1 for one tile (the recurrences of k_tile_factor's inverse):
 // x_kk = rinv_k; for i > k, x_ik = -mulfx(sum_{j=k}^{i-1} mulfx(l_ij, x_jk), rinv_i), every product
-// truncated, the sum exact (modulo 2^(32 WL)). Step i: the four waves take the products j = k..i-1
-// in turn, one lane_mulfx each, and add them into a running lane sum; wave 0 adds the four sums
+// truncated, the sum exact (modulo 2^(32 WL)). Step i: the NT / 64 waves take the products j = k..i-1
+// in turn, one lane_mulfx each, and add them into a running lane sum; wave 0 adds the waves' sums
 // and finishes x_ik with one lane_mulfx. Row i + 1 of L is loaded while step i computes.
-// Grid (nb, BT), TT threads. X entries above the diagonal are written as zeros.
-__global__ void __launch_bounds__(TT) k_tile_inverse(const u32* __restrict__ Wg, const u32* __restrict__ Rg,
+// Grid (nb, BT), NT threads. X entries above the diagonal are written as zeros.
+template <int NT>
+__global__ void __launch_bounds__(NT) k_tile_inverse(const u32* __restrict__ Wg, const u32* __restrict__ Rg,
                                                     u32* __restrict__ Xg, int BT, int WL, int LF) {
+    constexpr int NW = NT / 64;
     SDPV_DYNAMIC_LDS(u32, lds);
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const i64 tile = blockIdx.x;
@@ -798,18 +800,18 @@ __global__ void __launch_bounds__(TT) k_tile_inverse(const u32* __restrict__ Wg,
     u32* XT = lds;                      // BT x WL: x_jk
     u32* RI = XT + BT * WL;             // BT x WL: rinv
     u32* LR = RI + BT * WL;             // 2 x BT x WL: rows of L (slot j holds l_ij), double buffered
-    u32* PART = LR + 2 * BT * WL;       // TWAVES x WL: the waves' partial sums
-    u32* PX = PART + TWAVES * WL + 128 * wave;
+    u32* PART = LR + 2 * BT * WL;       // NW x WL: the waves' partial sums
+    u32* PX = PART + NW * WL + 128 * wave;
     u32* PY = PX + 64;
     const u32* W = Wg + tile * BT * BT * WL;
     const u32* R = Rg + tile * BT * WL;
     u32* X = Xg + tile * BT * BT * WL;
-    for (int idx = tid; idx < BT * WL; idx += TT) RI[idx] = R[idx];
+    for (int idx = tid; idx < BT * WL; idx += NT) RI[idx] = R[idx];
     for (int i = 0; i < k; i++)
-        for (int q = tid; q < WL; q += TT) X[((i64)i * BT + k) * WL + q] = 0u;
+        for (int q = tid; q < WL; q += NT) X[((i64)i * BT + k) * WL + q] = 0u;
     // row k + 1 into buffer 0 (entries j = k..k, at slot j)
     if (k + 1 < BT)
-        for (int idx = tid; idx < WL; idx += TT) LR[(i64)k * WL + idx] = W[((i64)(k + 1) * BT + k) * WL + idx];
+        for (int idx = tid; idx < WL; idx += NT) LR[(i64)k * WL + idx] = W[((i64)(k + 1) * BT + k) * WL + idx];
     __syncthreads();
     if (wave == 0 && lane < WL) {       // x_kk = r_k
         const u32 r = RI[k * WL + lane];
@@ -826,12 +828,12 @@ __global__ void __launch_bounds__(TT) k_tile_inverse(const u32* __restrict__ Wg,
         const u32* src = W + ((i64)(i + 1) * BT + k) * WL;
 #pragma unroll
         for (int u = 0; u < INV_PRE; u++) {
-            const int idx = tid + u * TT;
+            const int idx = tid + u * NT;
             pre[u] = idx < nw ? src[idx] : 0u;
         }
         // this wave's products, summed exactly
         u32 acc = 0u;
-        for (int j = k + wave; j < i; j += TWAVES) {
+        for (int j = k + wave; j < i; j += NW) {
             const u32 lij = lane < WL ? cur[j * WL + lane] : 0u;
             const u32 xjk = lane < WL ? XT[j * WL + lane] : 0u;
             acc = lane_add(acc, lane_mulfx(lij, xjk, 0, WL, LF, PX, PY), 0u, WL);
@@ -839,13 +841,13 @@ __global__ void __launch_bounds__(TT) k_tile_inverse(const u32* __restrict__ Wg,
         if (lane < WL) PART[wave * WL + lane] = acc;
 #pragma unroll
         for (int u = 0; u < INV_PRE; u++) {
-            const int idx = tid + u * TT;
+            const int idx = tid + u * NT;
             if (idx < nw) nxt[k * WL + idx] = pre[u];
         }
         __syncthreads();
         if (wave == 0) {
             u32 sum = lane < WL ? PART[lane] : 0u;
-            for (int w = 1; w < TWAVES; w++) sum = lane_add(sum, lane < WL ? PART[w * WL + lane] : 0u, 0u, WL);
+            for (int w = 1; w < NW; w++) sum = lane_add(sum, lane < WL ? PART[w * WL + lane] : 0u, 0u, WL);
             const u32 x = lane_neg(lane_mulfx(sum, lane < WL ? RI[i * WL + lane] : 0u, 0, WL, LF, PX, PY), WL);
             if (lane < WL) {
                 X[((i64)i * BT + k) * WL + lane] = x;
@@ -931,11 +933,23 @@ inline int tile_lane_pairs() {
     return v;
 }
 
-// threads of the LDS-resident factorisation: 512 unless SDPV_TILE_THREADS=256
+// threads per (tile, column) of the inverse: 512 unless SDPV_INV_THREADS=256
+inline int inverse_threads() {
+    static const int v = [] {
+        const char* e = getenv("SDPV_INV_THREADS");
+        return (e && atoi(e) == 256) ? 256 : 512;
+    }();
+    return v;
+}
+
+// threads of the LDS-resident factorisation: 1024 unless SDPV_TILE_THREADS is 256 or 512. Measured
+// at L = 24: chol(Q) of N = 3000 in 0.241 s with 256 threads, 0.203 s with 512 and 0.197 s with 1024
+// (all with the 512-thread inverse; 0.210 s with 512 and the 256-thread inverse).
 inline int tile_threads() {
     static const int v = [] {
         const char* e = getenv("SDPV_TILE_THREADS");
-        return (e && atoi(e) == 256) ? 256 : 512;
+        const int t = e ? atoi(e) : 1024;
+        return (t == 256 || t == 512) ? t : 1024;
     }();
     return v;
 }
@@ -967,22 +981,32 @@ int sdpv_tile_chol(const int32_t* A, int64_t L, int64_t nb, int64_t BT, int32_t*
                                   160 * 1024);
         (void)hipFuncSetAttribute((const void*)k_tile_chol_lds<512>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                   160 * 1024);
+        (void)hipFuncSetAttribute((const void*)k_tile_chol_lds<1024>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  160 * 1024);
         attr = true;
     }
     const int nt = tile_threads();
     const size_t lds_f = (size_t)lds_factor_words((int)BT, WL, nt / 64) * 4;
-    const size_t lds_i = (size_t)lds_inverse_words((int)BT, WL) * 4;
-    if (WL <= 64 && lds_f <= 160 * 1024 && BT * WL <= INV_PRE * TT) {
+    const int ni = inverse_threads();
+    const size_t lds_i = (size_t)lds_inverse_words((int)BT, WL, ni / 64) * 4;
+    if (WL <= 64 && lds_f <= 160 * 1024 && BT * WL <= INV_PRE * 256) {
         // the tile stays in LDS; the inverse one workgroup per column
-        if (nt == 512)
+        if (nt == 1024)
+            hipLaunchKernelGGL(k_tile_chol_lds<1024>, dim3((unsigned)nb), dim3(1024), lds_f, st, W, RI, (int)BT, WL,
+                               (int)L, tile_lane_pairs());
+        else if (nt == 512)
             hipLaunchKernelGGL(k_tile_chol_lds<512>, dim3((unsigned)nb), dim3(512), lds_f, st, W, RI, (int)BT, WL,
                                (int)L, tile_lane_pairs());
         else
             hipLaunchKernelGGL(k_tile_chol_lds<256>, dim3((unsigned)nb), dim3(256), lds_f, st, W, RI, (int)BT, WL,
                                (int)L, tile_lane_pairs());
         SDPV_TRY_LAUNCH();
-        hipLaunchKernelGGL(k_tile_inverse, dim3((unsigned)nb, (unsigned)BT), dim3(TT), lds_i, st, W, RI, X, (int)BT,
-                           WL, (int)L);
+        if (ni == 512)
+            hipLaunchKernelGGL(k_tile_inverse<512>, dim3((unsigned)nb, (unsigned)BT), dim3(512), lds_i, st, W, RI, X,
+                               (int)BT, WL, (int)L);
+        else
+            hipLaunchKernelGGL(k_tile_inverse<256>, dim3((unsigned)nb, (unsigned)BT), dim3(256), lds_i, st, W, RI, X,
+                               (int)BT, WL, (int)L);
         SDPV_TRY_LAUNCH();
     } else {
         hipLaunchKernelGGL(k_tile_factor, dim3((unsigned)nb), dim3(TT), lds, st, W, X, S, RI, (int)BT, WL, (int)L);
