@@ -157,10 +157,10 @@ def bgemm_nt(A, Bt, out_exp="col", kc=4096, bt_rowexp=None):
 
 # products with at most NARROW right-hand-side columns and at most NARROW_MK rows times inner length
 # are formed directly (sdpv_exact_dot); larger ones go through residues, which are faster there
-# (one MI355X, L = 24, one column: direct 0.085 / 0.138 / 0.221 ms against residues 0.135 / 0.139 /
-# 0.140 ms at 32^2 / 94^2 / 188^2)
+# (one MI355X, L = 24, one column, M x K: direct 0.084 / 0.104 / 0.106 / 0.164 ms against residues
+# 0.135 / 0.134 / 0.132 / 0.135 ms at 32 x 32, 188 x 188, 32 x 1504, 375 x 375)
 NARROW = 2
-NARROW_MK = 16384
+NARROW_MK = 65536
 
 
 def _narrow(M, N, K, L):

@@ -6,7 +6,7 @@
 // gives, so the rounding that follows, and every result, is unchanged.
 //
 // Entry (b, i, j): S = sum_k A[b, i, k] B[b, j, k] over L-word two's-complement operands. One wave
-// takes one entry and one slice of at most KSL values of k; lane c (and c + 64) owns column c of
+// takes one entry and one slice of k (slice_len); lane c (and c + 64) owns column c of
 // the word products: it sums the unsigned products a_u b_v, u + v = c, as 32-bit halves, plus the
 // corrections that turn the unsigned reading of the two top words into the signed one
 // (a = a_unsigned - s_a 2^(32 L)). k runs in chunks of 64 staged in LDS (rows of 65 words, so that
@@ -20,7 +20,19 @@ namespace {
 constexpr int DW = 4;        // waves per workgroup
 constexpr int KC = 64;       // k per chunk
 constexpr int KS = KC + 1;   // LDS row stride in words
-constexpr int KSL = 256;     // k per slice
+constexpr int KSL = 256;     // longest slice of k
+
+// k per slice for E entries and inner dimension K: slices of at most KSL values; shorter ones
+// (multiples of the chunk, at least one chunk) while the launch would have fewer than about 1024
+// waves. The sums are exact, so the partition changes no result.
+__host__ inline i64 slice_len(i64 E, i64 K) {
+    if (K <= KC) return K > 0 ? K : 1;
+    i64 ns = (K + KSL - 1) / KSL;
+    const i64 want = (1024 + E - 1) / E, nmax = (K + KC - 1) / KC;
+    if (want > ns) ns = want < nmax ? want : nmax;
+    const i64 len = (K + ns - 1) / ns;
+    return (len + KC - 1) / KC * KC;
+}
 
 // column sums (lo, hi) of one entry over k0 <= k < k1, columns lane and lane + 64
 __device__ __forceinline__ void column_sums(const i32* a, i64 planeA, i64 ldAk, const i32* bv, i64 planeB, i64 ldBk,
@@ -120,7 +132,7 @@ __device__ __forceinline__ void finish(const i64* lo, const i64* hi, int L, int 
 __global__ void __launch_bounds__(64 * DW) k_exact_dot(const i32* __restrict__ A, i64 planeA, i64 ldAb, i64 ldAr,
                                                       i64 ldAk, const i32* __restrict__ B, i64 planeB, i64 ldBb,
                                                       i64 ldBr, i64 ldBk, int L, i64 M, i64 N, i64 K, i64 E,
-                                                      int nslices, i32* __restrict__ W, int NW,
+                                                      int nslices, i64 ksl, i32* __restrict__ W, int NW,
                                                       i64* __restrict__ bl, i64* __restrict__ Pl,
                                                       i64* __restrict__ Ph) {
     SDPV_DYNAMIC_LDS(u32, lds);
@@ -132,7 +144,7 @@ __global__ void __launch_bounds__(64 * DW) k_exact_dot(const i32* __restrict__ A
     u32* As = lds + (i64)wave * 2 * L * KS;
     u32* Bs = As + (i64)L * KS;
     i64 lo[2] = {0, 0}, hi[2] = {0, 0};
-    const i64 k0 = sl * KSL, k1 = (k0 + KSL < K) ? k0 + KSL : K;
+    const i64 k0 = sl * ksl, k1 = (k0 + ksl < K) ? k0 + ksl : K;
     column_sums(A + b * ldAb + i * ldAr, planeA, ldAk, B + b * ldBb + j * ldBr, planeB, ldBk, L,
                 nslices == 1 ? 0 : k0, nslices == 1 ? K : k1, As, Bs, lane, lo, hi);
     if (nslices == 1) {
@@ -175,8 +187,8 @@ extern "C" {
 
 int64_t sdpv_exact_dot_work(int64_t L, int64_t nb, int64_t M, int64_t N, int64_t K) {
     if (L < 1 || nb < 0 || M < 0 || N < 0 || K < 0) return SDPV_EARG;
-    const i64 ns = (K + KSL - 1) / KSL;
-    return ns <= 1 ? 0 : 2 * ns * nb * M * N * (2 * L + 1);
+    const i64 E = nb * M * N, ns = (K + slice_len(E, K) - 1) / slice_len(E, K);
+    return ns <= 1 ? 0 : 2 * ns * E * (2 * L + 1);
 }
 
 int sdpv_exact_dot(const int32_t* A, int64_t planeA, int64_t ldAb, int64_t ldAr, int64_t ldAk, const int32_t* B,
@@ -188,7 +200,7 @@ int sdpv_exact_dot(const int32_t* A, int64_t planeA, int64_t ldAb, int64_t ldAr,
     const i64 E = nb * M * N;
     if (E == 0) return SDPV_OK;
     if (!W || !bitlen || (K > 0 && (!A || !B))) return SDPV_EARG;
-    const i64 ns = K > KSL ? (K + KSL - 1) / KSL : 1;
+    const i64 ksl = slice_len(E, K), ns = K > 0 ? (K + ksl - 1) / ksl : 1;
     const i64 need = ns > 1 ? 2 * ns * E * (2 * L + 1) : 0;
     if (need > 0 && (!work || nwork < need)) return SDPV_EARG;
     const i64 grid = (E * ns + DW - 1) / DW;
@@ -204,7 +216,7 @@ int sdpv_exact_dot(const int32_t* A, int64_t planeA, int64_t ldAb, int64_t ldAr,
     i64* Ph = need ? work + need / 2 : nullptr;
     hipLaunchKernelGGL(k_exact_dot, dim3((unsigned)grid), dim3(64 * DW), lds, st, A, (i64)planeA, (i64)ldAb,
                        (i64)ldAr, (i64)ldAk, B, (i64)planeB, (i64)ldBb, (i64)ldBr, (i64)ldBk, (int)L, (i64)M, (i64)N,
-                       (i64)K, E, (int)ns, W, (int)NW, bitlen, Pl, Ph);
+                       (i64)K, E, (int)ns, ksl, W, (int)NW, bitlen, Pl, Ph);
     SDPV_TRY_LAUNCH();
     if (ns > 1) {
         hipLaunchKernelGGL(k_exact_finish, dim3((unsigned)((E + DW - 1) / DW)), dim3(64 * DW), 0, st, Pl, Ph, (int)L,
