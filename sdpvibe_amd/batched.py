@@ -88,17 +88,34 @@ def badd(A, B, alpha=1, out_exp=None):
 
 
 def _residue_desc(R, X, nb, rows, K):
-    """residues of a fixed-point (L, nb, rows, K) limb tensor, all primes; panels of rows j0..j0+jw"""
+    """residues of a fixed-point (L, nb, rows, K) limb tensor, all primes; panels of rows j0..j0+jw.
+    X is contiguous, or the per-block transpose of a contiguous (L, nb, K, rows) tensor (read in
+    place through its strides)."""
     L = X.shape[0]
+    if X.is_contiguous():
+        base, ldb, ldr, ldk = X, rows * K, K, 1
+    else:
+        base = X.transpose(2, 3)
+        if not base.is_contiguous():
+            raise ValueError("residue operands are contiguous or a contiguous tensor transposed per block")
+        ldb, ldr, ldk = rows * K, 1, rows
 
-    def f(k0, kc, kp, j0, jw, X=X):
+    def f(k0, kc, kp, j0, jw):
         r0, rn = (0, rows) if j0 is None else (j0, jw)
         Rh = torch.empty((R.n, kp // 16, nb * rn, 16), dtype=torch.int8, device=X.device)
         Rl = torch.empty_like(Rh)
-        # a panel is read in place: rows r0.. of every block, offset r0*K into each limb plane
-        R.split(X, None, None, L, nb * rows * K, nb, rn, rows * K, K, 1, k0, kc, kp, Rh, Rl, off=r0 * K)
+        # a panel is read in place: rows r0.. of every block, offset r0*ldr into each limb plane
+        R.split(base, None, None, L, nb * rows * K, nb, rn, ldb, ldr, ldk, k0, kc, kp, Rh, Rl, off=r0 * ldr)
         return Rh, Rl
     return f
+
+
+def _rows_operand(X):
+    """X as _residue_desc reads it: unchanged when contiguous or a per-block transpose of a
+    contiguous tensor, else a contiguous copy"""
+    if X.is_contiguous() or X.transpose(2, 3).is_contiguous():
+        return X
+    return X.contiguous()
 
 
 def bgemm_nt(A, Bt, out_exp="col", kc=4096, bt_rowexp=None):
@@ -114,9 +131,9 @@ def bgemm_nt(A, Bt, out_exp="col", kc=4096, bt_rowexp=None):
     eA = A.exp if isinstance(A.exp, int) else A.exp[:, 0]
     cache = {}
     if not _narrow(M, N, K, L):
-        Al = A.limbs.contiguous()
-        Bl = Bt.limbs.contiguous()
-        same = Al.data_ptr() == Bl.data_ptr()
+        Al = _rows_operand(A.limbs)
+        Bl = _rows_operand(Bt.limbs)
+        same = Al.data_ptr() == Bl.data_ptr() and Al.stride() == Bl.stride()
         fa = _residue_desc(R, Al, nb, M, K)
         fb = _residue_desc(R, Bl, nb, N, K)
 
@@ -219,7 +236,10 @@ def _bcrt_round_panel(R, acc, nb, M, N, j0, jw, e_in, out_exp, L, out, exps):
     ej = ein[:, j0:j0 + jw] + sj
     s = sj[:, None, :].expand(nb, M, jw).reshape(-1)
     exps[:, j0:j0 + jw] = ej
-    out[:, :, :, j0:j0 + jw] = shift_round(Wd, s, L).view(L, nb, M, jw)
+    if (j0, jw) == (0, N):                                          # the whole width: rounded in place
+        shift_round(Wd, s, L, out=out.view(L, nb * M * N))
+    else:
+        out[:, :, :, j0:j0 + jw] = shift_round(Wd, s, L).view(L, nb, M, jw)
 
 
 # ------------------------------------------------------------------ batched Cholesky / TRSM
@@ -326,8 +346,9 @@ def btrsm_right(B, Lf, tiles, bt=32, refine=1):
 
 
 def _as_rows(B):
-    """B^T per block as the row-exponent operand of bgemm_nt (its column exponents go separately)"""
-    return BMP(B.limbs.transpose(2, 3).contiguous(), 0)
+    """B^T per block as the row-exponent operand of bgemm_nt (its column exponents go separately): a
+    transposed view, which the residue split reads through its strides"""
+    return BMP(B.limbs.transpose(2, 3), 0)
 
 
 def _apply_tile(X, Lm, B, refine):

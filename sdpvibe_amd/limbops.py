@@ -30,13 +30,16 @@ def check_errors():
         raise OverflowError("a rounding step overflowed its limb budget")
 
 
-def shift_round(X, s, Lout, check=False):
+def shift_round(X, s, Lout, check=False, out=None):
     """X: (Lin, B) int32 limbs; s: (B,) int64 -> (Lout, B) = round_half_up(X / 2^s)
-    (limbops.py:173-185)."""
+    (limbops.py:173-185); into `out` when given (contiguous (Lout, B) int32, not overlapping X)."""
     Lin, B = X.shape
     X = X.contiguous()
     s = s.contiguous()
-    out = torch.empty((Lout, B), dtype=torch.int32, device=X.device)
+    if out is None:
+        out = torch.empty((Lout, B), dtype=torch.int32, device=X.device)
+    elif tuple(out.shape) != (Lout, B) or out.dtype != torch.int32 or not out.is_contiguous():
+        raise ValueError(f"shift_round: out must be contiguous int32 {(Lout, B)}")
     flag = torch.zeros((B,), dtype=torch.int32, device=X.device) if check else _errbuf()
     call("sdpv_shift_round", ptr(X), Lin, B, ptr(s), ptr(out), Lout, ptr(flag), int(check), stream())
     if check and bool(flag.any()):

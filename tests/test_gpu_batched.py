@@ -186,3 +186,26 @@ def test_column_panels_and_k_chunks_equal_one_pass(monkeypatch):
             assert torch.equal(C.exp, D.exp), mode
         else:
             assert C.exp == D.exp, mode
+
+
+@pytest.mark.parametrize("panels", [False, True])
+def test_transposed_view_operand_equals_contiguous(panels, monkeypatch):
+    """bgemm_nt with Bt given as the per-block transpose of a contiguous tensor (read through its
+    strides, as the triangular solves pass B^T) gives the bits of the contiguous copy; with column
+    panels forced, the panel offsets run through the strides too"""
+    from sdpvibe_amd import mpmat
+    L, nb, M, K, N = 6, 3, 40, 70, 150
+    rng = random.Random(41)
+    A = Bm.BMP(cu(_fx(rng, L, (nb, M, K), 32 * L - 40)), -(32 * L - 40))
+    B = cu(_fx(rng, L, (nb, K, N), 32 * L - 40))                     # (L, nb, K, N): Bt = B^T per block
+    rowexp = torch.from_numpy(np.array([[rng.randrange(-3000, -2000) for _ in range(N)] for _ in range(nb)],
+                                       np.int64)).cuda()
+    if panels:
+        R = mpmat.ctx(L)
+        monkeypatch.setattr(mpmat, "ACC_BUDGET", R.n * nb * M * 2 * 64)
+    for mode in ("col", "mat", -(64 * L)):
+        view = Bm.bgemm_nt(A, Bm.BMP(B.transpose(2, 3), 0), out_exp=mode, bt_rowexp=rowexp)
+        copy = Bm.bgemm_nt(A, Bm.BMP(B.transpose(2, 3).contiguous(), 0), out_exp=mode, bt_rowexp=rowexp)
+        assert torch.equal(view.limbs, copy.limbs), mode
+        if torch.is_tensor(copy.exp):
+            assert torch.equal(view.exp, copy.exp), mode
