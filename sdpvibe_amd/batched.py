@@ -89,33 +89,33 @@ def badd(A, B, alpha=1, out_exp=None):
 
 def _residue_desc(R, X, nb, rows, K):
     """residues of a fixed-point (L, nb, rows, K) limb tensor, all primes; panels of rows j0..j0+jw.
-    X is contiguous, or the per-block transpose of a contiguous (L, nb, K, rows) tensor (read in
-    place through its strides)."""
+    X may be any view with non-negative strides (a transpose, a row or column slice): the split
+    reads it in place through its strides."""
     L = X.shape[0]
     if X.is_contiguous():
-        base, ldb, ldr, ldk = X, rows * K, K, 1
+        base, plane, ldb, ldr, ldk = X, nb * rows * K, rows * K, K, 1
     else:
-        base = X.transpose(2, 3)
-        if not base.is_contiguous():
-            raise ValueError("residue operands are contiguous or a contiguous tensor transposed per block")
-        ldb, ldr, ldk = rows * K, 1, rows
+        plane, ldb, ldr, ldk = X.stride()
+        if min(plane, ldb, ldr, ldk) < 0:
+            X = X.contiguous()
+            base, plane, ldb, ldr, ldk = X, nb * rows * K, rows * K, K, 1
+        else:                                # the storage from X's first entry to its last, flat
+            extent = (L - 1) * plane + (nb - 1) * ldb + (rows - 1) * ldr + (K - 1) * ldk + 1
+            base = torch.as_strided(X, (extent,), (1,))
 
     def f(k0, kc, kp, j0, jw):
         r0, rn = (0, rows) if j0 is None else (j0, jw)
         Rh = torch.empty((R.n, kp // 16, nb * rn, 16), dtype=torch.int8, device=X.device)
         Rl = torch.empty_like(Rh)
         # a panel is read in place: rows r0.. of every block, offset r0*ldr into each limb plane
-        R.split(base, None, None, L, nb * rows * K, nb, rn, ldb, ldr, ldk, k0, kc, kp, Rh, Rl, off=r0 * ldr)
+        R.split(base, None, None, L, plane, nb, rn, ldb, ldr, ldk, k0, kc, kp, Rh, Rl, off=r0 * ldr)
         return Rh, Rl
     return f
 
 
-def _rows_operand(X):
-    """X as _residue_desc reads it: unchanged when contiguous or a per-block transpose of a
-    contiguous tensor, else a contiguous copy"""
-    if X.is_contiguous() or X.transpose(2, 3).is_contiguous():
-        return X
-    return X.contiguous()
+def _rows(B, r0, r1=None):
+    """rows r0..r1 of every block as a view (the exponents are per column, shared)"""
+    return BMP(B.limbs[:, :, r0:r1, :], B.exp)
 
 
 def bgemm_nt(A, Bt, out_exp="col", kc=4096, bt_rowexp=None):
@@ -131,9 +131,8 @@ def bgemm_nt(A, Bt, out_exp="col", kc=4096, bt_rowexp=None):
     eA = A.exp if isinstance(A.exp, int) else A.exp[:, 0]
     cache = {}
     if not _narrow(M, N, K, L):
-        Al = _rows_operand(A.limbs)
-        Bl = _rows_operand(Bt.limbs)
-        same = Al.data_ptr() == Bl.data_ptr() and Al.stride() == Bl.stride()
+        Al, Bl = A.limbs, Bt.limbs                  # views are read through their strides
+        same = Al.data_ptr() == Bl.data_ptr() and Al.stride() == Bl.stride() and Al.shape == Bl.shape
         fa = _residue_desc(R, Al, nb, M, K)
         fb = _residue_desc(R, Bl, nb, N, K)
 
@@ -385,14 +384,14 @@ def _left_solve(node, Lf, B, tiles, refine, upper):
     k = node.k
     head, tail = slice(None, k), slice(k, None)
     if not upper:
-        P1 = _left_solve(node.head, _view(Lf, head, head), B[:k, :], tiles, refine, upper)
+        P1 = _left_solve(node.head, _view(Lf, head, head), _rows(B, 0, k), tiles, refine, upper)
         coupled = bgemm_nt(_view(Lf, tail, head), _as_rows(P1), out_exp="col", bt_rowexp=P1.expt())
-        P2 = _left_solve(node.tail, _view(Lf, tail, tail), badd(B[k:, :], coupled, -1, out_exp="align"), tiles,
+        P2 = _left_solve(node.tail, _view(Lf, tail, tail), badd(_rows(B, k), coupled, -1, out_exp="align"), tiles,
                          refine, upper)
     else:
-        P2 = _left_solve(node.tail, _view(Lf, tail, tail), B[k:, :], tiles, refine, upper)
+        P2 = _left_solve(node.tail, _view(Lf, tail, tail), _rows(B, k), tiles, refine, upper)
         coupled = bgemm_nt(_view(Lf, tail, head, transpose=True), _as_rows(P2), out_exp="col", bt_rowexp=P2.expt())
-        P1 = _left_solve(node.head, _view(Lf, head, head), badd(B[:k, :], coupled, -1, out_exp="align"), tiles,
+        P1 = _left_solve(node.head, _view(Lf, head, head), badd(_rows(B, 0, k), coupled, -1, out_exp="align"), tiles,
                          refine, upper)
     return _stack_common(P1, P2)
 

@@ -227,7 +227,7 @@ class RNS:
                                  f"got {tuple(Rx.shape)}")
         if nrows and kc:
             last = off + (nb - 1) * ldb + (rows - 1) * ldr + (k0 + kc - 1) * ldk
-            if min(ldb, ldr, ldk, k0, off) < 0 or last >= plane or X.numel() < Lin * plane:
+            if min(ldb, ldr, ldk, k0, off) < 0 or last >= plane or (Lin - 1) * plane + last >= X.numel():
                 raise ValueError("split: entry offsets exceed the limb planes")
             if e is not None and (e.numel() < plane or Erow is None or Erow.numel() < nrows):
                 raise ValueError("split: exponent arrays too small")
