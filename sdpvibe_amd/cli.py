@@ -271,6 +271,8 @@ class Run:
         sdp = self.sdp
         self.out.say(f"Initialize SDP solver\n\tprimal dimension: {sum(g.nb * g.P for g in sdp.groups)}"
                      f"\n\tdual dimension: {sdp.N}\n\tSDP blocks: {sdp.nblocks}")
+        if getattr(sdp, "read_info", None):
+            self.out.say(f"\t{sdp.read_info}", level=2)
         if "z" in p["writeSolutionParts"] and sdp.normalization is None:
             return "Please provide SDP with valid normalization.json or exclude z from --writeSolution arguments."
         self.solver = Solver(sdp, self.L, precision=p["precision"], **solver_params(p))

@@ -16,6 +16,7 @@ per-block form). The even and odd parities of a group are stacked in one batch o
 same rows in the same order and the two layouts give bitwise identical iterates.
 """
 import math
+import time
 
 import mpmath as mp
 import numpy as np
@@ -116,6 +117,7 @@ class SDP:
         from .layout import Layout
         from .sdp_io import SdpSource, read_block, read_normalization, read_objectives
         from .store import Store
+        t0 = time.time()
         src = SdpSource(path)
         nblocks = src.json("control.json")["num_blocks"]
         const, b_strs = read_objectives(src)
@@ -128,6 +130,7 @@ class SDP:
             shapes.append((r["dim"], r["K"], max(he, ho)))
             hs.append((he, ho))
         mp.mp.prec = F.NBITS(L) + 64
+        t1 = time.time()
         self = cls._empty(L, N, mp.mpf(const), layout)
         if layout == "plain":
             self._plan(shapes)
@@ -156,6 +159,8 @@ class SDP:
                 raw[j] = None
         self._finish(F.from_strings(b_strs, L, (N, 1)))
         self.normalization = read_normalization(src)
+        plan = self.layout_plan.describe() if self.layout_plan is not None else f"{len(self.groups)} groups, resident"
+        self.read_info = f"read {t1 - t0:.1f}s, assemble {time.time() - t1:.1f}s; {plan}"
         return self
 
     # ---------------------------------------------------------------- construction
@@ -167,6 +172,7 @@ class SDP:
         self.normalization = None
         self.store = None
         self.layout_plan = None
+        self.read_info = ""
         return self
 
     def _plan(self, shapes):

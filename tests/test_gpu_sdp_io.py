@@ -82,6 +82,7 @@ def test_streamed_load_iterates_like_the_resident_load(tmp_path, monkeypatch, st
     assert (placed.store is not None) == (stream == "all")
     if stream == "all":
         assert all(g.B is None for g in placed.groups)
+    assert placed.read_info.endswith(placed.layout_plan.describe()) and "chunks" in placed.read_info
     A, B = Solver(base, L), Solver(placed, L)
     A.run(max_iter=4)
     B.run(max_iter=4)
