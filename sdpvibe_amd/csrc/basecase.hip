@@ -933,11 +933,12 @@ inline int tile_lane_pairs() {
     return v;
 }
 
-// threads per (tile, column) of the inverse: 512 unless SDPV_INV_THREADS=256
+// threads per (tile, column) of the inverse: 512 unless SDPV_INV_THREADS is 256 or 1024
 inline int inverse_threads() {
     static const int v = [] {
         const char* e = getenv("SDPV_INV_THREADS");
-        return (e && atoi(e) == 256) ? 256 : 512;
+        const int t = e ? atoi(e) : 512;
+        return (t == 256 || t == 1024) ? t : 512;
     }();
     return v;
 }
@@ -1001,7 +1002,10 @@ int sdpv_tile_chol(const int32_t* A, int64_t L, int64_t nb, int64_t BT, int32_t*
             hipLaunchKernelGGL(k_tile_chol_lds<256>, dim3((unsigned)nb), dim3(256), lds_f, st, W, RI, (int)BT, WL,
                                (int)L, tile_lane_pairs());
         SDPV_TRY_LAUNCH();
-        if (ni == 512)
+        if (ni == 1024)
+            hipLaunchKernelGGL(k_tile_inverse<1024>, dim3((unsigned)nb, (unsigned)BT), dim3(1024), lds_i, st, W, RI,
+                               X, (int)BT, WL, (int)L);
+        else if (ni == 512)
             hipLaunchKernelGGL(k_tile_inverse<512>, dim3((unsigned)nb, (unsigned)BT), dim3(512), lds_i, st, W, RI, X,
                                (int)BT, WL, (int)L);
         else
