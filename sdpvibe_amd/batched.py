@@ -118,11 +118,12 @@ def _rows(B, r0, r1=None):
     return BMP(B.limbs[:, :, r0:r1, :], B.exp)
 
 
-def bgemm_nt(A, Bt, out_exp="col", kc=4096, bt_rowexp=None):
+def bgemm_nt(A, Bt, out_exp="col", kc=4096, bt_rowexp=None, bt_cache=None):
     """C[b] = A[b] @ Bt[b]^T for all b (exact, then rounded). A: uniform exponent per block (int or
     (nb, C) tensor read at column 0). Bt: int exponent or per-row exponents bt_rowexp (nb, N).
-    out_exp: 'col' (dynamic per (block, column)), 'mat' (dynamic per block), or int (fixed)
-    (batched.py:165-197)."""
+    out_exp: 'col' (dynamic per (block, column)), 'mat' (dynamic per block), or int (fixed).
+    bt_cache: a dict that keeps Bt's residues between calls with the same Bt (its limbs must not
+    change while the dict is in use); the residues depend on the limbs only (batched.py:165-197)."""
     nb, M, K = A.shape
     nb2, N, K2 = Bt.shape
     assert nb == nb2 and K == K2
@@ -143,7 +144,12 @@ def bgemm_nt(A, Bt, out_exp="col", kc=4096, bt_rowexp=None):
     def fb0(k0, kc_, kp, j0, jw):
         if same and (j0, jw) == (0, N):
             return cache["a"]
-        return fb(k0, kc_, kp, j0, jw)
+        if bt_cache is None:
+            return fb(k0, kc_, kp, j0, jw)
+        key = (L, k0, kc_, kp, j0, jw)
+        if key not in bt_cache:
+            bt_cache[key] = fb(k0, kc_, kp, j0, jw)
+        return bt_cache[key]
 
     eB = Bt.exp if bt_rowexp is None else bt_rowexp
     if isinstance(eA, int) and isinstance(eB, int):
@@ -244,8 +250,9 @@ def _bcrt_round_panel(R, acc, nb, M, N, j0, jw, e_in, out_exp, L, out, exps):
 # ------------------------------------------------------------------ batched Cholesky / TRSM
 class BTile:
     """a factored diagonal tile: Lmp, the factor (fixed point), and X, its explicit inverse (one
-    exponent per block)"""
-    __slots__ = ("X", "Lmp")
+    exponent per block); res: residues of X and Lmp as right operands, kept while the factorisation
+    that made the tile reuses them (None otherwise)"""
+    __slots__ = ("X", "Lmp", "res")
 
 
 def bchol_tile(A, LI=24, threads=256):
@@ -261,6 +268,7 @@ def bchol_tile(A, LI=24, threads=256):
     Lw, Xw, _ = tile_factor(A.limbs, b)
     top = bitlen_words(Xw.reshape(L + LI, -1)).view(nb, b * b).amax(1) - Fb
     tile = BTile()
+    tile.res = None
     tile.Lmp = BMP(Lw, A.exp)
     wide = BMP(Xw, torch.full((nb, b), -Fb, dtype=torch.int64, device=Xw.device))
     tile.X = brescale(wide, (top - headroom_bits(L)).unsqueeze(1).expand(nb, b).contiguous(), L)
@@ -292,6 +300,12 @@ def _tree(n, bt):
     return _Node(0, n, bt, 0)
 
 
+# a factorisation keeps the residues of its tiles' X and factor, which the panel solves below them
+# use at every level of the split tree (331 leaf solves over 94 tiles at n = 3008), when they take
+# at most this many bytes
+TILE_RES_BUDGET = 1 << 29
+
+
 def bchol(A, bt=32, refine=1):
     """Cholesky factors of nb SPD equilibrated fixed-point blocks (nb, n, n), n a multiple of bt:
     (L, list of factored diagonal tiles in tree order). Each internal node factors its head, solves
@@ -301,6 +315,8 @@ def bchol(A, bt=32, refine=1):
     e = A.exp
     out = torch.zeros((A.L, nb, n, n), dtype=torch.int32, device=A.limbs.device)
     tiles = []
+    R = ctx(A.L)
+    keep_res = nb * (n // bt) * 2 * 2 * R.n * R.kp(bt) * bt <= TILE_RES_BUDGET
 
     def factor(node, M):
         lo = node.lo
@@ -308,6 +324,8 @@ def bchol(A, bt=32, refine=1):
             if node.n != bt:
                 raise ValueError("the dimension must be a multiple of the tile size")
             Lt, t = bchol_tile(M)
+            if keep_res:
+                t.res = {"X": {}, "L": {}}
             tiles.append(t)
             out[:, :, lo:lo + bt, lo:lo + bt] = Lt.limbs
             return
@@ -319,6 +337,8 @@ def bchol(A, bt=32, refine=1):
         factor(node.tail, badd(M[k:, k:], bgemm_nt(panel, panel, out_exp=e), -1, out_exp=e))
 
     factor(_tree(n, bt), A)
+    for t in tiles:
+        t.res = None
     return BMP(out, e), tiles
 
 
@@ -327,10 +347,11 @@ def _right_solve(node, B, Lf, tiles, refine, base):
     e = Lf.exp
     if node.head is None:
         t = tiles[node.first]
-        X = bgemm_nt(B, t.X, out_exp=e, bt_rowexp=t.X.exp)
+        cx, cl = (None, None) if t.res is None else (t.res["X"], t.res["L"])
+        X = bgemm_nt(B, t.X, out_exp=e, bt_rowexp=t.X.exp, bt_cache=cx)
         for _ in range(refine):                              # one residual correction per step
-            resid = badd(B, bgemm_nt(X, t.Lmp, out_exp=e), -1, out_exp=e)
-            X = badd(X, bgemm_nt(resid, t.X, out_exp=e, bt_rowexp=t.X.exp), 1, out_exp=e)
+            resid = badd(B, bgemm_nt(X, t.Lmp, out_exp=e, bt_cache=cl), -1, out_exp=e)
+            X = badd(X, bgemm_nt(resid, t.X, out_exp=e, bt_rowexp=t.X.exp, bt_cache=cx), 1, out_exp=e)
         return X
     k = node.k
     X1 = _right_solve(node.head, B[:, :k], Lf[:k, :k], tiles, refine, base)

@@ -70,3 +70,36 @@ def test_chol_not_pd():
     m, e = mpf_host(vals, L)
     with pytest.raises(ValueError):
         F.Chol(F.MPF(cu(m.reshape(L, n, n)), cu(e.reshape(n, n))))
+
+
+def test_tile_residues_kept_through_the_factorisation(monkeypatch):
+    """the factorisation reuses the residues of its tiles' X and factor in the panel solves: the
+    factor and tiles are those of a run that splits every tile operand again, with fewer splits,
+    and no tile keeps residues afterwards"""
+    from sdpvibe_amd import batched as Bm
+    L, nb, n = 8, 2, 200
+    m, e, _ = spd_blocks(nb, n, L, seed=17, uspread=12)
+    A = F.MPF(cu(m), cu(e))
+    calls = [0]
+    desc = Bm._residue_desc
+
+    def counted(*a):
+        f = desc(*a)
+
+        def g(*b):
+            calls[0] += 1
+            return f(*b)
+        return g
+    monkeypatch.setattr(Bm, "_residue_desc", counted)
+    runs = []
+    for budget in (Bm.TILE_RES_BUDGET, 0):
+        monkeypatch.setattr(Bm, "TILE_RES_BUDGET", budget)
+        calls[0] = 0
+        C = F.Chol(A)
+        runs.append((C, calls[0]))
+        assert all(t.res is None for t in C.tiles)
+    (C1, n1), (C0, n0) = runs
+    assert n1 < n0, (n1, n0)
+    assert torch.equal(C1.Lt.limbs, C0.Lt.limbs) and torch.equal(C1.k, C0.k)
+    for t1, t0 in zip(C1.tiles, C0.tiles):
+        assert torch.equal(t1.X.limbs, t0.X.limbs) and torch.equal(t1.Lmp.limbs, t0.Lmp.limbs)
