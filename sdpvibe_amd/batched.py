@@ -152,6 +152,14 @@ def bgemm_nt(A, Bt, out_exp="col", kc=4096, bt_rowexp=None, bt_cache=None):
         return bt_cache[key]
 
     eB = Bt.exp if bt_rowexp is None else bt_rowexp
+    out = torch.empty((L, nb, M, N), dtype=torch.int32, device=dev)
+    if isinstance(out_exp, int) and not _narrow(M, N, K, L):
+        # a fixed output exponent: the reconstruction rounds by per-(block, column) shifts alone
+        s = _fixed_shift(eA, eB, out_exp, nb, N, bt_cache)
+        for j0, jw, acc in residue_gemm(R, fa0, fb0, nb, M, N, K):
+            R.crt(acc, nb * M, jw, 0, jw, 2, outm=out, L=L, plane=nb * M * N, ldo=N, jo=j0, Mb=M, scol=s,
+                  flag=_errbuf(), scaled=True)
+        return BMP(out, out_exp)
     if isinstance(eA, int) and isinstance(eB, int):
         e_in = eA + eB
     else:
@@ -160,16 +168,14 @@ def bgemm_nt(A, Bt, out_exp="col", kc=4096, bt_rowexp=None, bt_cache=None):
         if eB_t.dim() == 1:
             eB_t = eB_t[:, None].expand(nb, N)
         e_in = (eA_t[:, None] + eB_t).contiguous()
-    out = torch.empty((L, nb, M, N), dtype=torch.int32, device=dev)
-    fixed = isinstance(e_in, int) and isinstance(out_exp, int)
-    exps = None if fixed else torch.empty((nb, N), dtype=torch.int64, device=dev)
+    exps = None if isinstance(out_exp, int) else torch.empty((nb, N), dtype=torch.int64, device=dev)
     if _narrow(M, N, K, L):
         W, bl = _exact_narrow(A.limbs, Bt.limbs, nb, M, N, K, L)
         _round_exact(W, bl, nb, M, N, e_in, out_exp, L, out, exps)
     else:
         for j0, jw, acc in residue_gemm(R, fa0, fb0, nb, M, N, K):
             _bcrt_round_panel(R, acc, nb, M, N, j0, jw, e_in, out_exp, L, out, exps)
-    if fixed or isinstance(out_exp, int):
+    if isinstance(out_exp, int):
         return BMP(out, out_exp)
     C = BMP(out, exps)
     if out_exp == "mat":
@@ -214,7 +220,6 @@ def _round_exact(W, bl, nb, M, N, e_in, out_exp, L, out, exps):
             s = torch.full((nb * M * N,), out_exp - e_in, dtype=torch.int64, device=dev)
         else:
             s = (out_exp - e_in)[:, None, :].expand(nb, M, N).reshape(-1).contiguous()
-            exps[:, :] = out_exp
     else:
         ein = e_in if torch.is_tensor(e_in) else torch.full((nb, N), e_in, dtype=torch.int64, device=dev)
         sj = bl.view(nb, M, N).amax(1) - headroom_bits(L)
@@ -223,18 +228,30 @@ def _round_exact(W, bl, nb, M, N, e_in, out_exp, L, out, exps):
     out.copy_(shift_round(W, s, L).view(L, nb, M, N))
 
 
+def _fixed_shift(eA, eB, out_exp, nb, N, cache=None):
+    """out_exp - (eA + eB) per (block, column), (nb * N,) int64: the rounding shifts of a product
+    with a fixed output exponent. eA: int or (nb,); eB: int, (nb,) or (nb, N). Kept in `cache`
+    when eA is an int (the cache of a Bt whose exponents do not change)"""
+    key = ("shift", eA, out_exp) if cache is not None and isinstance(eA, int) else None
+    if key is not None and key in cache:
+        return cache[key]
+    if isinstance(eA, int) and isinstance(eB, int):
+        s = torch.full((nb * N,), out_exp - (eA + eB), dtype=torch.int64, device=dev)
+    else:
+        eA_t = eA if torch.is_tensor(eA) else torch.full((nb,), eA, dtype=torch.int64, device=dev)
+        eB_t = eB if torch.is_tensor(eB) else torch.full((nb, N), eB, dtype=torch.int64, device=dev)
+        if eB_t.dim() == 1:
+            eB_t = eB_t[:, None].expand(nb, N)
+        s = (out_exp - (eA_t[:, None] + eB_t)).reshape(-1).contiguous()
+    if key is not None:
+        cache[key] = s
+    return s
+
+
 def _bcrt_round_panel(R, acc, nb, M, N, j0, jw, e_in, out_exp, L, out, exps):
-    """_bcrt_round (batched.py:200-231) of one column panel: exact products -> L limbs"""
+    """_bcrt_round (batched.py:200-231) of one column panel with dynamic output exponents: exact
+    products -> L limbs (a fixed output exponent is rounded in bgemm_nt's reconstruction)"""
     rows = nb * M
-    if isinstance(out_exp, int):
-        if isinstance(e_in, int):
-            s = torch.full((nb * N,), out_exp - e_in, dtype=torch.int64, device=dev)
-        else:
-            s = (out_exp - e_in).reshape(-1).contiguous()
-            exps[:, j0:j0 + jw] = out_exp
-        R.crt(acc, rows, jw, 0, jw, 2, outm=out, L=L, plane=nb * M * N, ldo=N, jo=j0, Mb=M, scol=s, flag=_errbuf(),
-              scaled=True)
-        return
     Wd, bl, Bq = R.crt_block(acc, jw, 0, jw, rows, True)               # entries (b*M + i)*jw + jj
     ein = e_in if torch.is_tensor(e_in) else torch.full((nb, N), e_in, dtype=torch.int64, device=dev)
     sj = bl.view(nb, M, jw).amax(1) - headroom_bits(L)             # (nb, jw)
