@@ -347,11 +347,12 @@ def bchol(A, bt=32, refine=1):
             out[:, :, lo:lo + bt, lo:lo + bt] = Lt.limbs
             return
         k, hi = node.k, lo + node.n
-        factor(node.head, M[:k, :k])
-        L11 = BMP(out[:, :, lo:lo + k, lo:lo + k].contiguous(), e)
-        panel = _right_solve(node.head, M[k:, :k], L11, tiles, refine, lo)
+        head, tail = slice(None, k), slice(k, None)
+        factor(node.head, _view(M, head, head))
+        L11 = _view(BMP(out, e), slice(lo, lo + k), slice(lo, lo + k))
+        panel = _right_solve(node.head, _view(M, tail, head), L11, tiles, refine, lo)
         out[:, :, lo + k:hi, lo:lo + k] = panel.limbs
-        factor(node.tail, badd(M[k:, k:], bgemm_nt(panel, panel, out_exp=e), -1, out_exp=e))
+        factor(node.tail, badd(_view(M, tail, tail), bgemm_nt(panel, panel, out_exp=e), -1, out_exp=e))
 
     factor(_tree(n, bt), A)
     for t in tiles:
@@ -371,9 +372,10 @@ def _right_solve(node, B, Lf, tiles, refine, base):
             X = badd(X, bgemm_nt(resid, t.X, out_exp=e, bt_rowexp=t.X.exp, bt_cache=cx), 1, out_exp=e)
         return X
     k = node.k
-    X1 = _right_solve(node.head, B[:, :k], Lf[:k, :k], tiles, refine, base)
-    rest = badd(B[:, k:], bgemm_nt(X1, Lf[k:, :k], out_exp=e), -1, out_exp=e)
-    X2 = _right_solve(node.tail, rest, Lf[k:, k:], tiles, refine, base)
+    head, tail, every = slice(None, k), slice(k, None), slice(None)
+    X1 = _right_solve(node.head, _view(B, every, head), _view(Lf, head, head), tiles, refine, base)
+    rest = badd(_view(B, every, tail), bgemm_nt(X1, _view(Lf, tail, head), out_exp=e), -1, out_exp=e)
+    X2 = _right_solve(node.tail, rest, _view(Lf, tail, tail), tiles, refine, base)
     return BMP(torch.cat([X1.limbs, X2.limbs], 3), e)
 
 
@@ -405,8 +407,8 @@ def _stack_common(P1, P2):
 
 
 def _view(A, rows, cols, transpose=False):
-    """a sub-block of the uniform-exponent factor as a view (no copy); bgemm_nt copies what it
-    needs contiguous"""
+    """a sub-block of a uniform-exponent BMP as a view (no copy): bgemm_nt reads views through their
+    strides, badd and the tile factorisation copy what they need contiguous"""
     limbs = A.limbs[:, :, rows, cols]
     return BMP(limbs.transpose(2, 3) if transpose else limbs, A.exp)
 
