@@ -225,7 +225,7 @@ def _round_exact(W, bl, nb, M, N, e_in, out_exp, L, out, exps):
         sj = bl.view(nb, M, N).amax(1) - headroom_bits(L)
         exps[:, :] = ein + sj
         s = sj[:, None, :].expand(nb, M, N).reshape(-1).contiguous()
-    out.copy_(shift_round(W, s, L).view(L, nb, M, N))
+    shift_round(W, s, L, out=out.view(L, nb * M * N))
 
 
 def _fixed_shift(eA, eB, out_exp, nb, N, cache=None):
