@@ -54,8 +54,9 @@ class BMP:
         return self.limbs.reshape(self.L, -1)
 
 
-def brescale(A, new_exp, Lout=None):
-    """A re-rounded to new_exp (int, or (nb, C) per block and column), into Lout limbs"""
+def brescale(A, new_exp, Lout=None, share_exp=False):
+    """A re-rounded to new_exp (int, or (nb, C) per block and column), into Lout limbs; the result
+    holds a copy of a tensor new_exp, or new_exp itself with share_exp"""
     nb, R, C = A.shape
     Lo = Lout or A.L
     if isinstance(new_exp, int) and isinstance(A.exp, int):
@@ -64,7 +65,7 @@ def brescale(A, new_exp, Lout=None):
     if not torch.is_tensor(new_exp):
         new_exp = torch.full((nb, C), int(new_exp), dtype=torch.int64, device=A.limbs.device)
     shift = (new_exp - A.expt()).unsqueeze(1).expand(nb, R, C).reshape(-1)
-    return BMP(shift_round(A.flat(), shift, Lo).view(Lo, nb, R, C), new_exp.clone())
+    return BMP(shift_round(A.flat(), shift, Lo).view(Lo, nb, R, C), new_exp if share_exp else new_exp.clone())
 
 
 def badd(A, B, alpha=1, out_exp=None):
@@ -403,7 +404,7 @@ def _apply_tile(X, Lm, B, refine):
 def _stack_common(P1, P2):
     """[P1; P2] with one exponent per (block, column): both parts re-rounded to the larger"""
     ee = torch.maximum(P1.expt(), P2.expt())
-    return BMP(torch.cat([brescale(P1, ee).limbs, brescale(P2, ee).limbs], 2), ee)
+    return BMP(torch.cat([brescale(P1, ee, share_exp=True).limbs, brescale(P2, ee, share_exp=True).limbs], 2), ee)
 
 
 def _view(A, rows, cols, transpose=False):
